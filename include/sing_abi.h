@@ -1,0 +1,230 @@
+/* include/sing_abi.h — C ABI of the MI355X offline null-geodesic renderer.
+ *
+ * Drop-in boundary for the reference's CUDA offline path:
+ *   - render/include/render_backend.hpp:107-132   RenderBackend {initialize, shutdown,
+ *     resize, render_frame, capture_frame, name}
+ *   - render/cuda/cuda_backend.cu:27-28,144-148    singularity_geodesic_kernel(uint8_t*, Uniforms)
+ *     launched by CudaBackend::render_frame
+ *   - render/cuda/cuda_backend.cu:45-101           pack_uniforms_cuda(Scene, CameraState, w, h, t)
+ *   - cuda_cli/main.cpp:68-109                     build_orbital_camera(...)
+ *   - core/include/scene/scene_config.hpp:134-199  load_scene_config_from_{stream,file}
+ *
+ * Plain C types only: pointers, sizes, PODs.  Every entry point returns an int
+ * status (SING_OK = 0) and never aborts; the per-context message is available
+ * from sing_last_error().  Contexts own all device memory; host buffers passed
+ * to capture calls belong to the caller.  One context per host thread; calls on
+ * different contexts may run concurrently.
+ */
+#ifndef SING_ABI_H
+#define SING_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SING_ABI_VERSION 1
+
+/* --- the 160-byte uniform block, byte-for-byte shared_shader/uniforms.h:54-95 --- */
+typedef struct SingVec4 {
+    float x, y, z, w;
+} SingVec4;
+
+typedef struct SingUniforms {
+    SingVec4 cam_pos;   /*   0 */
+    SingVec4 cam_fwd;   /*  16 */
+    SingVec4 cam_right; /*  32 */
+    SingVec4 cam_up;    /*  48 */
+    float mass_M;       /*  64 */
+    float spin_a;       /*  68 */
+    float rs;           /*  72 */
+    float tan_half_fov; /*  76 */
+    float disc_r_inner; /*  80 */
+    float disc_r_outer; /*  84 */
+    float disc_peak_T;  /*  88 */
+    float aspect;       /*  92 */
+    float h_step;       /*  96 */
+    float escape_r;     /* 100 */
+    float horizon_cut;  /* 104 */
+    float time_sec;     /* 108 */
+    uint32_t width;     /* 112 */
+    uint32_t height;    /* 116 */
+    uint32_t metric_type; /* 120 */
+    uint32_t flags;     /* 124 */
+    uint32_t max_steps; /* 128 */
+    uint32_t frame_index; /* 132 */
+    uint32_t supersample; /* 136: samples per pixel, 1..1024 */
+    uint32_t pad_b;     /* 140 */
+    float exposure;     /* 144 */
+    float bloom_threshold; /* 148 */
+    float bloom_strength;  /* 152 */
+    float disc_turbulence; /* 156 */
+} SingUniforms;
+
+/* uniforms.h:41-52 */
+#define SING_FLAG_DOPPLER_ON (1u << 0)
+#define SING_FLAG_REDSHIFT_ON (1u << 1)
+#define SING_FLAG_DISC_ON (1u << 2)
+#define SING_FLAG_STARFIELD_ON (1u << 3)
+#define SING_FLAG_ADAPTIVE_STEP (1u << 4)
+#define SING_METRIC_SCHWARZSCHILD 0u
+#define SING_METRIC_KERR 1u
+
+/* Per-pixel termination class (class of the pixel's first sample). */
+#define SING_CLASS_HORIZON 0u  /* captured, including non-finite radius */
+#define SING_CLASS_ESCAPE 1u   /* r > escape_r: starfield */
+#define SING_CLASS_DISC 2u     /* crossed the equator inside the annulus */
+#define SING_CLASS_BUDGET 3u   /* max_steps exhausted */
+#define SING_CLASS_INITFAIL 4u /* Kerr null quadratic had no real root at the camera */
+
+/* --- status codes ------------------------------------------------------- */
+#define SING_OK 0
+#define SING_ERR_ARG 1    /* invalid argument (null pointer, size, range) */
+#define SING_ERR_HIP 2    /* HIP runtime error */
+#define SING_ERR_STATE 3  /* call not valid in the context's state */
+#define SING_ERR_NOMEM 4  /* device allocation failed */
+#define SING_ERR_IO 5     /* file could not be read or written */
+#define SING_ERR_PARSE 6  /* scene file parse error */
+#define SING_ERR_KEY 7    /* scene file unknown key */
+
+/* --- precision of the trace ---------------------------------------------- */
+#define SING_PREC_FP32 0u  /* reference arithmetic in FP32 */
+#define SING_PREC_FP64 1u  /* the same discrete algorithm in FP64 */
+#define SING_PREC_MIXED 2u /* FP32 bulk, FP64 for rays in the photon-sphere band */
+
+/* --- output selection (bit mask) ------------------------------------------ */
+#define SING_OUT_RGBA8 (1u << 0)   /* reference semantics: mean of per-sample ACES, sRGB, u8 */
+#define SING_OUT_HDR (1u << 1)     /* float RGB: mean of per-sample radiance (before exposure) */
+#define SING_OUT_CLASS (1u << 2)   /* u8 termination class of sample 0 */
+#define SING_OUT_STEPS (1u << 3)   /* u32 RK4 steps summed over the pixel's samples */
+#define SING_OUT_HDR16 (1u << 4)   /* half RGBA HDR (post-pass input) */
+
+/* --- post pass selection --------------------------------------------------- */
+#define SING_POST_NONE 0u
+#define SING_POST_BLOOM 1u /* half HDR -> extract -> 7-tap H/V -> composite+exposure -> ACES -> sRGB into RGBA8 */
+
+/* Options beyond the 160-byte block.  Zero-initialised = reference semantics,
+ * FP32, RGBA8 only. */
+typedef struct SingRenderOpts {
+    uint32_t precision;   /* SING_PREC_* */
+    uint32_t outputs;     /* SING_OUT_* mask; 0 means SING_OUT_RGBA8 */
+    uint32_t post;        /* SING_POST_* */
+    uint32_t turb_seed;   /* seeded turbulence band phases; 0 = reference bands */
+    float adapt_tol;      /* > 0: error-controlled RK4 (step doubling) with this per-step tolerance */
+    float band_width;     /* mixed precision: half-width of the photon-sphere band (0 = default) */
+    float peak_T_scale;   /* multiplies disc_peak_T; 0 is read as 1 */
+    uint32_t step_batch;  /* RK4 steps between lane-refill checks (0 = tuned default) */
+    uint32_t waves_per_cu;/* persistent grid size in waves per CU (0 = tuned default) */
+    uint32_t reserved[7];
+} SingRenderOpts;
+
+typedef struct SingRenderStats {
+    double kernel_ms;     /* trace kernel time, HIP events on the context stream */
+    double post_ms;       /* post-pass kernels time (0 when no post pass) */
+    uint64_t rays;        /* pixel samples traced */
+    uint64_t steps;       /* RK4 steps executed over all rays */
+    uint64_t fp64_rays;   /* rays traced in FP64 (mixed precision) */
+    uint64_t reserved[3];
+} SingRenderStats;
+
+typedef struct SingContext SingContext;
+
+/* --- lifecycle (RenderBackend::initialize/shutdown/resize) ---------------- */
+/* Creates a context for width x height on HIP device `device`; NULL on failure
+ * (reason from sing_last_error(NULL)). */
+SingContext* sing_create(uint32_t width, uint32_t height, int device);
+void sing_destroy(SingContext* ctx);
+int sing_resize(SingContext* ctx, uint32_t width, uint32_t height);
+/* Last error message for ctx (or the process-wide last error for ctx == NULL). */
+const char* sing_last_error(const SingContext* ctx);
+const char* sing_backend_name(void); /* "HIP-gfx950" (RenderBackend::name) */
+int sing_abi_version(void);
+
+/* --- render (RenderBackend::render_frame + the kernel launch) --------------
+ * Synchronous: returns after the frame is complete.  `u` is copied at call
+ * time.  u->width/height must equal the context size. */
+int sing_render(SingContext* ctx, const SingUniforms* u, const SingRenderOpts* opts, SingRenderStats* stats);
+
+/* --- capture (RenderBackend::capture_frame) ---------------------------------
+ * Copies the last frame's outputs into caller buffers of at least the stated
+ * byte size: RGBA8 W*H*4, HDR W*H*3*4, class W*H, steps W*H*4, HDR16 W*H*8. */
+int sing_capture_rgba8(SingContext* ctx, uint8_t* dst, size_t bytes);
+int sing_capture_hdr(SingContext* ctx, float* dst, size_t bytes);
+int sing_capture_class(SingContext* ctx, uint8_t* dst, size_t bytes);
+int sing_capture_steps(SingContext* ctx, uint32_t* dst, size_t bytes);
+int sing_capture_hdr16(SingContext* ctx, uint16_t* dst, size_t bytes);
+
+/* --- host-side scene ingestion (no GPU needed) ------------------------------ */
+/* render_backend.hpp:46-82 Scene (the fields cuda_cli uses) */
+typedef struct SingScene {
+    uint32_t metric;          /* SING_METRIC_* */
+    float mass;               /* mass_solar */
+    float spin_a_over_M;
+    float disc_inner_M;
+    float disc_outer_M;
+    uint32_t doppler_on;
+    uint32_t redshift_on;
+    uint32_t disc_on;
+    uint32_t supersample;     /* samples per pixel */
+    float exposure;
+    float bloom_threshold;
+    float bloom_strength;
+    float disc_turbulence;
+    float disc_peak_T_K;
+    uint32_t max_steps;       /* 0 = backend default (2500) */
+    float h_step;             /* 0 = backend default (0.4) */
+    float escape_r_M;         /* 0 = backend default (200 M) */
+    float horizon_cut_rs;     /* 0 = backend default (1.02 rs) */
+} SingScene;
+
+/* render_backend.hpp:84-99 CameraState */
+typedef struct SingCamera {
+    float position[3];
+    float basis[9];           /* rows: right, up, -forward */
+    float fov_y_radians;
+} SingCamera;
+
+/* scene_config.hpp:39-58 SceneConfig */
+typedef struct SingSceneConfig {
+    float mass;
+    float spin;
+    float disc_r_inner;       /* -1 = ISCO (the loader keeps the raw value) */
+    float disc_r_outer;
+    uint32_t disc_doppler_on;
+    uint32_t disc_redshift_on;
+    float camera_fov_deg;
+    float camera_distance;
+    float camera_elevation;
+    float h_step;
+    int32_t max_steps;
+} SingSceneConfig;
+
+/* Scene{} defaults (render_backend.hpp:52-74). */
+void sing_scene_defaults(SingScene* out);
+/* SceneConfig{} defaults (scene_config.hpp:39-58). */
+void sing_scene_config_defaults(SingSceneConfig* out);
+/* Parses `key = value` text (scene_config.hpp:134-191).  On failure returns
+ * SING_ERR_PARSE or SING_ERR_KEY, with *line (1-based) and the offending
+ * line/key copied into errbuf. */
+int sing_parse_scene_config(const char* text, SingSceneConfig* out, int* line, char* errbuf, size_t errbuf_len);
+/* scene_config.hpp:193-199; SING_ERR_IO when the file cannot be opened. */
+int sing_load_scene_config(const char* path, SingSceneConfig* out, int* line, char* errbuf, size_t errbuf_len);
+/* cuda_cli/main.cpp:193-217: SceneConfig -> Scene (metric = Kerr iff spin > 0). */
+void sing_scene_from_config(const SingSceneConfig* cfg, SingScene* scene);
+/* cuda_cli/main.cpp:68-109 */
+void sing_orbital_camera(float distance_M, float elevation_rad, float azimuth_rad, float fov_y_deg, SingCamera* out);
+/* cuda_backend.cu:45-101 */
+void sing_pack_uniforms(const SingScene* scene, const SingCamera* cam, uint32_t width, uint32_t height,
+                        float time_sec, SingUniforms* out);
+/* Kerr closed forms (core/include/physics/kerr.hpp:36-108), double precision:
+ * which = 0 r+, 1 r-, 2 ISCO prograde, 3 ISCO retrograde, 4 photon orbit prograde,
+ * 5 photon orbit retrograde. */
+double sing_kerr_radius(int which, double mass, double spin_a);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SING_ABI_H */

@@ -1,0 +1,194 @@
+"""oracle — CPU restatement of the reference renderer (test infrastructure).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this package, and only as the checker or the timed CPU baseline; the product
+(singularity_amd) never does.
+
+Libraries (built by oracle/Makefile):
+    liboracle.so       strict IEEE build (-O2 -fno-fast-math -ffp-contract=off)
+    liboracle_fast.so  reference Release flags (-O3 -ffast-math): CPU baseline
+    _ref/libref.so     the reference's own physics headers, built from
+                       /root/reference where that tree exists
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_LIBS: dict[str, C.CDLL] = {}
+
+
+class Extras(C.Structure):
+    _fields_ = [("soft_edge", C.c_double), ("band_phase1", C.c_double), ("band_phase2", C.c_double),
+                ("adapt_tol", C.c_double), ("peak_T_scale", C.c_double)]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def _load(name: str) -> C.CDLL:
+    if name not in _LIBS:
+        path = HERE / name
+        if not path.exists():
+            build()
+        L = C.CDLL(str(path))
+        if name != "_ref/libref.so":
+            _type_oracle(L)
+        _LIBS[name] = L
+    return _LIBS[name]
+
+
+def strict() -> C.CDLL:
+    return _load("liboracle.so")
+
+
+def fast() -> C.CDLL:
+    return _load("liboracle_fast.so")
+
+
+def ref_available() -> bool:
+    return (HERE / "_ref" / "libref.so").exists()
+
+
+def ref() -> C.CDLL:
+    return _load("_ref/libref.so")
+
+
+def _type_oracle(L: C.CDLL) -> None:
+    P = C.c_void_p
+    L.orc_render.restype = C.c_int
+    L.orc_render.argtypes = [P, C.c_int, P, P, C.c_int64, C.c_int, P, P, P, P, P, P]
+    L.orc_trace_ray.restype = C.c_int
+    L.orc_trace_ray.argtypes = [P, C.c_int, P, C.c_uint32, C.c_uint32, C.c_double, C.c_double, P]
+    L.orc_halton.restype = C.c_float
+    L.orc_halton.argtypes = [C.c_uint32, C.c_uint32]
+    L.orc_kerr_closed_form.restype = C.c_double
+    L.orc_kerr_closed_form.argtypes = [C.c_int, C.c_double, C.c_double]
+    L.orc_kerr_closed_form_f32.restype = C.c_float
+    L.orc_kerr_closed_form_f32.argtypes = [C.c_int, C.c_float, C.c_float]
+    L.orc_disc_temperature_f32.restype = C.c_float
+    L.orc_disc_temperature_f32.argtypes = [C.c_float, C.c_float, C.c_float]
+    L.orc_disc_temperature_f64.restype = C.c_double
+    L.orc_disc_temperature_f64.argtypes = [C.c_double, C.c_double, C.c_double]
+    L.orc_band_mask_f32.restype = C.c_float
+    L.orc_band_mask_f32.argtypes = [C.c_float] * 7
+    L.orc_blackbody_f32.argtypes = [C.c_float, P]
+    L.orc_blackbody_f64.argtypes = [C.c_double, P]
+    L.orc_starfield_f32.argtypes = [C.c_float, C.c_float, C.c_float, P]
+    L.orc_mix32.restype = C.c_uint32
+    L.orc_mix32.argtypes = [C.c_uint32, C.c_uint32]
+    L.orc_aces_f32.argtypes = [P, P]
+    L.orc_linear_to_srgb_f32.restype = C.c_float
+    L.orc_linear_to_srgb_f32.argtypes = [C.c_float]
+    L.orc_to_u8_f32.restype = C.c_uint8
+    L.orc_to_u8_f32.argtypes = [C.c_float]
+    L.orc_camera_ray_f32.argtypes = [P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, P]
+    L.orc_kerr_rhs_f32.argtypes = [P, P, P]
+    L.orc_kerr_rhs_f64.argtypes = [P, P, P]
+    L.orc_kerr_two_H_f32.restype = C.c_float
+    L.orc_kerr_two_H_f32.argtypes = [P, P]
+    L.orc_kerr_two_H_f64.restype = C.c_double
+    L.orc_kerr_two_H_f64.argtypes = [P, P]
+    L.orc_kerr_rk4_f32.argtypes = [P, C.c_float, P, C.c_int, P]
+    L.orc_kerr_rk4_f64.argtypes = [P, C.c_double, P, C.c_int, P]
+    L.orc_schw_rhs_f32.argtypes = [P, C.c_float, P]
+    L.orc_schw_rhs_f64.argtypes = [P, C.c_double, P]
+    L.orc_schw_rk4_f32.argtypes = [P, C.c_float, C.c_float, C.c_int, P]
+    L.orc_schw_rk4_f64.argtypes = [P, C.c_double, C.c_double, C.c_int, P]
+    L.orc_crossing_f32.argtypes = [C.c_float, C.c_float, P, P]
+    L.orc_to_half.argtypes = [P, C.c_int64]
+    L.orc_bloom_extract.argtypes = [P, C.c_int, C.c_int, C.c_float, P, C.c_int]
+    L.orc_bloom_blur.argtypes = [P, C.c_int, C.c_int, C.c_int, P, C.c_int]
+    L.orc_bloom_composite.argtypes = [P, P, C.c_int, C.c_int, C.c_float, C.c_float, P]
+    L.orc_count_kerr_step.argtypes = [P]
+    L.orc_count_schw_step.argtypes = [P]
+    L.orc_make_uniforms.argtypes = [P, P, C.c_double, C.c_double, C.c_double, P]
+    L.orc_loss_grad_dual.restype = C.c_double
+    L.orc_loss_grad_dual.argtypes = [P, P, P, C.c_double, C.c_double, C.c_double, P, C.c_int64, P, C.c_int, P, P]
+    L.orc_loss_f64.restype = C.c_double
+    L.orc_loss_f64.argtypes = [P, P, P, C.c_double, C.c_double, C.c_double, P, C.c_int64, P, C.c_int, P]
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data
+
+
+def _ubuf(u) -> C.Array:
+    b = bytes(u)
+    assert len(b) == 160, "uniform block must be 160 bytes"
+    return C.create_string_buffer(b, 160)
+
+
+def render(u, precision: int = 32, pixels: np.ndarray | None = None, threads: int = 0,
+           extras: Extras | None = None, lib: C.CDLL | None = None) -> dict:
+    """Render all pixels (or the listed linear pixel indices) with the oracle.
+
+    Returns hdr (n,3) f64, ldr (n,3) f64, rgba (n,4) u8, cls (n,) u8, steps (n,) u32,
+    drift (n,3) f64 (max |invariant(end) - invariant(start)| over the pixel's samples).
+    """
+    L = lib or strict()
+    ub = _ubuf(u)
+    width = int.from_bytes(bytes(u)[112:116], "little")
+    height = int.from_bytes(bytes(u)[116:120], "little")
+    if pixels is None:
+        n = width * height
+        px = None
+    else:
+        px = np.ascontiguousarray(pixels, dtype=np.int64)
+        n = int(px.size)
+    out = {
+        "hdr": np.zeros((n, 3), np.float64), "ldr": np.zeros((n, 3), np.float64),
+        "rgba": np.zeros((n, 4), np.uint8), "cls": np.zeros(n, np.uint8),
+        "steps": np.zeros(n, np.uint32), "drift": np.zeros((n, 3), np.float64),
+    }
+    ex = C.byref(extras) if extras is not None else None
+    rc = L.orc_render(ub, precision, ex, _ptr(px), n, threads or (os.cpu_count() or 1), _ptr(out["hdr"]),
+                      _ptr(out["ldr"]), _ptr(out["rgba"]), _ptr(out["cls"]), _ptr(out["steps"]), _ptr(out["drift"]))
+    assert rc == 0
+    return out
+
+
+def trace_ray(u, gx: int, gy: int, ox: float = 0.5, oy: float = 0.5, precision: int = 32,
+              extras: Extras | None = None) -> dict:
+    o = np.zeros(14, np.float64)
+    ex = C.byref(extras) if extras is not None else None
+    strict().orc_trace_ray(_ubuf(u), precision, ex, gx, gy, ox, oy, _ptr(o))
+    return {"rgb": o[0:3].copy(), "cls": int(o[3]), "steps": int(o[4]), "r_cross": o[5],
+            "inv0": o[6:9].copy(), "inv1": o[9:12].copy(), "E": o[12], "Lz": o[13]}
+
+
+def count_step(kerr: bool = True) -> dict:
+    c = np.zeros(6, np.float64)
+    (strict().orc_count_kerr_step if kerr else strict().orc_count_schw_step)(_ptr(c))
+    return dict(zip(["add", "mul", "div", "sqrt", "trig", "other"], c.astype(int).tolist()))
+
+
+def kerr_radius(which: int, M: float, a: float) -> float:
+    return float(strict().orc_kerr_closed_form(which, M, a))
+
+
+def bloom_pipeline(hdr_rgb: np.ndarray, threshold: float, strength: float, exposure: float,
+                   half_store: bool = True) -> dict:
+    """Post passes on an (H, W, 3) float32 HDR image; returns all intermediates."""
+    L = strict()
+    H, W, _ = hdr_rgb.shape
+    h = np.ascontiguousarray(hdr_rgb, dtype=np.float32).copy()
+    if half_store:
+        L.orc_to_half(_ptr(h), h.size)
+    Wb, Hb = W // 2, H // 2
+    ext = np.zeros((Hb, Wb, 3), np.float32)
+    bh = np.zeros_like(ext)
+    bv = np.zeros_like(ext)
+    L.orc_bloom_extract(_ptr(h), W, H, threshold, _ptr(ext), int(half_store))
+    L.orc_bloom_blur(_ptr(ext), Wb, Hb, 1, _ptr(bh), int(half_store))
+    L.orc_bloom_blur(_ptr(bh), Wb, Hb, 0, _ptr(bv), int(half_store))
+    rgba = np.zeros((H, W, 4), np.uint8)
+    L.orc_bloom_composite(_ptr(h), _ptr(bv), W, H, strength, exposure, _ptr(rgba))
+    return {"hdr": h, "extract": ext, "blur_h": bh, "blur_v": bv, "rgba": rgba}

@@ -1,0 +1,326 @@
+"""singularity_amd — MI355X-native offline null-geodesic renderer.
+
+Python view of the C ABI in ``include/sing_abi.h`` (libsing_hip.so, built
+in-tree by ``singularity_amd.build``).  ``HipBackend`` mirrors the reference's
+``RenderBackend`` (render/include/render_backend.hpp:107-132): initialize,
+shutdown, resize, render_frame, capture_frame, name.  The host helpers mirror
+``cuda_cli``'s scene ingestion (cuda_cli/main.cpp:68-109,193-218) and
+``pack_uniforms_cuda`` (render/cuda/cuda_backend.cu:45-101).
+
+There is no CPU fallback: every render call goes through the HIP library, and
+loading fails loudly when it has not been built.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libsing_hip.so"
+
+# --- constants (include/sing_abi.h) ------------------------------------------
+FLAG_DOPPLER = 1 << 0
+FLAG_REDSHIFT = 1 << 1
+FLAG_DISC = 1 << 2
+FLAG_STARFIELD = 1 << 3
+FLAG_ADAPTIVE = 1 << 4
+METRIC_SCHWARZSCHILD = 0
+METRIC_KERR = 1
+CLASS_HORIZON, CLASS_ESCAPE, CLASS_DISC, CLASS_BUDGET, CLASS_INITFAIL = range(5)
+PREC_FP32, PREC_FP64, PREC_MIXED = 0, 1, 2
+OUT_RGBA8, OUT_HDR, OUT_CLASS, OUT_STEPS, OUT_HDR16 = 1, 2, 4, 8, 16
+POST_NONE, POST_BLOOM = 0, 1
+SING_OK = 0
+ERR_NAMES = {1: "ARG", 2: "HIP", 3: "STATE", 4: "NOMEM", 5: "IO", 6: "PARSE", 7: "KEY"}
+
+
+class Vec4(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("w", C.c_float)]
+
+
+class Uniforms(C.Structure):
+    """The 160-byte block of shared_shader/uniforms.h:54-95."""
+
+    _fields_ = [
+        ("cam_pos", Vec4), ("cam_fwd", Vec4), ("cam_right", Vec4), ("cam_up", Vec4),
+        ("mass_M", C.c_float), ("spin_a", C.c_float), ("rs", C.c_float), ("tan_half_fov", C.c_float),
+        ("disc_r_inner", C.c_float), ("disc_r_outer", C.c_float), ("disc_peak_T", C.c_float), ("aspect", C.c_float),
+        ("h_step", C.c_float), ("escape_r", C.c_float), ("horizon_cut", C.c_float), ("time_sec", C.c_float),
+        ("width", C.c_uint32), ("height", C.c_uint32), ("metric_type", C.c_uint32), ("flags", C.c_uint32),
+        ("max_steps", C.c_uint32), ("frame_index", C.c_uint32), ("supersample", C.c_uint32), ("pad_b", C.c_uint32),
+        ("exposure", C.c_float), ("bloom_threshold", C.c_float), ("bloom_strength", C.c_float),
+        ("disc_turbulence", C.c_float),
+    ]
+
+    def to_bytes(self) -> bytes:
+        return bytes(self)
+
+    def copy(self) -> "Uniforms":
+        return Uniforms.from_buffer_copy(bytes(self))
+
+
+class RenderOpts(C.Structure):
+    _fields_ = [
+        ("precision", C.c_uint32), ("outputs", C.c_uint32), ("post", C.c_uint32), ("turb_seed", C.c_uint32),
+        ("adapt_tol", C.c_float), ("band_width", C.c_float), ("peak_T_scale", C.c_float),
+        ("step_batch", C.c_uint32), ("waves_per_cu", C.c_uint32), ("reserved", C.c_uint32 * 7),
+    ]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [
+        ("kernel_ms", C.c_double), ("post_ms", C.c_double), ("rays", C.c_uint64), ("steps", C.c_uint64),
+        ("fp64_rays", C.c_uint64), ("reserved", C.c_uint64 * 3),
+    ]
+
+
+class Scene(C.Structure):
+    """render_backend.hpp:46-82 Scene (the fields the offline path uses)."""
+
+    _fields_ = [
+        ("metric", C.c_uint32), ("mass", C.c_float), ("spin_a_over_M", C.c_float), ("disc_inner_M", C.c_float),
+        ("disc_outer_M", C.c_float), ("doppler_on", C.c_uint32), ("redshift_on", C.c_uint32),
+        ("disc_on", C.c_uint32), ("supersample", C.c_uint32), ("exposure", C.c_float),
+        ("bloom_threshold", C.c_float), ("bloom_strength", C.c_float), ("disc_turbulence", C.c_float),
+        ("disc_peak_T_K", C.c_float), ("max_steps", C.c_uint32), ("h_step", C.c_float),
+        ("escape_r_M", C.c_float), ("horizon_cut_rs", C.c_float),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("basis", C.c_float * 9), ("fov_y_radians", C.c_float)]
+
+
+class SceneConfig(C.Structure):
+    _fields_ = [
+        ("mass", C.c_float), ("spin", C.c_float), ("disc_r_inner", C.c_float), ("disc_r_outer", C.c_float),
+        ("disc_doppler_on", C.c_uint32), ("disc_redshift_on", C.c_uint32), ("camera_fov_deg", C.c_float),
+        ("camera_distance", C.c_float), ("camera_elevation", C.c_float), ("h_step", C.c_float),
+        ("max_steps", C.c_int32),
+    ]
+
+
+class SingError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{ERR_NAMES.get(code, code)}] {msg}")
+        self.code = code
+
+
+_LIB: C.CDLL | None = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "sing_abi_version": (C.c_int, []),
+    "sing_backend_name": (C.c_char_p, []),
+    "sing_last_error": (C.c_char_p, [C.c_void_p]),
+    "sing_create": (C.c_void_p, [C.c_uint32, C.c_uint32, C.c_int]),
+    "sing_destroy": (None, [C.c_void_p]),
+    "sing_resize": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "sing_render": (C.c_int, [C.c_void_p, C.POINTER(Uniforms), C.POINTER(RenderOpts), C.POINTER(RenderStats)]),
+    "sing_capture_rgba8": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "sing_capture_hdr": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "sing_capture_class": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "sing_capture_steps": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "sing_capture_hdr16": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "sing_scene_defaults": (None, [C.POINTER(Scene)]),
+    "sing_scene_config_defaults": (None, [C.POINTER(SceneConfig)]),
+    "sing_parse_scene_config": (
+        C.c_int, [C.c_char_p, C.POINTER(SceneConfig), C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
+    "sing_load_scene_config": (
+        C.c_int, [C.c_char_p, C.POINTER(SceneConfig), C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
+    "sing_scene_from_config": (None, [C.POINTER(SceneConfig), C.POINTER(Scene)]),
+    "sing_orbital_camera": (None, [C.c_float, C.c_float, C.c_float, C.c_float, C.POINTER(Camera)]),
+    "sing_pack_uniforms": (
+        None, [C.POINTER(Scene), C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_float, C.POINTER(Uniforms)]),
+    "sing_kerr_radius": (C.c_double, [C.c_int, C.c_double, C.c_double]),
+}
+
+
+def lib() -> C.CDLL:
+    """Load libsing_hip.so (built in-tree); raises when it is missing."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is not built; run `python -m singularity_amd.build` (no CPU fallback exists)"
+            )
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        for name in ("sing_backward", "sing_render_batch"):
+            if hasattr(L, name):
+                pass  # typed by their own modules
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols() -> list[str]:
+    return sorted(_SIGS)
+
+
+# --- host-side scene ingestion ----------------------------------------------
+def scene_defaults() -> Scene:
+    s = Scene()
+    lib().sing_scene_defaults(C.byref(s))
+    return s
+
+
+def scene_config_defaults() -> SceneConfig:
+    c = SceneConfig()
+    lib().sing_scene_config_defaults(C.byref(c))
+    return c
+
+
+def parse_scene_config(text: str) -> SceneConfig:
+    c = scene_config_defaults()
+    line = C.c_int(0)
+    buf = C.create_string_buffer(512)
+    rc = lib().sing_parse_scene_config(text.encode(), C.byref(c), C.byref(line), buf, 512)
+    if rc != SING_OK:
+        raise SingError(rc, f"scene parse failed (line {line.value}): {buf.value.decode()}")
+    return c
+
+
+def load_scene_config(path: str | Path) -> SceneConfig:
+    c = scene_config_defaults()
+    line = C.c_int(0)
+    buf = C.create_string_buffer(512)
+    rc = lib().sing_load_scene_config(str(path).encode(), C.byref(c), C.byref(line), buf, 512)
+    if rc != SING_OK:
+        raise SingError(rc, f"scene load failed (line {line.value}): {buf.value.decode()}")
+    return c
+
+
+def scene_from_config(cfg: SceneConfig, base: Scene | None = None) -> Scene:
+    s = base if base is not None else scene_defaults()
+    lib().sing_scene_from_config(C.byref(cfg), C.byref(s))
+    return s
+
+
+def orbital_camera(distance: float = 30.0, elevation: float = 0.15, azimuth: float = 0.0,
+                   fov_y_deg: float = 60.0) -> Camera:
+    cam = Camera()
+    lib().sing_orbital_camera(distance, elevation, azimuth, fov_y_deg, C.byref(cam))
+    return cam
+
+
+def pack_uniforms(scene: Scene, cam: Camera, width: int, height: int, time_sec: float = 0.0) -> Uniforms:
+    u = Uniforms()
+    lib().sing_pack_uniforms(C.byref(scene), C.byref(cam), width, height, time_sec, C.byref(u))
+    return u
+
+
+def kerr_radius(which: int, mass: float, spin_a: float) -> float:
+    return float(lib().sing_kerr_radius(which, mass, spin_a))
+
+
+def isco_prograde(a_over_M: float, mass: float = 1.0) -> float:
+    return kerr_radius(2, mass, a_over_M * mass)
+
+
+@dataclass
+class Frame:
+    rgba: np.ndarray | None = None
+    hdr: np.ndarray | None = None
+    cls: np.ndarray | None = None
+    steps: np.ndarray | None = None
+    hdr16: np.ndarray | None = None
+    stats: RenderStats | None = None
+
+
+class HipBackend:
+    """RenderBackend mirror over the C ABI (render/include/render_backend.hpp:107-132)."""
+
+    def __init__(self, width: int = 1280, height: int = 720, device: int = 0):
+        self._ctx = None
+        self.width = self.height = 0
+        self.frame_time_sec = 0.0
+        self.initialize(width, height, device)
+
+    def name(self) -> str:
+        return lib().sing_backend_name().decode()
+
+    def initialize(self, width: int, height: int, device: int = 0) -> bool:
+        self.shutdown()
+        ctx = lib().sing_create(width, height, device)
+        if not ctx:
+            raise SingError(2, lib().sing_last_error(None).decode())
+        self._ctx = ctx
+        self.width, self.height = width, height
+        self.frame_time_sec = 0.0
+        return True
+
+    def shutdown(self) -> None:
+        if self._ctx:
+            lib().sing_destroy(self._ctx)
+        self._ctx = None
+
+    def __del__(self):
+        try:
+            self.shutdown()
+        except Exception:
+            pass
+
+    def resize(self, width: int, height: int) -> None:
+        self._check(lib().sing_resize(self._ctx, width, height))
+        self.width, self.height = width, height
+
+    def _check(self, rc: int) -> None:
+        if rc != SING_OK:
+            raise SingError(rc, lib().sing_last_error(self._ctx).decode())
+
+    def render(self, u: Uniforms, opts: RenderOpts | None = None) -> RenderStats:
+        """One synchronous frame from a packed uniform block."""
+        st = RenderStats()
+        o = opts if opts is not None else RenderOpts()
+        self._check(lib().sing_render(self._ctx, C.byref(u), C.byref(o), C.byref(st)))
+        return st
+
+    def render_frame(self, scene: Scene, camera: Camera, opts: RenderOpts | None = None) -> RenderStats:
+        """CudaBackend::render_frame semantics: pack, render, advance the 1/30 s virtual clock."""
+        u = pack_uniforms(scene, camera, self.width, self.height, self.frame_time_sec)
+        st = self.render(u, opts)
+        self.frame_time_sec += np.float32(1.0) / np.float32(30.0)
+        self.frame_time_sec = float(np.float32(self.frame_time_sec))
+        return st
+
+    def capture(self, outputs: int) -> Frame:
+        n = self.width * self.height
+        f = Frame()
+        if outputs & OUT_RGBA8:
+            f.rgba = np.empty((self.height, self.width, 4), np.uint8)
+            self._check(lib().sing_capture_rgba8(self._ctx, f.rgba.ctypes.data, f.rgba.nbytes))
+        if outputs & OUT_HDR:
+            f.hdr = np.empty((self.height, self.width, 3), np.float32)
+            self._check(lib().sing_capture_hdr(self._ctx, f.hdr.ctypes.data, f.hdr.nbytes))
+        if outputs & OUT_CLASS:
+            f.cls = np.empty((self.height, self.width), np.uint8)
+            self._check(lib().sing_capture_class(self._ctx, f.cls.ctypes.data, f.cls.nbytes))
+        if outputs & OUT_STEPS:
+            f.steps = np.empty((self.height, self.width), np.uint32)
+            self._check(lib().sing_capture_steps(self._ctx, f.steps.ctypes.data, f.steps.nbytes))
+        if outputs & OUT_HDR16:
+            f.hdr16 = np.empty((self.height, self.width, 4), np.float16)
+            self._check(lib().sing_capture_hdr16(self._ctx, f.hdr16.ctypes.data, f.hdr16.nbytes))
+        assert n == self.width * self.height
+        return f
+
+    def capture_frame(self) -> np.ndarray:
+        return self.capture(OUT_RGBA8).rgba
+
+    def render_outputs(self, u: Uniforms, outputs: int, precision: int = PREC_FP32, **kw) -> Frame:
+        o = RenderOpts()
+        o.precision = precision
+        o.outputs = outputs
+        for k, v in kw.items():
+            setattr(o, k, v)
+        st = self.render(u, o)
+        f = self.capture(outputs | (OUT_RGBA8 if o.post else 0))
+        f.stats = st
+        return f

@@ -1,0 +1,363 @@
+// singularity_amd/csrc/physics.hpp — device-side physics of the trace kernels,
+// templated on the arithmetic type R (float or double).
+//
+// The algorithm is the reference's (shared_shader/kerr_hamilton.h:97-178,
+// shared_shader/geodesic_math.h:101-134, shared_shader/disc_intersection.h:23-55,
+// render/cuda/kernels/geodesic_kernel.cu:77-403); the arithmetic is arranged
+// for the CDNA4 vector ALU:
+//   * the integrated time coordinate t is dropped (no output depends on it);
+//   * each RHS evaluates 3 reciprocals (1/Sigma, 1/Delta, 1/sin^2) instead of
+//     13 divides, and per-ray products of E and L_z are formed once per step;
+//   * sin/cos of theta for the pole damping are reused by the first RK4 stage.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "scene_const.hpp"
+
+namespace sing {
+
+// ---- scalar helpers ---------------------------------------------------------
+__device__ __forceinline__ void dsincos(float x, float* s, float* c) { sincosf(x, s, c); }
+__device__ __forceinline__ void dsincos(double x, double* s, double* c) { sincos(x, s, c); }
+__device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float dlog(float x) { return logf(x); }
+__device__ __forceinline__ double dlog(double x) { return log(x); }
+__device__ __forceinline__ float dpow(float x, float y) { return powf(x, y); }
+__device__ __forceinline__ double dpow(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ float dcos(float x) { return cosf(x); }
+__device__ __forceinline__ double dcos(double x) { return cos(x); }
+__device__ __forceinline__ float dsin(float x) { return sinf(x); }
+__device__ __forceinline__ double dsin(double x) { return sin(x); }
+__device__ __forceinline__ float dacos(float x) { return acosf(x); }
+__device__ __forceinline__ double dacos(double x) { return acos(x); }
+__device__ __forceinline__ float datan2(float y, float x) { return atan2f(y, x); }
+__device__ __forceinline__ double datan2(double y, double x) { return atan2(y, x); }
+__device__ __forceinline__ float dmax(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double dmax(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ float dmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ double dmin(double a, double b) { return fmin(a, b); }
+// Reciprocal on the RHS hot path: v_rcp_f32 (1 ulp) in FP32, exact division in FP64.
+__device__ __forceinline__ float drcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double drcp(double x) { return 1.0 / x; }
+template <class R> __device__ __forceinline__ R dclamp(R x, R lo, R hi) { return dmin(dmax(x, lo), hi); }
+
+// ---- Kerr Hamiltonian RHS (kerr_hamilton.h:97-155) ----------------------------
+// State (r, theta, phi, p_r, p_theta); st/ct = sin/cos(theta).
+template <class R> struct KerrK {
+    R EE, L, L2, cB1, cB0, cphi;  // E^2, L_z, L_z^2, 4 M a E L_z, a^2 L_z^2, 2 M a E
+};
+template <class R> __device__ __forceinline__ KerrK<R> kerr_k(const SceneConst<R>& sc, R E, R L) {
+    KerrK<R> k;
+    k.EE = E * E;
+    k.L = L;
+    k.L2 = L * L;
+    k.cB1 = R(2) * sc.twoM * sc.a * E * L;
+    k.cB0 = sc.a2 * k.L2;
+    k.cphi = sc.twoM * sc.a * E;
+    return k;
+}
+
+template <class R>
+__device__ __forceinline__ void kerr_rhs_sc(R r, R pr, R pth, R st, R ct, const SceneConst<R>& sc, const KerrK<R>& k,
+                                            R& dr, R& dth, R& dph, R& dpr, R& dpth) {
+    const R a2 = sc.a2;
+    const R s2 = st * st;
+    const R c2 = ct * ct;
+    const R safe = (s2 < R(1e-10)) ? R(1e-10) : s2;
+    const R r2 = r * r;
+    const R Sig = r2 + a2 * c2;
+    const R Del = r2 - sc.twoM * r + a2;
+    const R r2a2 = r2 + a2;
+    const R A = r2a2 * r2a2 - a2 * Del * s2;
+    const R iS = drcp(Sig);
+    const R iD = drcp(Del);
+    const R iss = drcp(safe);
+    const R iSD = iS * iD;
+    dr = Del * pr * iS;
+    dth = pth * iS;
+    dph = (k.cphi * r * s2 + (Del - a2 * s2) * k.L) * (iSD * iss);
+    // 2H = B/(Sigma Delta) + L^2/(Sigma sin^2) + (Delta p_r^2 + p_th^2)/Sigma,
+    // B = -A E^2 + 4 M r a E L - a^2 L^2  (kerr_hamilton.h:121-152)
+    const R B = k.cB1 * r - A * k.EE - k.cB0;
+    const R twor = R(2) * r;
+    const R Del_r = twor - sc.twoM;
+    const R A_r = R(2) * twor * r2a2 - a2 * Del_r * s2;
+    const R B_r = k.cB1 - A_r * k.EE;
+    const R sin2t = R(2) * st * ct;
+    const R Sig_t = -a2 * sin2t;
+    const R B_t = a2 * Del * sin2t * k.EE;
+    const R iS2 = iS * iS;
+    const R pr2 = pr * pr;
+    const R Dp2 = Del * pr2 + pth * pth;
+    const R dHdr = B_r * iSD - B * (twor * Del + Sig * Del_r) * (iSD * iSD) - k.L2 * twor * (iS2 * iss)
+                   + Del_r * pr2 * iS - Dp2 * twor * iS2;
+    const R dHdth = B_t * iSD - B * Sig_t * (iS2 * iD) - k.L2 * (Sig_t * safe + Sig * sin2t) * (iS2 * iss * iss)
+                    - Dp2 * Sig_t * iS2;
+    dpr = R(-0.5) * dHdr;
+    dpth = R(-0.5) * dHdth;
+}
+
+// Classical RK4 with the reference's pole damping h*clamp(5|sin theta|, 0.05, 1)
+// (kerr_hamilton.h:165-178).  hs is the undamped step.
+template <class R>
+__device__ __forceinline__ void kerr_rk4(R& r, R& th, R& ph, R& pr, R& pth, R hs, const SceneConst<R>& sc,
+                                         const KerrK<R>& k) {
+    R st, ct;
+    dsincos(th, &st, &ct);
+    R ps = (st < R(0) ? -st : st) * R(5);
+    ps = (ps > R(1)) ? R(1) : ((ps < R(0.05)) ? R(0.05) : ps);
+    const R h = hs * ps;
+    const R hh = R(0.5) * h;
+    R k_r, k_th, k_ph, k_pr, k_pth;
+    kerr_rhs_sc(r, pr, pth, st, ct, sc, k, k_r, k_th, k_ph, k_pr, k_pth);
+    R a_r = k_r, a_th = k_th, a_ph = k_ph, a_pr = k_pr, a_pth = k_pth;
+    R y_r = r + hh * k_r, y_th = th + hh * k_th, y_pr = pr + hh * k_pr, y_pth = pth + hh * k_pth;
+    dsincos(y_th, &st, &ct);
+    kerr_rhs_sc(y_r, y_pr, y_pth, st, ct, sc, k, k_r, k_th, k_ph, k_pr, k_pth);
+    a_r += R(2) * k_r; a_th += R(2) * k_th; a_ph += R(2) * k_ph; a_pr += R(2) * k_pr; a_pth += R(2) * k_pth;
+    y_r = r + hh * k_r; y_th = th + hh * k_th; y_pr = pr + hh * k_pr; y_pth = pth + hh * k_pth;
+    dsincos(y_th, &st, &ct);
+    kerr_rhs_sc(y_r, y_pr, y_pth, st, ct, sc, k, k_r, k_th, k_ph, k_pr, k_pth);
+    a_r += R(2) * k_r; a_th += R(2) * k_th; a_ph += R(2) * k_ph; a_pr += R(2) * k_pr; a_pth += R(2) * k_pth;
+    y_r = r + h * k_r; y_th = th + h * k_th; y_pr = pr + h * k_pr; y_pth = pth + h * k_pth;
+    dsincos(y_th, &st, &ct);
+    kerr_rhs_sc(y_r, y_pr, y_pth, st, ct, sc, k, k_r, k_th, k_ph, k_pr, k_pth);
+    a_r += k_r; a_th += k_th; a_ph += k_ph; a_pr += k_pr; a_pth += k_pth;
+    const R h6 = h * R(1.0 / 6.0);
+    r += h6 * a_r;
+    th += h6 * a_th;
+    ph += h6 * a_ph;
+    pr += h6 * a_pr;
+    pth += h6 * a_pth;
+}
+
+// ---- Schwarzschild geodesic RHS (geodesic_math.h:101-123) ----------------------
+// State (r, theta, phi, u^t, u^r, u^theta, u^phi).
+template <class R>
+__device__ __forceinline__ void schw_rhs(R r, R th, R ut, R ur, R uth, R uph, R rs, R& d_ut, R& d_ur, R& d_uth,
+                                         R& d_uph) {
+    R st, ct;
+    dsincos(th, &st, &ct);
+    const R ir = drcp(r);
+    const R f = R(1) - rs * ir;
+    const R iF = drcp(f);
+    const R is = drcp(st);
+    const R q = rs * ir * ir;  // rs / r^2
+    d_ut = -(q * iF) * ut * ur;
+    d_ur = R(-0.5) * (q * f) * ut * ut + R(0.5) * (q * iF) * ur * ur + r * f * (uth * uth + st * st * uph * uph);
+    d_uth = R(-2) * ur * uth * ir + st * ct * uph * uph;
+    d_uph = R(-2) * ur * uph * ir - R(2) * (ct * is) * uth * uph;
+}
+
+template <class R>
+__device__ __forceinline__ void schw_rk4(R& r, R& th, R& ph, R& ut, R& ur, R& uth, R& uph, R h, R rs) {
+    const R hh = R(0.5) * h;
+    R d_ut, d_ur, d_uth, d_uph;
+    // stage 1: position derivatives are the velocities themselves
+    schw_rhs(r, th, ut, ur, uth, uph, rs, d_ut, d_ur, d_uth, d_uph);
+    R a_r = ur, a_th = uth, a_ph = uph, a_ut = d_ut, a_ur = d_ur, a_uth = d_uth, a_uph = d_uph;
+    R y_r = r + hh * ur, y_th = th + hh * uth;
+    R y_ut = ut + hh * d_ut, y_ur = ur + hh * d_ur, y_uth = uth + hh * d_uth, y_uph = uph + hh * d_uph;
+    // stage 2
+    schw_rhs(y_r, y_th, y_ut, y_ur, y_uth, y_uph, rs, d_ut, d_ur, d_uth, d_uph);
+    a_r += R(2) * y_ur; a_th += R(2) * y_uth; a_ph += R(2) * y_uph;
+    a_ut += R(2) * d_ut; a_ur += R(2) * d_ur; a_uth += R(2) * d_uth; a_uph += R(2) * d_uph;
+    R n_r = r + hh * y_ur, n_th = th + hh * y_uth;
+    R n_ut = ut + hh * d_ut, n_ur = ur + hh * d_ur, n_uth = uth + hh * d_uth, n_uph = uph + hh * d_uph;
+    // stage 3
+    schw_rhs(n_r, n_th, n_ut, n_ur, n_uth, n_uph, rs, d_ut, d_ur, d_uth, d_uph);
+    a_r += R(2) * n_ur; a_th += R(2) * n_uth; a_ph += R(2) * n_uph;
+    a_ut += R(2) * d_ut; a_ur += R(2) * d_ur; a_uth += R(2) * d_uth; a_uph += R(2) * d_uph;
+    y_r = r + h * n_ur; y_th = th + h * n_uth;
+    y_ut = ut + h * d_ut; y_ur = ur + h * d_ur; y_uth = uth + h * d_uth; y_uph = uph + h * d_uph;
+    // stage 4
+    schw_rhs(y_r, y_th, y_ut, y_ur, y_uth, y_uph, rs, d_ut, d_ur, d_uth, d_uph);
+    a_r += y_ur; a_th += y_uth; a_ph += y_uph;
+    a_ut += d_ut; a_ur += d_ur; a_uth += d_uth; a_uph += d_uph;
+    const R h6 = h * R(1.0 / 6.0);
+    r += h6 * a_r; th += h6 * a_th; ph += h6 * a_ph;
+    ut += h6 * a_ut; ur += h6 * a_ur; uth += h6 * a_uth; uph += h6 * a_uph;
+}
+
+// ---- shading (geodesic_kernel.cu:77-171) ----------------------------------------
+template <class R> __device__ __forceinline__ void blackbody(R T, R* rgb) {
+    T = dclamp(T, R(1000), R(40000)) * R(0.01);
+    R r, g, b;
+    if (T <= R(66)) {
+        r = R(1);
+        g = dclamp(R(0.39008157876901960784) * dlog(T) - R(0.63184144378862745098), R(0), R(1));
+    } else {
+        const R t1 = T - R(60);
+        r = dclamp(R(1.29293618606274509804) * dpow(t1, R(-0.1332047592)), R(0), R(1));
+        g = dclamp(R(1.12989086089529411765) * dpow(t1, R(-0.0755148492)), R(0), R(1));
+    }
+    if (T >= R(66)) b = R(1);
+    else if (T <= R(19)) b = R(0);
+    else b = dclamp(R(0.54320678911019607843) * dlog(T - R(10)) - R(1.19625408914), R(0), R(1));
+    rgb[0] = r; rgb[1] = g; rgb[2] = b;
+}
+
+template <class R> __device__ __forceinline__ R disc_temperature(R rc, R r_in, R peak) {
+    if (rc < r_in) return R(0);
+    const R ratio = r_in / rc;
+    const R q = dpow(ratio, R(0.75));
+    const R edge = R(1) - dsqrt(ratio);
+    return peak * (q * dmax(edge, R(0))) / R(0.186);
+}
+
+template <class R> __device__ __forceinline__ R band_mask(const SceneConst<R>& sc, R rc, R phi) {
+    const R x = rc / sc.r_in;
+    const R rho = dlog(dmax(x, R(0.1)));
+    const R shear = R(4) / dmax(dpow(x, R(1.5)), R(0.25));
+    const R omega = dpow(dmax(x, R(0.1)), R(-1.5));
+    const R pa = phi - R(0.8) * sc.time_sec * omega;
+    const R b1 = R(0.5) + R(0.5) * dcos(R(10) * rho + R(3) * pa + shear + sc.ph1);
+    const R b2 = R(0.5) + R(0.5) * dcos(R(22) * rho - R(1.7) * pa + sc.ph2);
+    const R combined = R(0.8) + sc.turbulence * (b1 * b1 * b2 - R(0.8));
+    return dclamp(R(0.65) + R(0.55) * combined, R(0), R(1.3));
+}
+
+__device__ __forceinline__ std::uint32_t mix32(std::uint32_t a, std::uint32_t b) {
+    std::uint32_t x = a * 0x9E3779B9u + b;
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return x;
+}
+
+// Escape colour: starfield in the final direction (geodesic_kernel.cu:135-171,251-259).
+template <class R> __device__ __forceinline__ void shade_escape(const SceneConst<R>& sc, R th, R ph, R* rgb) {
+    rgb[0] = rgb[1] = rgb[2] = R(0);
+    if (!(sc.flags & SING_FLAG_STARFIELD_ON)) return;
+    R st, ct, sp, cp;
+    dsincos(th, &st, &ct);
+    dsincos(ph, &sp, &cp);
+    const R x = st * cp, y = st * sp, z = ct;
+    const R len = dsqrt(x * x + y * y + z * z);
+    if (len < R(1e-6)) {
+        rgb[2] = R(0.005);
+        return;
+    }
+    const R il = R(1) / len;
+    const R pi = R(3.14159265358979323846);
+    const R theta = dacos(dclamp(z * il, R(-1), R(1)));
+    const R phi = datan2(y * il, x * il) + pi;
+    int ix = int(phi / (R(2) * pi) * R(768)) % 768;
+    int iy = int(theta / pi * R(384)) % 384;
+    if (ix < 0) ix += 768;
+    if (iy < 0) iy += 384;
+    const std::uint32_t seed = mix32(std::uint32_t(ix), std::uint32_t(iy));
+    R r = R(0.0015), g = R(0.002), b = R(0.005);
+    if ((seed & 0xFFu) < 4u) {
+        const R br = R((seed >> 8) & 0xFFu) / R(255);
+        const R b2 = br * br;
+        const R brightness = b2 * b2;
+        const R hue = R((seed >> 16) & 0xFFu) / R(255);
+        r = dmin(R(1), r + brightness * (R(1) - R(0.5) * hue));
+        g = dmin(R(1), g + brightness * (R(0.8) + R(0.2) * hue * (R(1) - hue) * R(4)));
+        b = dmin(R(1), b + brightness * (R(0.4) + R(0.6) * hue));
+    }
+    rgb[0] = r * R(0.6);
+    rgb[1] = g * R(0.6);
+    rgb[2] = b * R(0.6);
+}
+
+// Disc emission; kerr: cos psi = clamp(L_z/r) and the Kerr Keplerian rate
+// (geodesic_kernel.cu:366-394); schwarzschild: cos psi = clamp(-u^phi r)
+// (geodesic_kernel.cu:264-289).
+template <class R, bool KERR>
+__device__ __forceinline__ void shade_disc(const SceneConst<R>& sc, R rc, R phi, R dop_arg, R* rgb) {
+    rgb[0] = rgb[1] = rgb[2] = R(0);
+    const R T = disc_temperature(rc, sc.r_in, sc.peak_T);
+    if (!(T > R(0))) return;
+    R f_emit, g_grav = R(1), g_dop = R(1);
+    if (KERR) {
+        f_emit = R(1) - sc.twoM / rc;
+        if (sc.flags & SING_FLAG_REDSHIFT_ON) g_grav = dsqrt(dmax(f_emit, R(1e-6)) / dmax(sc.f_obs, R(1e-6)));
+    } else {
+        f_emit = R(1) - sc.rs / rc;
+        if (sc.flags & SING_FLAG_REDSHIFT_ON) g_grav = dsqrt(dmax(f_emit / sc.f_cam, R(0)));
+    }
+    if (sc.flags & SING_FLAG_DOPPLER_ON) {
+        R om = dsqrt(sc.M / (rc * rc * rc));
+        if (KERR) om = om / (R(1) + sc.a * om);
+        const R v = rc * om / dsqrt(dmax(f_emit, R(1e-6)));
+        const R cos_psi = KERR ? dclamp(dop_arg / rc, R(-1), R(1)) : dclamp(-dop_arg * rc, R(-1), R(1));
+        g_dop = dsqrt(dmax(R(1) - v * v, R(0))) / dmax(R(1) - v * cos_psi, R(1e-4));
+    }
+    const R g = g_grav * g_dop;
+    R col[3];
+    blackbody(g * T, col);
+    const R q = sc.r_in / rc;
+    const R flux = dclamp(q * q * q, R(0.05), R(1.5));
+    const R bands = band_mask(sc, rc, phi);
+    const R g2 = g * g;
+    const R w = flux * bands * g2 * g2;
+    rgb[0] = col[0] * w;
+    rgb[1] = col[1] * w;
+    rgb[2] = col[2] * w;
+}
+
+// ---- tone map (geodesic_kernel.cu:413-431) ---------------------------------------
+template <class R> __device__ __forceinline__ R aces1(R x) {
+    return dclamp((x * (R(2.51) * x + R(0.03))) / (x * (R(2.43) * x + R(0.59)) + R(0.14)), R(0), R(1));
+}
+__device__ __forceinline__ std::uint8_t to_u8(float c) {
+    c = fminf(fmaxf(c, 0.0f), 1.0f);
+    const float s = c <= 0.0031308f ? 12.92f * c : 1.055f * powf(c, 1.0f / 2.4f) - 0.055f;
+    return std::uint8_t(fminf(fmaxf(s * 255.0f + 0.5f, 0.0f), 255.0f));
+}
+__device__ __forceinline__ std::uint8_t to_u8(double c) {
+    c = fmin(fmax(c, 0.0), 1.0);
+    const double s = c <= 0.0031308 ? 12.92 * c : 1.055 * pow(c, 1.0 / 2.4) - 0.055;
+    return std::uint8_t(fmin(fmax(s * 255.0 + 0.5, 0.0), 255.0));
+}
+
+// ---- camera ray (geodesic_kernel.cu:188-223) -----------------------------------------
+template <class R>
+__device__ __forceinline__ void camera_ray(const SceneConst<R>& sc, std::uint32_t gx, std::uint32_t gy, R ox, R oy,
+                                           R& ur, R& uth, R& uph) {
+    const R fx = R(gx) + ox, fy = R(gy) + oy;
+    const R un = (R(2) * fx / sc.width_f - R(1)) * sc.aspect * sc.tan_half_fov;
+    const R vn = -(R(2) * fy / sc.height_f - R(1)) * sc.tan_half_fov;
+    R d0 = sc.fwd[0] + (sc.right[0] * un + sc.up[0] * vn);
+    R d1 = sc.fwd[1] + (sc.right[1] * un + sc.up[1] * vn);
+    R d2 = sc.fwd[2] + (sc.right[2] * un + sc.up[2] * vn);
+    const R len = dsqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    const R ik = R(1) / dmax(len, R(1e-20));
+    d0 *= ik;
+    d1 *= ik;
+    d2 *= ik;
+    ur = d0 * sc.rhat[0] + d1 * sc.rhat[1] + d2 * sc.rhat[2];
+    uth = (d0 * sc.that[0] + d1 * sc.that[1] + d2 * sc.that[2]) / sc.cam_r;
+    uph = (d0 * sc.phat[0] + d1 * sc.phat[1] + d2 * sc.phat[2]) / sc.uphi_den;
+}
+
+// Photon-orbit margin of a ray with impact parameters xi = L/E, eta = Q/E^2:
+// R(r)/E^2 = r^4 + (a^2 - xi^2 - eta) r^2 + 2 M ((xi - a)^2 + eta) r - a^2 eta.
+// A ray is near-critical (it winds around the photon shell and amplifies
+// rounding exponentially) when the local minimum of R outside the horizon is
+// close to zero.  Returns |R(r*)| / r*^4 at the largest stationary point r*,
+// or a large value when R has no such minimum outside r+.
+__device__ __forceinline__ double photon_margin(double xi, double eta, double a, double M, double r_plus) {
+    const double C = (xi - a) * (xi - a) + eta;
+    const double P = 0.25 * 2.0 * (a * a - xi * xi - eta);  // R'/4 = r^3 + P r + Qc
+    const double Qc = 0.5 * M * C;
+    if (!(P < 0.0)) return 1e30;
+    const double disc = -(4.0 * P * P * P + 27.0 * Qc * Qc);
+    if (!(disc > 0.0)) return 1e30;
+    const double m = 2.0 * sqrt(-P / 3.0);
+    double arg = (3.0 * Qc / (2.0 * P)) * sqrt(-3.0 / P);
+    arg = fmin(fmax(arg, -1.0), 1.0);
+    const double rs = m * cos(acos(arg) / 3.0);
+    if (!(rs > r_plus)) return 1e30;
+    const double r2 = rs * rs;
+    const double Rv = r2 * r2 + (a * a - xi * xi - eta) * r2 + 2.0 * M * C * rs - a * a * eta;
+    return fabs(Rv) / (r2 * r2);
+}
+
+}  // namespace sing

@@ -1,0 +1,178 @@
+"""bench.py — headline benchmark of the offline null-geodesic renderer on one MI355X.
+
+Workload (the reference's published headline, README.md:39,130; docs/PHASE8_CUDA.md:84-87):
+scenes/kerr_a094.scene (Kerr a/M = 0.94, disc [1.7 M, 30 M], Doppler + redshift) at
+cuda_cli defaults (camera 30 M, elevation 0.15 rad, FOV 60 deg), 3840x2160, 256 spp
+Halton(2,3), h = 0.4 M, <= 2500 steps, escape 200 M.  The reference renders it in
+162 s on an RTX 3090 (13.1 Mrays/s, FP32).
+
+One step = one full frame in the mixed FP32/FP64 form (photon-band rays in FP64).
+Warmup frames are untimed; the K timed frames are bracketed by device-synchronous
+calls (sing_render returns after the frame completes).  After them, one plain-FP32
+frame and one all-FP64 frame of the same scene give the precision ratios and the
+per-pixel radiance error of mixed and plain FP32 against FP64.
+
+Prints ONE JSON line.  The CPU baseline is the repo's C++ restatement of the
+reference path compiled with the reference's Release flags (-O3 -ffast-math),
+row-parallel over the host cores, timed on a seeded pixel subset.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+import singularity_amd as sa
+from singularity_amd import flops as FL
+
+METRIC = "Mrays/s Kerr 4K (≤1e-3 err vs FP64), mixed vs plain FP32; grads/s"
+PUBLISHED_MRAYS = 13.1  # 4K Kerr a/M=0.94, 256 spp, RTX 3090 (README.md:39,130)
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def headline_uniforms(width: int, height: int, spp: int) -> sa.Uniforms:
+    # scenes/kerr_a094.scene: spin 0.94, disc_r_inner 1.7, disc_r_outer 30, doppler/redshift on
+    cfg = sa.parse_scene_config("mass = 1.0\nspin = 0.94\ndisc_r_inner = 1.7\ndisc_r_outer = 30.0\n"
+                                "disc_doppler_on = 1\ndisc_redshift_on = 1\n")
+    scene = sa.scene_from_config(cfg)
+    scene.supersample = spp
+    cam = sa.orbital_camera(30.0, 0.15, 0.0, 60.0)
+    return sa.pack_uniforms(scene, cam, width, height, 0.0)
+
+
+def err_stats(a: np.ndarray, ref: np.ndarray, ca: np.ndarray, cref: np.ndarray) -> dict:
+    a = a.reshape(-1, 3).astype(np.float64)
+    b = ref.reshape(-1, 3).astype(np.float64)
+    rel = np.abs(a - b).max(-1) / np.maximum(np.abs(b).max(-1), 1e-6)
+    return {
+        "frac_pixels_rel_le_1e-3": float(np.mean(rel <= 1e-3)),
+        "median_rel": float(np.median(rel)),
+        "p99_rel": float(np.quantile(rel, 0.99)),
+        "p999_rel": float(np.quantile(rel, 0.999)),
+        "image_rel_l1": float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-30)),
+        "class_agreement": float(np.mean(ca.ravel() == cref.ravel())),
+    }
+
+
+def cpu_baseline(u: sa.Uniforms, budget_s: float, seed: int = 1234) -> dict:
+    import oracle  # CPU baseline leg only
+
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    L = oracle.fast()
+    rng = np.random.default_rng(seed)
+    P = u.width * u.height
+    order = rng.permutation(P)
+    done, t0, batch = 0, time.perf_counter(), 256
+    rays_per_px = max(1, u.supersample)
+    while True:
+        px = order[done:done + batch]
+        oracle.render(u, 32, pixels=px, threads=threads, lib=L)
+        done += px.size
+        el = time.perf_counter() - t0
+        if el >= budget_s or done >= P:
+            break
+        batch = min(4096, batch * 2)
+    mrays = done * rays_per_px / el / 1e6
+    return {
+        "value": round(mrays, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+        "sample": f"{done} seeded-random pixels x {rays_per_px} spp of the same scene "
+                  f"({done * rays_per_px} rays, {el:.1f} s); FP32 C++ restatement, -O3 -ffast-math, "
+                  f"{threads} std::threads pulling pixels from an atomic counter",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--res", default="3840x2160")
+    ap.add_argument("--spp", type=int, default=256)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-compare", action="store_true", help="skip the plain-FP32 and FP64 comparison frames")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU; the renderer has no multi-GPU path "
+              "(rays shard by pixel tiles with no exchange inside the trace)", file=sys.stderr)
+        return 2
+    W, H = (int(v) for v in args.res.lower().split("x"))
+    u = headline_uniforms(W, H, args.spp)
+    be = sa.HipBackend(W, H)
+    outs = sa.OUT_RGBA8 | sa.OUT_HDR | sa.OUT_CLASS
+    mixed = sa.RenderOpts(precision=sa.PREC_MIXED, outputs=outs)
+    rays_per_frame = W * H * args.spp
+
+    for i in range(args.warmup):
+        st = be.render(u, mixed)
+        log(f"warmup {i}: {st.kernel_ms:.0f} ms")
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        stats.append(be.render(u, mixed))
+        log(f"step {i}: {stats[-1].kernel_ms:.0f} ms")
+    wall = time.perf_counter() - t0
+    f_mixed = be.capture(outs)
+    value = rays_per_frame * args.steps / wall / 1e6
+    trace_ms = float(np.mean([s.trace_ms for s in stats]))
+    kern_ms = float(np.mean([s.kernel_ms for s in stats]))
+    steps_total = stats[-1].steps
+    achieved = FL.KERR_STEP_FLOPS * steps_total / (trace_ms * 1e-3) / 1e12
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": round(value / PUBLISHED_MRAYS, 3) if (W, H, args.spp) == (3840, 2160, 256) else None,
+        "dtype": "fp32+fp64", "data": "synthetic",
+        "config": {
+            "workload": f"kerr_a094 scene {W}x{H} {args.spp} spp mixed FP32/FP64",
+            "scene": "Kerr a/M=0.94, disc [1.7 M, 30 M], Doppler+redshift, 4500 K (scenes/kerr_a094.scene)",
+            "camera": "r=30 M, elevation 0.15 rad, azimuth 0, FOV-y 60 deg",
+            "integrator": "Hamiltonian RK4 h=0.4 M with pole damping, <=2500 steps, escape 200 M, horizon 1.02 r+",
+            "resolution": f"{W}x{H}", "spp": args.spp, "precision": "mixed (photon-band rays FP64)",
+        },
+        "roofline": {
+            "bound": "valu", "achieved": round(achieved, 3), "peak": FL.PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FL.PEAK_FP32_TFLOPS, 4), "traffic": None,
+            "kernel": "trace (FP32 pass + FP64 band pass)",
+            "flops_per_step": FL.KERR_STEP_FLOPS, "steps_per_launch": int(steps_total),
+            "frac_of_nonpacked_fp32": round(achieved / FL.PEAK_FP32_NONPACKED_TFLOPS, 4),
+        },
+        "kernel_ms_per_frame": round(kern_ms, 2), "trace_ms_per_frame": round(trace_ms, 2),
+        "shade_ms_per_frame": round(float(np.mean([s.shade_ms for s in stats])), 2),
+        "rk4_steps_per_s": round(steps_total / (kern_ms * 1e-3), 1),
+        "fp64_rays_per_frame": int(stats[-1].fp64_rays),
+        "chunks_per_frame": int(stats[-1].chunks),
+    }
+    if not args.no_compare:
+        st32 = be.render(u, sa.RenderOpts(precision=sa.PREC_FP32, outputs=outs))
+        f32 = be.capture(outs)
+        log(f"plain fp32: {st32.kernel_ms:.0f} ms")
+        st64 = be.render(u, sa.RenderOpts(precision=sa.PREC_FP64, outputs=outs))
+        f64 = be.capture(outs)
+        log(f"fp64: {st64.kernel_ms:.0f} ms")
+        out["plain_fp32"] = {"mrays_s": round(rays_per_frame / st32.kernel_ms / 1e3, 3),
+                             "kernel_ms": round(st32.kernel_ms, 2),
+                             "error_vs_fp64": err_stats(f32.hdr, f64.hdr, f32.cls, f64.cls)}
+        out["fp64"] = {"mrays_s": round(rays_per_frame / st64.kernel_ms / 1e3, 3), "kernel_ms": round(st64.kernel_ms, 2)}
+        out["mixed_error_vs_fp64"] = err_stats(f_mixed.hdr, f64.hdr, f_mixed.cls, f64.cls)
+        out["mixed_over_plain_time"] = round(kern_ms / st32.kernel_ms, 3)
+        out["fp64_over_mixed_time"] = round(st64.kernel_ms / kern_ms, 3)
+    be.shutdown()
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(u, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -121,12 +121,14 @@ typedef struct SingRenderOpts {
 } SingRenderOpts;
 
 typedef struct SingRenderStats {
-    double kernel_ms;     /* trace kernel time, HIP events on the context stream */
-    double post_ms;       /* post-pass kernels time (0 when no post pass) */
+    double kernel_ms;     /* all kernels of the frame, HIP events on the context stream */
+    double trace_ms;      /* trace kernels (FP32 pass and FP64 pass) */
+    double shade_ms;      /* shade/accumulate kernels */
+    double post_ms;       /* post-pass kernels (0 when no post pass) */
     uint64_t rays;        /* pixel samples traced */
     uint64_t steps;       /* RK4 steps executed over all rays */
-    uint64_t fp64_rays;   /* rays traced in FP64 (mixed precision) */
-    uint64_t reserved[3];
+    uint64_t fp64_rays;   /* rays traced in FP64 */
+    uint64_t chunks;      /* sample chunks the frame was split into */
 } SingRenderStats;
 
 typedef struct SingContext SingContext;

@@ -26,7 +26,8 @@ _LIBS: dict[str, C.CDLL] = {}
 
 class Extras(C.Structure):
     _fields_ = [("soft_edge", C.c_double), ("band_phase1", C.c_double), ("band_phase2", C.c_double),
-                ("adapt_tol", C.c_double), ("peak_T_scale", C.c_double)]
+                ("adapt_tol", C.c_double), ("peak_T_scale", C.c_double), ("jitter_x", C.c_double),
+                ("jitter_y", C.c_double)]
 
 
 def build() -> None:
@@ -153,6 +154,18 @@ def render(u, precision: int = 32, pixels: np.ndarray | None = None, threads: in
                       _ptr(out["ldr"]), _ptr(out["rgba"]), _ptr(out["cls"]), _ptr(out["steps"]), _ptr(out["drift"]))
     assert rc == 0
     return out
+
+
+def boundary_band(u, pixels: np.ndarray | None = None, eps: float = 1e-4, threads: int = 0) -> np.ndarray:
+    """Pixels whose FP64 termination class changes when the sample moves by
+    +-eps pixel in x or y: the chaotic boundary band (photon ring, disc edges,
+    shadow edge) where FP32 and FP64 may legitimately disagree."""
+    base = render(u, 64, pixels, threads)["cls"]
+    band = np.zeros(base.shape, bool)
+    for jx, jy in ((eps, 0.0), (-eps, 0.0), (0.0, eps), (0.0, -eps)):
+        ex = Extras(jitter_x=jx, jitter_y=jy)
+        band |= render(u, 64, pixels, threads, extras=ex)["cls"] != base
+    return band
 
 
 def trace_ray(u, gx: int, gy: int, ox: float = 0.5, oy: float = 0.5, precision: int = 32,

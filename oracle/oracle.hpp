@@ -240,6 +240,8 @@ template <class T> struct Extras {
     T band_phase2 = T(0.0);
     T adapt_tol = T(0.0);   // >0: error-controlled RK4 (step doubling), tolerance per step
     T peak_T_scale = T(1.0);
+    T jitter_x = T(0.0);    // added to every sample offset (boundary-band probes)
+    T jitter_y = T(0.0);
 };
 
 template <class T> struct Uni {
@@ -923,6 +925,10 @@ inline PixelOut<T> render_pixel(const Uni<T>& u, const CamFrame<T>& f, std::uint
         if (spp > 1u) {
             ox = T(double(halton_xy[2 * s]));
             oy = T(double(halton_xy[2 * s + 1]));
+        }
+        if (val(u.ext.jitter_x) != 0.0 || val(u.ext.jitter_y) != 0.0) {
+            ox = ox + u.ext.jitter_x;
+            oy = oy + u.ext.jitter_y;
         }
         const RayOut<T> r = trace_ray(u, f, gx, gy, ox, oy);
         if (s == 0) p.cls = std::uint8_t(r.cls);

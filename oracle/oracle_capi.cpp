@@ -16,6 +16,8 @@ struct OrcExtras {  // mirrors orc::Extras in double; all zero = reference seman
     double band_phase2;
     double adapt_tol;
     double peak_T_scale;  // 0 is read as 1
+    double jitter_x;      // added to every sample offset
+    double jitter_y;
 };
 
 template <class T> Uni<T> load_uni(const void* raw, const OrcExtras* ex) {
@@ -28,6 +30,8 @@ template <class T> Uni<T> load_uni(const void* raw, const OrcExtras* ex) {
         o.ext.band_phase2 = T(ex->band_phase2);
         o.ext.adapt_tol = T(ex->adapt_tol);
         o.ext.peak_T_scale = T(ex->peak_T_scale == 0.0 ? 1.0 : ex->peak_T_scale);
+        o.ext.jitter_x = T(ex->jitter_x);
+        o.ext.jitter_y = T(ex->jitter_y);
     }
     return o;
 }

@@ -1,18 +1,23 @@
 // singularity_amd/csrc/context.hip — the C ABI's render context: device
-// buffers, the launch of the trace and post-pass kernels, stats and capture.
-// Replaces CudaBackend (render/cuda/cuda_backend.cu:105-166): the context owns
-// every device allocation, render is synchronous, capture copies to caller
-// memory, errors are returned as status codes with a per-context message.
+// buffers, the frame pipeline (trace -> [FP64 pass] -> shade per sample chunk,
+// then the optional post passes), stats and capture.  Replaces CudaBackend
+// (render/cuda/cuda_backend.cu:105-166): the context owns every device
+// allocation, render is synchronous, capture copies to caller memory, errors
+// are returned as status codes with a per-context message.
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/sing_abi.h"
+#include "context.hpp"
 #include "post.hpp"
 #include "scene_const.hpp"
 #include "trace.hpp"
@@ -22,70 +27,113 @@ namespace {
 std::mutex g_err_mu;
 std::string g_last_error;
 
-void set_global_error(const std::string& m) {
+// Device memory budget for one chunk of termination records (16 B per ray).
+constexpr std::size_t kRecordBudget = std::size_t(2) << 30;
+
+}  // namespace
+
+void sing_set_global_error(const std::string& m) {
     std::lock_guard<std::mutex> lk(g_err_mu);
     g_last_error = m;
 }
 
-}  // namespace
+int SingContext::fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    err = buf;
+    sing_set_global_error(err);
+    return code;
+}
 
-struct SingContext {
-    int device = 0;
-    int num_cu = 256;
-    std::uint32_t width = 0, height = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    uchar4* d_rgba = nullptr;
-    float* d_hdr = nullptr;
-    uint2* d_hdr16 = nullptr;
-    std::uint8_t* d_cls = nullptr;
-    std::uint32_t* d_steps = nullptr;
-    float* d_bloom_a = nullptr;  // half-res ping-pong (RGBA16F stored as uint2)
-    float* d_bloom_b = nullptr;
-    sing::QueueCtl* d_ctl = nullptr;
-    float2* d_halton = nullptr;
-    std::uint32_t outputs_valid = 0;  // SING_OUT_* of the last frame
-    std::string err;
+int SingContext::hip_fail(hipError_t e, const char* where) {
+    return fail(SING_ERR_HIP, "%s failed: %s", where, hipGetErrorString(e));
+}
 
-    int fail(int code, const char* fmt, ...) {
-        char buf[512];
-        va_list ap;
-        va_start(ap, fmt);
-        std::vsnprintf(buf, sizeof(buf), fmt, ap);
-        va_end(ap);
-        err = buf;
-        set_global_error(err);
-        return code;
+void SingContext::free_frame() {
+    hipFree(d_rgba);
+    hipFree(d_hdr);
+    hipFree(d_hdr16);
+    hipFree(d_cls);
+    hipFree(d_steps);
+    hipFree(d_bloom_a);
+    hipFree(d_bloom_b);
+    hipFree(d_acc);
+    d_rgba = nullptr;
+    d_hdr = nullptr;
+    d_hdr16 = nullptr;
+    d_cls = nullptr;
+    d_steps = nullptr;
+    d_bloom_a = nullptr;
+    d_bloom_b = nullptr;
+    d_acc = nullptr;
+    outputs_valid = 0;
+}
+
+int SingContext::alloc_frame(std::uint32_t w, std::uint32_t h) {
+    const std::size_t n = std::size_t(w) * h;
+    const std::size_t nb = std::size_t(std::max(w / 2u, 1u)) * std::max(h / 2u, 1u);
+    hipError_t e;
+    if ((e = hipMalloc(&d_rgba, n * 4)) != hipSuccess || (e = hipMalloc(&d_hdr, n * 12)) != hipSuccess ||
+        (e = hipMalloc(&d_hdr16, n * 8)) != hipSuccess || (e = hipMalloc(&d_cls, n)) != hipSuccess ||
+        (e = hipMalloc(&d_steps, n * 4)) != hipSuccess || (e = hipMalloc(&d_bloom_a, nb * 8)) != hipSuccess ||
+        (e = hipMalloc(&d_bloom_b, nb * 8)) != hipSuccess ||
+        (e = hipMalloc(&d_acc, n * sizeof(sing::PixAcc))) != hipSuccess) {
+        free_frame();
+        return fail(SING_ERR_NOMEM, "hipMalloc of %ux%u frame buffers failed: %s", w, h, hipGetErrorString(e));
     }
-    int hip_fail(hipError_t e, const char* where) {
-        return fail(SING_ERR_HIP, "%s failed: %s", where, hipGetErrorString(e));
+    width = w;
+    height = h;
+    return SING_OK;
+}
+
+int SingContext::ensure_records(std::size_t rays) {
+    if (rays <= rec_capacity) return SING_OK;
+    hipFree(d_rec);
+    hipFree(d_band);
+    d_rec = nullptr;
+    d_band = nullptr;
+    rec_capacity = 0;
+    hipError_t e;
+    if ((e = hipMalloc(&d_rec, rays * sizeof(sing::Rec))) != hipSuccess ||
+        (e = hipMalloc(&d_band, rays * sizeof(std::uint32_t))) != hipSuccess) {
+        hipFree(d_rec);
+        d_rec = nullptr;
+        return fail(SING_ERR_NOMEM, "hipMalloc of %zu ray records failed: %s", rays, hipGetErrorString(e));
     }
-    void free_frame() {
-        hipFree(d_rgba); hipFree(d_hdr); hipFree(d_hdr16); hipFree(d_cls); hipFree(d_steps);
-        hipFree(d_bloom_a); hipFree(d_bloom_b);
-        d_rgba = nullptr; d_hdr = nullptr; d_hdr16 = nullptr; d_cls = nullptr; d_steps = nullptr;
-        d_bloom_a = nullptr; d_bloom_b = nullptr;
-        outputs_valid = 0;
+    rec_capacity = rays;
+    return SING_OK;
+}
+
+int SingContext::ensure_events(std::size_t n) {
+    while (events.size() < n) {
+        hipEvent_t e;
+        const hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) return hip_fail(r, "hipEventCreate");
+        events.push_back(e);
     }
-    int alloc_frame(std::uint32_t w, std::uint32_t h) {
-        const std::size_t n = std::size_t(w) * h;
-        const std::size_t nb = std::size_t(w / 2 > 0 ? w / 2 : 1) * (h / 2 > 0 ? h / 2 : 1);
-        hipError_t e;
-        if ((e = hipMalloc(&d_rgba, n * 4)) != hipSuccess ||
-            (e = hipMalloc(&d_hdr, n * 12)) != hipSuccess ||
-            (e = hipMalloc(&d_hdr16, n * 8)) != hipSuccess ||
-            (e = hipMalloc(&d_cls, n)) != hipSuccess ||
-            (e = hipMalloc(&d_steps, n * 4)) != hipSuccess ||
-            (e = hipMalloc(&d_bloom_a, nb * 8)) != hipSuccess ||
-            (e = hipMalloc(&d_bloom_b, nb * 8)) != hipSuccess) {
-            free_frame();
-            return fail(SING_ERR_NOMEM, "hipMalloc of %ux%u frame buffers failed: %s", w, h, hipGetErrorString(e));
-        }
-        width = w;
-        height = h;
-        return SING_OK;
-    }
-};
+    return SING_OK;
+}
+
+std::uint32_t SingContext::chunk_samples(std::uint32_t spp) const {
+    const std::size_t P = std::size_t(width) * height;
+    std::size_t S = kRecordBudget / (P * sizeof(sing::Rec));
+    S = std::max<std::size_t>(S, 1);
+    S = std::min<std::size_t>(S, spp);
+    S = std::min<std::size_t>(S, (std::size_t(1) << 31) / P);  // record indices stay in u32
+    return std::uint32_t(std::max<std::size_t>(S, 1));
+}
+
+sing::LaunchShape SingContext::launch_shape(const SingRenderOpts& o) const {
+    sing::LaunchShape sh{};
+    sh.step_batch = o.step_batch ? o.step_batch : 8u;
+    const std::uint32_t wpc = o.waves_per_cu ? o.waves_per_cu : 16u;
+    sh.blocks = std::uint32_t(num_cu) * ((wpc + 3u) / 4u);
+    sh.blocks64 = std::uint32_t(num_cu) * ((std::max(wpc / 2u, 4u) + 3u) / 4u);
+    return sh;
+}
 
 extern "C" {
 
@@ -100,16 +148,16 @@ const char* sing_last_error(const SingContext* ctx) {
 
 SingContext* sing_create(uint32_t width, uint32_t height, int device) {
     if (width < 1 || height < 1 || width > 32768 || height > 32768) {
-        set_global_error("sing_create: width/height must be in [1, 32768]");
+        sing_set_global_error("sing_create: width/height must be in [1, 32768]");
         return nullptr;
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-        set_global_error("sing_create: no HIP device available");
+        sing_set_global_error("sing_create: no HIP device available");
         return nullptr;
     }
     if (device < 0 || device >= ndev) {
-        set_global_error("sing_create: device index out of range");
+        sing_set_global_error("sing_create: device index out of range");
         return nullptr;
     }
     auto* c = new SingContext();
@@ -117,20 +165,20 @@ SingContext* sing_create(uint32_t width, uint32_t height, int device) {
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
     if (e == hipSuccess) e = hipMalloc(&c->d_ctl, sizeof(sing::QueueCtl));
     if (e == hipSuccess) e = hipMalloc(&c->d_halton, 1024 * sizeof(float2));
     if (e != hipSuccess) {
-        set_global_error(std::string("sing_create: HIP setup failed: ") + hipGetErrorString(e));
+        sing_set_global_error(std::string("sing_create: HIP setup failed: ") + hipGetErrorString(e));
         sing_destroy(c);
         return nullptr;
     }
     // Halton(2,3) offsets of samples 1..1024 (geodesic_kernel.cu:491-495).
-    float2 tab[1024];
-    for (std::uint32_t s = 0; s < 1024; ++s) tab[s] = make_float2(sing::halton_f(s + 1u, 2u), sing::halton_f(s + 1u, 3u));
-    e = hipMemcpy(c->d_halton, tab, sizeof(tab), hipMemcpyHostToDevice);
-    if (e != hipSuccess || c->alloc_frame(width, height) != SING_OK) {
-        set_global_error(e != hipSuccess ? std::string("sing_create: halton upload failed") : c->err);
+    std::vector<float2> tab(1024);
+    for (std::uint32_t s = 0; s < 1024; ++s)
+        tab[s] = make_float2(sing::halton_f(s + 1u, 2u), sing::halton_f(s + 1u, 3u));
+    e = hipMemcpy(c->d_halton, tab.data(), tab.size() * sizeof(float2), hipMemcpyHostToDevice);
+    if (e != hipSuccess || c->alloc_frame(width, height) != SING_OK || c->ensure_events(8) != SING_OK) {
+        sing_set_global_error(e != hipSuccess ? std::string("sing_create: halton upload failed") : c->err);
         sing_destroy(c);
         return nullptr;
     }
@@ -144,8 +192,9 @@ void sing_destroy(SingContext* c) {
     c->free_frame();
     hipFree(c->d_ctl);
     hipFree(c->d_halton);
-    for (auto& e : c->ev)
-        if (e) hipEventDestroy(e);
+    hipFree(c->d_rec);
+    hipFree(c->d_band);
+    for (auto& e : c->events) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -170,7 +219,11 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
     if (opts_in) o = *opts_in;
     if (o.precision > SING_PREC_MIXED) return c->fail(SING_ERR_ARG, "sing_render: unknown precision %u", o.precision);
     if (o.post > SING_POST_BLOOM) return c->fail(SING_ERR_ARG, "sing_render: unknown post pass %u", o.post);
-    if (u->metric_type > SING_METRIC_KERR) return c->fail(SING_ERR_ARG, "sing_render: unknown metric %u", u->metric_type);
+    if (u->metric_type > SING_METRIC_KERR)
+        return c->fail(SING_ERR_ARG, "sing_render: unknown metric %u", u->metric_type);
+    if (o.adapt_tol > 0.0f) return c->fail(SING_ERR_ARG, "sing_render: adapt_tol needs sing_render_adaptive");
+    if (o.post == SING_POST_BLOOM && (c->width < 2 || c->height < 2))
+        return c->fail(SING_ERR_ARG, "sing_render: bloom needs at least 2x2 pixels");
     std::uint32_t outs = o.outputs ? o.outputs : SING_OUT_RGBA8;
     if (o.post == SING_POST_BLOOM) outs |= SING_OUT_HDR16 | SING_OUT_RGBA8;
     hipError_t e = hipSetDevice(c->device);
@@ -186,48 +239,85 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
     to.cls = (outs & SING_OUT_CLASS) ? c->d_cls : nullptr;
     to.steps = (outs & SING_OUT_STEPS) ? c->d_steps : nullptr;
 
-    sing::LaunchShape shape{};
-    shape.precision = o.precision;
-    shape.step_batch = o.step_batch ? o.step_batch : 8u;
-    const std::uint32_t waves_per_cu = o.waves_per_cu ? o.waves_per_cu : 16u;
-    shape.blocks = std::uint32_t(c->num_cu) * ((waves_per_cu + 3u) / 4u);
+    const std::uint32_t spp = sf.spp;
+    const std::uint32_t S = c->chunk_samples(spp);
+    const std::uint32_t n_chunks = (spp + S - 1u) / S;
+    const std::size_t P = std::size_t(c->width) * c->height;
+    int rc = c->ensure_records(std::size_t(S) * P);
+    if (rc != SING_OK) return rc;
+    if ((rc = c->ensure_events(4u * n_chunks + 4u)) != SING_OK) return rc;
+
+    sing::Chunk ch{};
+    ch.rec = c->d_rec;
+    ch.band = c->d_band;
+    ch.P = std::uint32_t(P);
+    ch.tiles_x = (c->width + 7u) / 8u;
+    ch.n_tiles = ch.tiles_x * ((c->height + 7u) / 8u);
+    const sing::LaunchShape shape = c->launch_shape(o);
 
     if ((e = hipMemsetAsync(c->d_ctl, 0, sizeof(sing::QueueCtl), c->stream)) != hipSuccess)
         return c->hip_fail(e, "hipMemsetAsync(queue)");
-    hipEventRecord(c->ev[0], c->stream);
-    if ((e = sing::launch_trace(sf, sd, c->d_halton, to, c->d_ctl, shape, c->stream)) != hipSuccess)
-        return c->hip_fail(e, "trace kernel launch");
-    hipEventRecord(c->ev[1], c->stream);
+    hipEvent_t* ev = c->events.data();
+    for (std::uint32_t k = 0; k < n_chunks; ++k) {
+        ch.s0 = k * S;
+        ch.S = std::min(S, spp - ch.s0);
+        // the work queues restart per chunk; the frame counters keep accumulating
+        if (k > 0 && (e = hipMemsetAsync(c->d_ctl, 0, offsetof(sing::QueueCtl, steps), c->stream)) != hipSuccess)
+            return c->hip_fail(e, "hipMemsetAsync(queue)");
+        hipEventRecord(ev[4 * k + 0], c->stream);
+        if ((e = sing::launch_trace_chunk(sf, sd, c->d_halton, ch, c->d_ctl, o.precision, shape, c->stream)) !=
+            hipSuccess)
+            return c->hip_fail(e, "trace kernel launch");
+        hipEventRecord(ev[4 * k + 1], c->stream);
+        if ((e = sing::launch_shade_chunk(sf, ch, c->d_acc, k == 0, k + 1 == n_chunks, to, c->stream)) != hipSuccess)
+            return c->hip_fail(e, "shade kernel launch");
+        hipEventRecord(ev[4 * k + 2], c->stream);
+    }
+    hipEvent_t post0 = ev[4 * n_chunks], post1 = ev[4 * n_chunks + 1];
     if (o.post == SING_POST_BLOOM) {
-        hipEventRecord(c->ev[2], c->stream);
+        hipEventRecord(post0, c->stream);
         e = sing::launch_bloom(c->d_hdr16, c->width, c->height, u->bloom_threshold, u->bloom_strength, u->exposure,
                                reinterpret_cast<uint2*>(c->d_bloom_a), reinterpret_cast<uint2*>(c->d_bloom_b),
                                c->d_rgba, c->stream);
         if (e != hipSuccess) return c->hip_fail(e, "bloom launch");
-        hipEventRecord(c->ev[3], c->stream);
+        hipEventRecord(post1, c->stream);
     }
     sing::QueueCtl ctl{};
     if ((e = hipMemcpyAsync(&ctl, c->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, c->stream)) != hipSuccess)
         return c->hip_fail(e, "hipMemcpyAsync(stats)");
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return c->hip_fail(e, "hipStreamSynchronize");
     c->outputs_valid = outs;
+    double trace_ms = 0.0, shade_ms = 0.0, post_ms = 0.0;
+    for (std::uint32_t k = 0; k < n_chunks; ++k) {
+        float a = 0.0f, b = 0.0f;
+        hipEventElapsedTime(&a, ev[4 * k + 0], ev[4 * k + 1]);
+        hipEventElapsedTime(&b, ev[4 * k + 1], ev[4 * k + 2]);
+        trace_ms += a;
+        shade_ms += b;
+    }
+    float total = 0.0f;
+    hipEventElapsedTime(&total, ev[0], ev[4 * (n_chunks - 1) + 2]);
+    if (o.post == SING_POST_BLOOM) {
+        float p = 0.0f;
+        hipEventElapsedTime(&p, post0, post1);
+        post_ms = p;
+        hipEventElapsedTime(&total, ev[0], post1);
+    }
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
-        float ms = 0.0f;
-        hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
-        stats->kernel_ms = ms;
-        if (o.post == SING_POST_BLOOM) {
-            hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
-            stats->post_ms = ms;
-        }
+        stats->kernel_ms = total;
+        stats->trace_ms = trace_ms;
+        stats->shade_ms = shade_ms;
+        stats->post_ms = post_ms;
         stats->rays = ctl.rays;
         stats->steps = ctl.steps;
         stats->fp64_rays = ctl.rays64;
+        stats->chunks = n_chunks;
     }
-    const std::uint64_t expect = std::uint64_t(c->width) * c->height * sf.spp;
+    const std::uint64_t expect = std::uint64_t(P) * spp;
     if (ctl.rays != expect)
-        return c->fail(SING_ERR_STATE, "sing_render: traced %llu rays, expected %llu",
-                       (unsigned long long)ctl.rays, (unsigned long long)expect);
+        return c->fail(SING_ERR_STATE, "sing_render: traced %llu rays, expected %llu", (unsigned long long)ctl.rays,
+                       (unsigned long long)expect);
     return SING_OK;
 }
 

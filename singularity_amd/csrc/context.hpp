@@ -1,0 +1,49 @@
+// singularity_amd/csrc/context.hpp — the render context behind the C ABI's
+// opaque SingContext (shared by the frame, adjoint and batch entry points).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/sing_abi.h"
+#include "trace.hpp"
+
+void sing_set_global_error(const std::string& m);
+
+struct SingContext {
+    int device = 0;
+    int num_cu = 256;
+    std::uint32_t width = 0, height = 0;
+    hipStream_t stream = nullptr;
+    std::vector<hipEvent_t> events;
+    // frame outputs
+    uchar4* d_rgba = nullptr;
+    float* d_hdr = nullptr;
+    uint2* d_hdr16 = nullptr;
+    std::uint8_t* d_cls = nullptr;
+    std::uint32_t* d_steps = nullptr;
+    float* d_bloom_a = nullptr;  // half-res ping-pong (RGBA16F stored as uint2)
+    float* d_bloom_b = nullptr;
+    sing::PixAcc* d_acc = nullptr;
+    // per-chunk ray records and the deferred (FP64) list
+    sing::Rec* d_rec = nullptr;
+    std::uint32_t* d_band = nullptr;
+    std::size_t rec_capacity = 0;
+    sing::QueueCtl* d_ctl = nullptr;
+    float2* d_halton = nullptr;
+    std::uint32_t outputs_valid = 0;  // SING_OUT_* of the last frame
+    std::string err;
+
+    int fail(int code, const char* fmt, ...);
+    int hip_fail(hipError_t e, const char* where);
+    void free_frame();
+    int alloc_frame(std::uint32_t w, std::uint32_t h);
+    int ensure_records(std::size_t rays);
+    int ensure_events(std::size_t n);
+    std::uint32_t chunk_samples(std::uint32_t spp) const;
+    sing::LaunchShape launch_shape(const SingRenderOpts& o) const;
+};

@@ -20,8 +20,58 @@
 namespace sing {
 
 // ---- scalar helpers ---------------------------------------------------------
-__device__ __forceinline__ void dsincos(float x, float* s, float* c) { sincosf(x, s, c); }
-__device__ __forceinline__ void dsincos(double x, double* s, double* c) { sincos(x, s, c); }
+// sin and cos of the polar angle on the step path: Cody-Waite reduction by
+// pi/2 (two-term constant, exact with FMA for the |x| < 2^20 this path sees)
+// to r in [-pi/4, pi/4], Taylor polynomials of sin r and cos r (coefficients
+// (-1)^n/n!; truncation below half an ulp on that interval), quadrant swap.
+// Branch-free and short-lived in registers, unlike a general-argument sincos.
+__device__ __forceinline__ void dsincos(float x, float* s, float* c) {
+    const float k = rintf(x * 0.636619772367581343f);
+    float r = fmaf(-k, 1.5707963705062866f, x);
+    r = fmaf(-k, -4.371138828673793e-08f, r);
+    const float r2 = r * r;
+    float ps = fmaf(r2, 2.7557319e-06f, -1.9841270e-04f);
+    ps = fmaf(r2, ps, 8.3333338e-03f);
+    ps = fmaf(r2, ps, -1.6666667e-01f);
+    const float sr = fmaf(r * r2, ps, r);
+    float pc = fmaf(r2, -2.7557320e-07f, 2.4801588e-05f);
+    pc = fmaf(r2, pc, -1.3888889e-03f);
+    pc = fmaf(r2, pc, 4.1666668e-02f);
+    pc = fmaf(r2, pc, -0.5f);
+    const float cr = fmaf(r2, pc, 1.0f);
+    const int q = int(k);
+    float ss = (q & 1) ? cr : sr;
+    float cc = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -ss : ss;
+    *c = ((q + 1) & 2) ? -cc : cc;
+}
+__device__ __forceinline__ void dsincos(double x, double* s, double* c) {
+    const double k = rint(x * 0.63661977236758134);
+    double r = fma(-k, 1.5707963267948966, x);
+    r = fma(-k, 6.123233995736766e-17, r);
+    const double r2 = r * r;
+    double ps = fma(r2, 2.8114572543455206e-15, -7.647163731819816e-13);
+    ps = fma(r2, ps, 1.6059043836821613e-10);
+    ps = fma(r2, ps, -2.505210838544172e-08);
+    ps = fma(r2, ps, 2.7557319223985893e-06);
+    ps = fma(r2, ps, -0.0001984126984126984);
+    ps = fma(r2, ps, 0.008333333333333333);
+    ps = fma(r2, ps, -0.16666666666666666);
+    const double sr = fma(r * r2, ps, r);
+    double pc = fma(r2, 4.779477332387385e-14, -1.1470745597729725e-11);
+    pc = fma(r2, pc, 2.08767569878681e-09);
+    pc = fma(r2, pc, -2.755731922398589e-07);
+    pc = fma(r2, pc, 2.48015873015873e-05);
+    pc = fma(r2, pc, -0.001388888888888889);
+    pc = fma(r2, pc, 0.041666666666666664);
+    pc = fma(r2, pc, -0.5);
+    const double cr = fma(r2, pc, 1.0);
+    const int q = int(k);
+    double ss = (q & 1) ? cr : sr;
+    double cc = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -ss : ss;
+    *c = ((q + 1) & 2) ? -cc : cc;
+}
 __device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float dlog(float x) { return logf(x); }
@@ -343,21 +393,21 @@ __device__ __forceinline__ void camera_ray(const SceneConst<R>& sc, std::uint32_
 // rounding exponentially) when the local minimum of R outside the horizon is
 // close to zero.  Returns |R(r*)| / r*^4 at the largest stationary point r*,
 // or a large value when R has no such minimum outside r+.
-__device__ __forceinline__ double photon_margin(double xi, double eta, double a, double M, double r_plus) {
-    const double C = (xi - a) * (xi - a) + eta;
-    const double P = 0.25 * 2.0 * (a * a - xi * xi - eta);  // R'/4 = r^3 + P r + Qc
-    const double Qc = 0.5 * M * C;
-    if (!(P < 0.0)) return 1e30;
-    const double disc = -(4.0 * P * P * P + 27.0 * Qc * Qc);
-    if (!(disc > 0.0)) return 1e30;
-    const double m = 2.0 * sqrt(-P / 3.0);
-    double arg = (3.0 * Qc / (2.0 * P)) * sqrt(-3.0 / P);
-    arg = fmin(fmax(arg, -1.0), 1.0);
-    const double rs = m * cos(acos(arg) / 3.0);
-    if (!(rs > r_plus)) return 1e30;
-    const double r2 = rs * rs;
-    const double Rv = r2 * r2 + (a * a - xi * xi - eta) * r2 + 2.0 * M * C * rs - a * a * eta;
-    return fabs(Rv) / (r2 * r2);
+template <class R> __device__ __forceinline__ R photon_margin(R xi, R eta, R a, R M, R r_plus) {
+    const R C = (xi - a) * (xi - a) + eta;
+    const R P = R(0.5) * (a * a - xi * xi - eta);  // R'/4 = r^3 + P r + Qc
+    const R Qc = R(0.5) * M * C;
+    if (!(P < R(0))) return R(1e30);
+    const R disc = -(R(4) * P * P * P + R(27) * Qc * Qc);
+    if (!(disc > R(0))) return R(1e30);
+    const R m = R(2) * dsqrt(-P / R(3));
+    const R arg = dclamp((R(3) * Qc / (R(2) * P)) * dsqrt(R(-3) / P), R(-1), R(1));
+    const R rs = m * dcos(dacos(arg) / R(3));
+    if (!(rs > r_plus)) return R(1e30);
+    const R r2 = rs * rs;
+    const R Rv = r2 * r2 + (a * a - xi * xi - eta) * r2 + R(2) * M * C * rs - a * a * eta;
+    const R v = Rv / (r2 * r2);
+    return v < R(0) ? -v : v;
 }
 
 }  // namespace sing

@@ -1,17 +1,17 @@
-// singularity_amd/csrc/trace.hip — the persistent wave64 trace kernels.
+// singularity_amd/csrc/trace.hip — trace and shade kernels of the frame pipeline.
 //
-// One launch renders a frame.  The grid is persistent: each lane owns one
-// pixel at a time and integrates that pixel's samples back to back, in sample
-// order, so the per-pixel mean is accumulated in a fixed order (no float
-// atomics) exactly like the reference's per-pixel sample loop
-// (render/cuda/kernels/geodesic_kernel.cu:462-510).  When a lane finishes its
-// pixel it takes the next one from a global queue; the wave claims new pixels
-// for all its idle lanes with a single atomic, so long rays never hold the
-// other 63 lanes of the wave hostage.  Rays are advanced in batches of
-// `step_batch` RK4 steps between refills.
+// Trace: a persistent wave64 kernel.  Each lane integrates one ray (one
+// (pixel, sample) pair) at a time; after every `step_batch` RK4 steps the lanes
+// whose ray terminated write its 16-byte record and the wave claims new rays
+// for all of them with a single atomic, so long rays never hold the other 63
+// lanes of the wave hostage.  The lean step loop carries only the ray state;
+// all shading lives in the shade kernel.
 //
-// PREC: 0 = FP32, 1 = FP64, 2 = mixed (FP32 bulk, FP64 for rays whose
-// photon-orbit margin is below the band threshold).
+// Shade: one thread per pixel walks its records in sample order and
+// accumulates exactly like the reference's per-pixel sample loop
+// (render/cuda/kernels/geodesic_kernel.cu:462-510): per-sample ACES of the
+// exposed radiance, mean over samples, sRGB, RGBA8.  The fixed order keeps
+// frames bit-reproducible with no float atomics.
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 
@@ -27,9 +27,9 @@ namespace {
 
 template <class S> struct RayState {
     S r, th, ph;
-    S v0, v1, v2, v3;     // Kerr: p_r, p_theta, E, L_z.  Schwarzschild: u^t, u^r, u^theta, u^phi
+    S v0, v1, v2, v3;  // Kerr: p_r, p_theta, E, L_z.  Schwarzschild: u^t, u^r, u^theta, u^phi
     S prev_r, prev_th, prev_uph;
-    S aux0, aux1;         // disc crossing radius, u^phi at the crossing (Schwarzschild)
+    S aux0, aux1;      // disc crossing radius, u^phi at the crossing (Schwarzschild)
     std::uint32_t steps;
 };
 
@@ -92,9 +92,10 @@ __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneCons
 
 // Ray initialisation from the camera (geodesic_kernel.cu:229-242, 303-347).
 // Returns false on a Kerr initialisation failure (no real root at the camera).
+// With margin_out, also evaluates the photon-orbit margin of the ray.
 template <class R, class S, bool KERR>
 __device__ __forceinline__ bool init_ray(RayState<S>& y, const SceneConst<R>& sc, std::uint32_t gx, std::uint32_t gy,
-                                         R ox, R oy, double* margin_out) {
+                                         R ox, R oy, R* margin_out) {
     R ur, uth, uph;
     camera_ray(sc, gx, gy, ox, oy, ur, uth, uph);
     y.r = S(sc.cam_r);
@@ -117,11 +118,9 @@ __device__ __forceinline__ bool init_ray(RayState<S>& y, const SceneConst<R>& sc
         y.v2 = S(E);
         y.v3 = S(L);
         if (margin_out) {
-            const double Ed = double(E), Ld = double(L), ptd = double(pt);
-            const double Q = ptd * ptd + double(sc.q_cos2) * (Ld * Ld * double(sc.q_inv_s2) - double(sc.a2) * Ed * Ed);
-            const double a = double(sc.a), M = double(sc.M);
-            const double rp = M + sqrt(fmax(M * M - a * a, 0.0));
-            *margin_out = photon_margin(Ld / Ed, Q / (Ed * Ed), a, M, rp);
+            const R Q = pt * pt + sc.q_cos2 * (L * L * sc.q_inv_s2 - sc.a2 * E * E);
+            const R rp = sc.M + dsqrt(dmax(sc.M * sc.M - sc.a2, R(0)));
+            *margin_out = photon_margin<R>(L / E, Q / (E * E), sc.a, sc.M, rp);
         }
     } else {
         const R cr = sc.cam_r;
@@ -133,24 +132,159 @@ __device__ __forceinline__ bool init_ray(RayState<S>& y, const SceneConst<R>& sc
         y.v3 = S(uph);
         y.prev_uph = y.v3;
         if (margin_out) {
-            const double crd = double(cr), s = double(sc.sin_th);
-            const double E = double(sc.f_cam) * double(ut);
-            const double L2 = crd * crd * crd * crd * (double(uth) * double(uth) + s * s * double(uph) * double(uph));
-            *margin_out = photon_margin(0.0, L2 / (E * E), 0.0, double(sc.M), 2.0 * double(sc.M));
+            const R E = sc.f_cam * ut;
+            const R L2 = cr * cr * cr * cr * (uth * uth + sc.sin_th * sc.sin_th * uph * uph);
+            *margin_out = photon_margin<R>(R(0), L2 / (E * E), R(0), sc.M, sc.twoM);
         }
     }
     return true;
 }
 
-template <class R, class S, bool KERR>
-__device__ __forceinline__ void shade(const RayState<S>& y, std::uint32_t term, const SceneConst<R>& sc, R* rgb) {
-    rgb[0] = rgb[1] = rgb[2] = R(0);
-    if (term == kTermEscape) {
-        shade_escape(sc, R(y.th), R(y.ph), rgb);
-    } else if (term == kTermDisc) {
-        // Kerr: phi at the crossing is the current phi (geodesic_kernel.cu:389);
-        // Schwarzschild: lerp(phi, phi) == phi (geodesic_kernel.cu:282).
-        shade_disc<R, KERR>(sc, R(y.aux0), R(y.ph), KERR ? R(y.v3) : R(y.aux1), rgb);
+template <class S, bool KERR>
+__device__ __forceinline__ Rec make_rec(const RayState<S>& y, std::uint32_t term) {
+    Rec rc;
+    rc.meta = term | (y.steps << 4);
+    if (term == kTermDisc) {
+        rc.a = float(y.aux0);
+        rc.b = float(y.ph);
+        rc.c = KERR ? float(y.v3) : float(y.aux1);
+    } else {
+        rc.a = float(y.th);
+        rc.b = float(y.ph);
+        rc.c = 0.0f;
+    }
+    return rc;
+}
+
+// Work item -> (record index, pixel, sample).  Items enumerate the chunk's
+// samples sample-major, pixels in 8x8 tiles so a wave starts on a compact tile.
+__device__ __forceinline__ bool item_to_ray(std::uint64_t item, const Chunk& ch, std::uint32_t W, std::uint32_t H,
+                                            std::uint32_t& rec_idx, std::uint32_t& gx, std::uint32_t& gy,
+                                            std::uint32_t& s) {
+    const std::uint64_t per_s = std::uint64_t(ch.n_tiles) * 64u;
+    const std::uint32_t sl = std::uint32_t(item / per_s);
+    const std::uint32_t rem = std::uint32_t(item - std::uint64_t(sl) * per_s);
+    const std::uint32_t tile = rem >> 6, w = rem & 63u;
+    gx = (tile % ch.tiles_x) * 8u + (w & 7u);
+    gy = (tile / ch.tiles_x) * 8u + (w >> 3);
+    s = ch.s0 + sl;
+    rec_idx = sl * ch.P + gy * W + gx;
+    return gx < W && gy < H;
+}
+
+// PASS 0: FP32 trace of every ray; 1: FP32 trace deferring photon-band rays;
+// 2: FP64 trace of every ray; 3: FP64 trace of the deferred list.
+template <int METRIC, int PASS>
+__global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneConst<double> sd,
+                                                    const float2* __restrict__ halton, Chunk ch,
+                                                    QueueCtl* __restrict__ ctl, std::uint32_t step_batch) {
+    constexpr bool KERR = (METRIC == 1);
+    constexpr bool F64 = (PASS >= 2);
+    using S = typename std::conditional<F64, double, float>::type;
+    const int lane = __lane_id();
+    const std::uint32_t W = sf.width, H = sf.height, spp = sf.spp;
+    const std::uint64_t n_items =
+        (PASS == 3) ? std::uint64_t(ctl->band_count) : std::uint64_t(ch.S) * ch.n_tiles * 64u;
+    std::uint64_t* counter = (PASS == 3) ? &ctl->next_band : &ctl->next_item;
+
+    bool retired = false, live = false, need = true;
+    std::uint32_t rec_idx = 0;
+    std::uint64_t lane_steps = 0, lane_rays = 0;
+    RayState<S> y{};
+
+    for (;;) {
+        // (1) claim rays for idle lanes with one atomic per wave
+        const std::uint64_t want = __ballot(need && !retired);
+        if (want) {
+            const int leader = __ffsll((unsigned long long)want) - 1;
+            std::uint64_t base = 0;
+            if (lane == leader) base = atomicAdd((unsigned long long*)counter, (unsigned long long)__popcll(want));
+            base = shfl_u64(base, leader);
+            if (need && !retired) {
+                const std::uint64_t item = base + __popcll(want & ((1ull << lane) - 1ull));
+                if (item >= n_items) {
+                    retired = true;
+                } else {
+                    std::uint32_t gx, gy, s;
+                    bool ok;
+                    if (PASS == 3) {
+                        rec_idx = ch.band[item];
+                        const std::uint32_t sl = rec_idx / ch.P, pix = rec_idx - sl * ch.P;
+                        gx = pix % W;
+                        gy = pix / W;
+                        s = ch.s0 + sl;
+                        ok = true;
+                    } else {
+                        ok = item_to_ray(item, ch, W, H, rec_idx, gx, gy, s);
+                    }
+                    if (ok) {
+                        float ox = 0.5f, oy = 0.5f;
+                        if (spp > 1u) {
+                            const float2 hv = halton[s];
+                            ox = hv.x;
+                            oy = hv.y;
+                        }
+                        bool started;
+                        bool deferred = false;
+                        if (F64) {
+                            started = init_ray<double, S, KERR>(y, sd, gx, gy, double(ox), double(oy), (double*)nullptr);
+                        } else if (PASS == 1) {
+                            float margin = 1e30f;
+                            started = init_ray<float, S, KERR>(y, sf, gx, gy, ox, oy, &margin);
+                            deferred = started && margin < sf.band;
+                        } else {
+                            started = init_ray<float, S, KERR>(y, sf, gx, gy, ox, oy, (float*)nullptr);
+                        }
+                        need = false;
+                        if (deferred) {
+                            const std::uint32_t slot = atomicAdd(&ctl->band_count, 1u);
+                            ch.band[slot] = rec_idx;
+                            need = true;
+                        } else if (!started) {
+                            Rec rc;
+                            rc.a = rc.b = rc.c = 0.0f;
+                            rc.meta = kTermInitFail;
+                            ch.rec[rec_idx] = rc;
+                            lane_rays += 1;
+                            need = true;
+                        } else {
+                            live = true;
+                        }
+                    }
+                }
+            }
+        }
+        if (__ballot(!retired) == 0ull) break;
+
+        // (2) integrate a batch of steps
+        std::uint32_t term = kTermLive;
+        for (std::uint32_t k = 0; k < step_batch; ++k) {
+            if (live) {
+                const std::uint32_t t = F64 ? advance<double, S, KERR>(y, sd) : advance<float, S, KERR>(y, sf);
+                if (t != kTermLive) {
+                    live = false;
+                    term = t;
+                }
+            }
+        }
+
+        // (3) emit finished rays
+        if (term != kTermLive) {
+            ch.rec[rec_idx] = make_rec<S, KERR>(y, term);
+            lane_steps += y.steps;
+            lane_rays += 1;
+            need = true;
+        }
+    }
+
+    for (int off = 32; off > 0; off >>= 1) {
+        lane_steps += shfl_u64(lane_steps, lane ^ off);
+        lane_rays += shfl_u64(lane_rays, lane ^ off);
+    }
+    if (lane == 0 && lane_rays) {
+        atomicAdd((unsigned long long*)&ctl->steps, (unsigned long long)lane_steps);
+        atomicAdd((unsigned long long*)&ctl->rays, (unsigned long long)lane_rays);
+        if (F64) atomicAdd((unsigned long long*)&ctl->rays64, (unsigned long long)lane_rays);
     }
 }
 
@@ -164,168 +298,100 @@ __device__ __forceinline__ std::uint8_t class_of(std::uint32_t term) {
     }
 }
 
-template <int METRIC, int PREC>
-__global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneConst<double> sd,
-                                                    const float2* __restrict__ halton, TraceOutputs out,
-                                                    QueueCtl* __restrict__ ctl, std::uint32_t n_items,
-                                                    std::uint32_t tiles_x, std::uint32_t step_batch) {
+template <int METRIC>
+__global__ __launch_bounds__(256) void shade_kernel(SceneConst<float> sc, const Rec* __restrict__ rec,
+                                                    PixAcc* __restrict__ acc, std::uint32_t P, std::uint32_t S,
+                                                    std::uint32_t first, std::uint32_t last, TraceOutputs out) {
     constexpr bool KERR = (METRIC == 1);
-    using S = typename std::conditional<PREC == 0, float, double>::type;   // ray storage
-    using A = S;                                                            // accumulators
-    const int lane = __lane_id();
-    const std::uint32_t W = sf.width, H = sf.height, spp = sf.spp;
-
-    bool retired = false, need_pixel = true, live = false, use64 = (PREC == 1);
-    std::uint32_t term = kTermLive, gx = 0, gy = 0, s = 0, px_steps = 0;
-    std::uint8_t cls0 = 0;
-    A ldr0 = A(0), ldr1 = A(0), ldr2 = A(0), hdr0 = A(0), hdr1 = A(0), hdr2 = A(0);
-    std::uint64_t lane_steps = 0, lane_rays = 0, lane_rays64 = 0;
-    RayState<S> y{};
-
-    for (;;) {
-        // (1) claim pixels for idle lanes with one atomic per wave
-        const std::uint64_t need = __ballot(need_pixel && !retired);
-        if (need) {
-            const int leader = __ffsll((unsigned long long)need) - 1;
-            std::uint64_t base = 0;
-            if (lane == leader) base = atomicAdd((unsigned long long*)&ctl->next_item, (unsigned long long)__popcll(need));
-            base = shfl_u64(base, leader);
-            if (need_pixel && !retired) {
-                const std::uint64_t item = base + __popcll(need & ((1ull << lane) - 1ull));
-                if (item >= n_items) {
-                    retired = true;
-                } else {
-                    const std::uint32_t tile = std::uint32_t(item >> 6), w = std::uint32_t(item & 63u);
-                    gx = (tile % tiles_x) * 8u + (w & 7u);
-                    gy = (tile / tiles_x) * 8u + (w >> 3);
-                    if (gx < W && gy < H) {
-                        need_pixel = false;
-                        s = 0;
-                        px_steps = 0;
-                        ldr0 = ldr1 = ldr2 = hdr0 = hdr1 = hdr2 = A(0);
-                    }
-                }
-            }
-        }
-        if (__ballot(!retired) == 0ull) break;
-
-        // (2) start the next sample of the lane's pixel
-        if (!retired && !need_pixel && !live) {
-            float ox = 0.5f, oy = 0.5f;
-            if (spp > 1u) {
-                const float2 hv = halton[s];
-                ox = hv.x;
-                oy = hv.y;
-            }
-            bool ok;
-            if (PREC == 0) {
-                ok = init_ray<float, S, KERR>(y, sf, gx, gy, ox, oy, nullptr);
-            } else if (PREC == 1) {
-                ok = init_ray<double, S, KERR>(y, sd, gx, gy, double(ox), double(oy), nullptr);
-            } else {
-                double margin = 1e30;
-                ok = init_ray<double, S, KERR>(y, sd, gx, gy, double(ox), double(oy), &margin);
-                use64 = margin < sd.band;
-            }
-            live = ok;
-            term = ok ? kTermLive : kTermInitFail;
-        }
-
-        // (3) integrate a batch of steps
-        for (std::uint32_t k = 0; k < step_batch; ++k) {
-            if (live) {
-                std::uint32_t t;
-                if (PREC == 0) t = advance<float, S, KERR>(y, sf);
-                else if (PREC == 1) t = advance<double, S, KERR>(y, sd);
-                else t = use64 ? advance<double, S, KERR>(y, sd) : advance<float, S, KERR>(y, sf);
-                if (t != kTermLive) {
-                    live = false;
-                    term = t;
-                }
-            }
-        }
-
-        // (4) shade and accumulate finished rays, emit finished pixels
-        if (!retired && !need_pixel && !live) {
-            A rgb[3];
-            if (PREC == 0) {
-                float c[3];
-                shade<float, S, KERR>(y, term, sf, c);
-                rgb[0] = c[0]; rgb[1] = c[1]; rgb[2] = c[2];
-            } else {
-                double c[3];
-                shade<double, S, KERR>(y, term, sd, c);
-                rgb[0] = A(c[0]); rgb[1] = A(c[1]); rgb[2] = A(c[2]);
-            }
-            const A ex = (PREC == 0) ? A(sf.exposure) : A(sd.exposure);
-            hdr0 += rgb[0];
-            hdr1 += rgb[1];
-            hdr2 += rgb[2];
-            ldr0 += aces1(rgb[0] * ex);
-            ldr1 += aces1(rgb[1] * ex);
-            ldr2 += aces1(rgb[2] * ex);
-            if (s == 0) cls0 = class_of(term);
-            px_steps += y.steps;
-            lane_steps += y.steps;
-            lane_rays += 1;
-            if (PREC == 2 && use64) lane_rays64 += 1;
-            if (PREC == 1) lane_rays64 += 1;
-            term = kTermLive;
-            s += 1;
-            if (s == spp) {
-                const std::uint32_t pix = gy * W + gx;
-                const A inv = (spp > 1u) ? A(1) / A(spp) : A(1);
-                const A l0 = ldr0 * inv, l1 = ldr1 * inv, l2 = ldr2 * inv;
-                if (out.rgba) out.rgba[pix] = make_uchar4(to_u8(l0), to_u8(l1), to_u8(l2), 255);
-                const float h0 = float(hdr0 * inv), h1 = float(hdr1 * inv), h2 = float(hdr2 * inv);
-                if (out.hdr) {
-                    out.hdr[3u * pix + 0u] = h0;
-                    out.hdr[3u * pix + 1u] = h1;
-                    out.hdr[3u * pix + 2u] = h2;
-                }
-                if (out.hdr16) {
-                    const __half2 ab = __floats2half2_rn(h0, h1);
-                    const __half2 cd = __floats2half2_rn(h2, 1.0f);
-                    out.hdr16[pix] = make_uint2(*reinterpret_cast<const std::uint32_t*>(&ab),
-                                                *reinterpret_cast<const std::uint32_t*>(&cd));
-                }
-                if (out.cls) out.cls[pix] = cls0;
-                if (out.steps) out.steps[pix] = px_steps;
-                need_pixel = true;
-            }
-        }
+    const std::uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    if (pix >= P) return;
+    float l0 = 0.0f, l1 = 0.0f, l2 = 0.0f, h0 = 0.0f, h1 = 0.0f, h2 = 0.0f;
+    std::uint32_t steps = 0, cls = 0;
+    if (!first) {
+        const PixAcc a = acc[pix];
+        l0 = a.ldr[0]; l1 = a.ldr[1]; l2 = a.ldr[2];
+        h0 = a.hdr[0]; h1 = a.hdr[1]; h2 = a.hdr[2];
+        steps = a.steps;
+        cls = a.cls;
     }
-
-    // per-wave totals
-    for (int off = 32; off > 0; off >>= 1) {
-        lane_steps += shfl_u64(lane_steps, lane ^ off);
-        lane_rays += shfl_u64(lane_rays, lane ^ off);
-        lane_rays64 += shfl_u64(lane_rays64, lane ^ off);
+    const float ex = sc.exposure;
+    for (std::uint32_t s = 0; s < S; ++s) {
+        const Rec r = rec[std::size_t(s) * P + pix];
+        const std::uint32_t term = r.meta & 7u;
+        float c[3] = {0.0f, 0.0f, 0.0f};
+        if (term == kTermEscape) shade_escape<float>(sc, r.a, r.b, c);
+        else if (term == kTermDisc) shade_disc<float, KERR>(sc, r.a, r.b, r.c, c);
+        h0 += c[0];
+        h1 += c[1];
+        h2 += c[2];
+        l0 += aces1(c[0] * ex);
+        l1 += aces1(c[1] * ex);
+        l2 += aces1(c[2] * ex);
+        steps += r.meta >> 4;
+        if (first && s == 0) cls = class_of(term);
     }
-    if (lane == 0) {
-        atomicAdd((unsigned long long*)&ctl->steps, (unsigned long long)lane_steps);
-        atomicAdd((unsigned long long*)&ctl->rays, (unsigned long long)lane_rays);
-        atomicAdd((unsigned long long*)&ctl->rays64, (unsigned long long)lane_rays64);
+    if (!last) {
+        PixAcc a;
+        a.ldr[0] = l0; a.ldr[1] = l1; a.ldr[2] = l2;
+        a.hdr[0] = h0; a.hdr[1] = h1; a.hdr[2] = h2;
+        a.steps = steps;
+        a.cls = cls;
+        acc[pix] = a;
+        return;
     }
+    const std::uint32_t spp = sc.spp;
+    if (spp > 1u) {
+        const float inv = 1.0f / float(spp);
+        l0 *= inv; l1 *= inv; l2 *= inv;
+        h0 *= inv; h1 *= inv; h2 *= inv;
+    }
+    if (out.rgba) out.rgba[pix] = make_uchar4(to_u8(l0), to_u8(l1), to_u8(l2), 255);
+    if (out.hdr) {
+        out.hdr[3u * pix + 0u] = h0;
+        out.hdr[3u * pix + 1u] = h1;
+        out.hdr[3u * pix + 2u] = h2;
+    }
+    if (out.hdr16) {
+        const __half2 ab = __floats2half2_rn(h0, h1);
+        const __half2 cd = __floats2half2_rn(h2, 1.0f);
+        out.hdr16[pix] = make_uint2(*reinterpret_cast<const std::uint32_t*>(&ab),
+                                    *reinterpret_cast<const std::uint32_t*>(&cd));
+    }
+    if (out.cls) out.cls[pix] = std::uint8_t(cls);
+    if (out.steps) out.steps[pix] = steps;
 }
 
 }  // namespace
 
-hipError_t launch_trace(const SceneConst<float>& sf, const SceneConst<double>& sd, const float2* halton,
-                        const TraceOutputs& out, QueueCtl* ctl, const LaunchShape& shape, hipStream_t stream) {
-    const std::uint32_t tiles_x = (sf.width + 7u) / 8u, tiles_y = (sf.height + 7u) / 8u;
-    const std::uint32_t n_items = tiles_x * tiles_y * 64u;
-    const dim3 block(256), grid(shape.blocks);
+hipError_t launch_trace_chunk(const SceneConst<float>& sf, const SceneConst<double>& sd, const float2* halton,
+                              const Chunk& ch, QueueCtl* ctl, std::uint32_t mode, const LaunchShape& shape,
+                              hipStream_t stream) {
+    const dim3 block(256);
     const std::uint32_t sb = shape.step_batch ? shape.step_batch : 8u;
     const bool kerr = sf.metric == SING_METRIC_KERR;
-#define SING_LAUNCH(M, P) \
-    hipLaunchKernelGGL((trace_kernel<M, P>), grid, block, 0, stream, sf, sd, halton, out, ctl, n_items, tiles_x, sb)
-    switch (shape.precision) {
-        case SING_PREC_FP32: if (kerr) SING_LAUNCH(1, 0); else SING_LAUNCH(0, 0); break;
-        case SING_PREC_FP64: if (kerr) SING_LAUNCH(1, 1); else SING_LAUNCH(0, 1); break;
-        default: if (kerr) SING_LAUNCH(1, 2); else SING_LAUNCH(0, 2); break;
+#define SING_TRACE(M, P, G) \
+    hipLaunchKernelGGL((trace_kernel<M, P>), dim3(G), block, 0, stream, sf, sd, halton, ch, ctl, sb)
+    if (mode == SING_PREC_FP32) {
+        if (kerr) SING_TRACE(1, 0, shape.blocks); else SING_TRACE(0, 0, shape.blocks);
+    } else if (mode == SING_PREC_FP64) {
+        if (kerr) SING_TRACE(1, 2, shape.blocks64); else SING_TRACE(0, 2, shape.blocks64);
+    } else {
+        if (kerr) SING_TRACE(1, 1, shape.blocks); else SING_TRACE(0, 1, shape.blocks);
+        if (kerr) SING_TRACE(1, 3, shape.blocks64); else SING_TRACE(0, 3, shape.blocks64);
     }
-#undef SING_LAUNCH
+#undef SING_TRACE
+    return hipGetLastError();
+}
+
+hipError_t launch_shade_chunk(const SceneConst<float>& sf, const Chunk& ch, PixAcc* acc, bool first, bool last,
+                              const TraceOutputs& out, hipStream_t stream) {
+    const dim3 grid((ch.P + 255u) / 256u), block(256);
+    if (sf.metric == SING_METRIC_KERR)
+        hipLaunchKernelGGL(shade_kernel<1>, grid, block, 0, stream, sf, ch.rec, acc, ch.P, ch.S, first ? 1u : 0u,
+                           last ? 1u : 0u, out);
+    else
+        hipLaunchKernelGGL(shade_kernel<0>, grid, block, 0, stream, sf, ch.rec, acc, ch.P, ch.S, first ? 1u : 0u,
+                           last ? 1u : 0u, out);
     return hipGetLastError();
 }
 

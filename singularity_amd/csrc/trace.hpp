@@ -1,4 +1,13 @@
-// singularity_amd/csrc/trace.hpp — host-visible declarations of the trace launch.
+// singularity_amd/csrc/trace.hpp — host-visible declarations of the frame pipeline.
+//
+// A frame is rendered in sample chunks.  For each chunk of S samples:
+//   1. trace   (FP32 or FP64): persistent wave64 kernel, one ray per lane with
+//      lane refill; writes one 16-byte termination record per (sample, pixel).
+//      In mixed precision the FP32 trace defers photon-band rays to a list.
+//   2. trace64 over the deferred list (mixed precision only).
+//   3. shade: one thread per pixel reads its S records in sample order, shades
+//      them, tone-maps each sample and accumulates; after the last chunk it
+//      writes the frame outputs.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -18,21 +27,54 @@ struct TraceOutputs {
     std::uint32_t* steps;  // W*H RK4 steps summed over samples
 };
 
-// Device-side work queue and counters, zeroed before each launch.
+// One termination record per traced sample.
+//   escape: a = theta, b = phi            disc: a = r_cross, b = phi, c = L_z (Kerr) or u^phi (Schwarzschild)
+//   meta  = term (bits 0..2) | deferred (bit 3) | steps << 4
+struct Rec {
+    float a, b, c;
+    std::uint32_t meta;
+};
+constexpr std::uint32_t kRecDeferred = 8u;
+
+// Running per-pixel sums between chunks.
+struct PixAcc {
+    float ldr[3];
+    float hdr[3];
+    std::uint32_t steps;
+    std::uint32_t cls;
+};
+
+// Device-side work queues and counters, zeroed before each chunk.
 struct QueueCtl {
-    std::uint64_t next_item;
+    std::uint64_t next_item;   // trace queue
+    std::uint64_t next_band;   // FP64 pass queue
+    std::uint32_t band_count;  // deferred records appended by the FP32 pass
+    std::uint32_t pad;
     std::uint64_t steps;
     std::uint64_t rays;
     std::uint64_t rays64;
 };
 
+struct Chunk {
+    Rec* rec;                   // S * P records, index s_local * P + pixel
+    std::uint32_t* band;        // deferred record indices (capacity S * P)
+    std::uint32_t s0, S;        // first sample and sample count of the chunk
+    std::uint32_t P;            // pixels
+    std::uint32_t tiles_x, n_tiles;
+};
+
 struct LaunchShape {
-    std::uint32_t precision;   // SING_PREC_*
     std::uint32_t blocks;      // persistent grid, 256 threads per block
+    std::uint32_t blocks64;    // persistent grid of the FP64 kernels
     std::uint32_t step_batch;  // RK4 steps between refills
 };
 
-hipError_t launch_trace(const SceneConst<float>& sf, const SceneConst<double>& sd, const float2* halton,
-                        const TraceOutputs& out, QueueCtl* ctl, const LaunchShape& shape, hipStream_t stream);
+// mode: SING_PREC_FP32 -> FP32 trace; SING_PREC_FP64 -> FP64 trace of every
+// ray; SING_PREC_MIXED -> FP32 trace with band deferral, then FP64 over the list.
+hipError_t launch_trace_chunk(const SceneConst<float>& sf, const SceneConst<double>& sd, const float2* halton,
+                              const Chunk& ch, QueueCtl* ctl, std::uint32_t mode, const LaunchShape& shape,
+                              hipStream_t stream);
+hipError_t launch_shade_chunk(const SceneConst<float>& sf, const Chunk& ch, PixAcc* acc, bool first, bool last,
+                              const TraceOutputs& out, hipStream_t stream);
 
 }  // namespace sing

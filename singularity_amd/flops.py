@@ -1,0 +1,26 @@
+"""Algorithmic work per unit of the trace kernels.
+
+Counts are the arithmetic of the reference algorithm as written
+(shared_shader/kerr_hamilton.h:97-178 for one damped Kerr RK4 step,
+shared_shader/geodesic_math.h:101-134 for one Schwarzschild RK4 step), taken
+by instantiating the CPU restatement with an operation-counting scalar
+(tests/test_oracle_math.py::test_op_counts_are_stable keeps them in sync).
+Every add, multiply, divide and sin/cos counts as one flop.
+"""
+
+KERR_STEP_OPS = {"add": 142, "mul": 348, "div": 53, "sqrt": 0, "trig": 9, "other": 0}
+SCHW_STEP_OPS = {"add": 80, "mul": 162, "div": 29, "sqrt": 0, "trig": 8, "other": 0}
+
+KERR_STEP_FLOPS = sum(KERR_STEP_OPS.values())  # 552
+SCHW_STEP_FLOPS = sum(SCHW_STEP_OPS.values())  # 279
+
+# MI355X published peaks
+PEAK_FP32_TFLOPS = 157.3       # vector/matrix FP32 (packed issue)
+PEAK_FP32_NONPACKED_TFLOPS = 78.6
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+# Post passes: algorithmic HBM bytes per full-resolution pixel (RGBA16F HDR in,
+# half-res RGBA16F bloom ping-pong, RGBA8 out):
+#   extract 8 read + 2 write, blur_h 2 + 2, blur_v 2 + 2, composite 8 + 2 read + 4 write
+POST_BYTES_PER_PIXEL = 32

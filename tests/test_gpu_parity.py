@@ -1,0 +1,195 @@
+"""GPU parity: the HIP path through the C ABI against the oracle.
+
+Bars (tolerances stated here):
+  FP32 kernel vs FP32 oracle   class agreement >= 99.9 %, RGBA8 within 1 LSB for >= 99.8 %,
+                               HDR within 1e-3 relative for >= 99.7 % of pixels, step totals within 0.1 %
+  FP64 kernel vs FP64 oracle   class agreement >= 99.99 %, HDR within 1e-6 relative for >= 99.9 %
+  mixed vs FP64 oracle         class agreement >= 99.9 % outside the boundary band, HDR within 1e-3
+                               relative for >= 99.9 % of non-band pixels
+The FP32 comparisons differ only through transcendental/divide rounding
+(OCML vs glibc, reciprocal form) amplified along chaotic rays.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import singularity_amd as sa
+from helpers import GOLDEN, config1, config2, config3, golden_scene, load_png, phash_distance, rel_err, scene_uniforms
+
+pytestmark = pytest.mark.gpu
+
+ALL = sa.OUT_RGBA8 | sa.OUT_HDR | sa.OUT_CLASS | sa.OUT_STEPS
+
+SCENES = {
+    "schw_golden": lambda: golden_scene("schw.png"),
+    "kerr_a05": lambda: golden_scene("kerr_a05.png"),
+    "kerr_a094": lambda: golden_scene("kerr_a094.png"),
+    "config1": lambda: config1(),
+    "config2_small": lambda: config2(160, 90, 4),
+    "config3_small": lambda: config3(192, 108, 2),
+}
+
+
+@pytest.fixture(scope="module")
+def backends():
+    cache = {}
+
+    def get(w, h):
+        if (w, h) not in cache:
+            cache[(w, h)] = sa.HipBackend(w, h)
+        return cache[(w, h)]
+
+    yield get
+    for b in cache.values():
+        b.shutdown()
+
+
+def render(backends, u, prec=sa.PREC_FP32, outputs=ALL, **kw):
+    print(f"render {u.width}x{u.height} spp={u.supersample} metric={u.metric_type} prec={prec} {kw}", flush=True)
+    return backends(u.width, u.height).render_outputs(u, outputs, prec, **kw)
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_fp32_matches_fp32_oracle(backends, orc, name):
+    u = SCENES[name]()
+    f = render(backends, u)
+    o = orc.render(u, 32)
+    n = u.width * u.height
+    assert f.stats.rays == n * max(1, u.supersample)
+    assert (f.cls.ravel() == o["cls"]).mean() >= 0.999
+    d = np.abs(f.rgba.reshape(-1, 4).astype(int) - o["rgba"].astype(int)).max(-1)
+    assert (d <= 1).mean() >= 0.998
+    assert (rel_err(f.hdr, o["hdr"]) <= 1e-3).mean() >= 0.997
+    assert abs(int(f.steps.sum()) - int(o["steps"].sum())) <= 1e-3 * o["steps"].sum()
+    assert int(f.steps.sum()) == f.stats.steps
+
+
+@pytest.mark.parametrize("name", ["kerr_a05", "kerr_a094", "config1", "config3_small"])
+def test_fp64_matches_fp64_oracle(backends, orc, name):
+    u = SCENES[name]()
+    f = render(backends, u, sa.PREC_FP64)
+    o = orc.render(u, 64)
+    assert f.stats.fp64_rays == f.stats.rays
+    assert (f.cls.ravel() == o["cls"]).mean() >= 0.9999
+    assert (rel_err(f.hdr, o["hdr"]) <= 1e-6).mean() >= 0.999
+
+
+@pytest.mark.parametrize("name", ["kerr_a05", "kerr_a094", "config1", "config3_small"])
+def test_mixed_matches_fp64_oracle_outside_band(backends, orc, name):
+    u = SCENES[name]()
+    f = render(backends, u, sa.PREC_MIXED)
+    o = orc.render(u, 64)
+    band = orc.boundary_band(u)
+    keep = ~band
+    assert 0 < f.stats.fp64_rays < f.stats.rays
+    assert (f.cls.ravel()[keep] == o["cls"][keep]).mean() >= 0.999
+    assert (rel_err(f.hdr, o["hdr"])[keep] <= 1e-3).mean() >= 0.999
+
+
+def test_golden_structure_from_gpu(backends):
+    for name in ("schw.png", "kerr_a05.png", "kerr_a094.png"):
+        f = render(backends, golden_scene(name), outputs=sa.OUT_RGBA8)
+        assert phash_distance(f.rgba, load_png(GOLDEN / "cuda" / name)) <= 4
+
+
+def test_frames_are_bit_reproducible(backends):
+    u = config3(256, 144, 3)
+    a = render(backends, u, sa.PREC_MIXED)
+    b = render(backends, u, sa.PREC_MIXED)
+    for k in ("rgba", "hdr", "cls", "steps"):
+        np.testing.assert_array_equal(getattr(a, k), getattr(b, k))
+    assert a.stats.steps == b.stats.steps and a.stats.fp64_rays == b.stats.fp64_rays
+
+
+def test_many_samples_match_oracle_small_image(backends, orc):
+    """spp up to the 1024 cap: Halton jitter, per-sample ACES mean, chunked accumulation."""
+    u = scene_uniforms(spin=0.94, r_in=1.7, r_out=30.0, width=8, height=6, spp=1024)
+    f = render(backends, u)
+    o = orc.render(u, 32)
+    assert f.stats.rays == 8 * 6 * 1024
+    d = np.abs(f.rgba.reshape(-1, 4).astype(int) - o["rgba"].astype(int)).max(-1)
+    assert d.max() <= 1
+    assert (rel_err(f.hdr, o["hdr"]) <= 1e-3).mean() >= 0.95
+
+
+@pytest.mark.parametrize("w,h", [(4, 4), (37, 23), (1, 1), (257, 3)])
+def test_ragged_sizes(backends, orc, w, h):
+    u = scene_uniforms(spin=0.5, r_in=4.0, r_out=30.0, width=w, height=h, spp=2)
+    f = render(backends, u)
+    o = orc.render(u, 32)
+    assert f.rgba.shape == (h, w, 4) and np.all(f.rgba[..., 3] == 255)
+    assert (f.cls.ravel() == o["cls"]).mean() >= 0.99
+
+
+def test_edge_cases_match_oracle(backends, orc):
+    cases = {
+        "budget_1_step": scene_uniforms(width=16, height=16, max_steps=1),
+        "escape_below_camera": scene_uniforms(width=16, height=16, escape_M=10.0),
+        "no_disc": scene_uniforms(width=32, height=32, disc=False),
+        "no_doppler_redshift": scene_uniforms(width=32, height=32, doppler=False, redshift=False),
+        "camera_inside_horizon": scene_uniforms(width=16, height=16, dist=1.5),
+        "negative_inner_radius": scene_uniforms(width=32, height=32, r_in=-1.0),
+        "schw_camera_inside": scene_uniforms(metric=sa.METRIC_SCHWARZSCHILD, spin=0.0, width=16, height=16, dist=1.5),
+    }
+    for name, u in cases.items():
+        f = render(backends, u)
+        o = orc.render(u, 32)
+        agree = (f.cls.ravel() == o["cls"]).mean()
+        d = np.abs(f.rgba.reshape(-1, 4).astype(int) - o["rgba"].astype(int)).max(-1)
+        assert agree >= 0.99, name
+        assert (d <= 1).mean() >= 0.99, name
+    f = render(backends, cases["budget_1_step"])
+    assert np.all(f.cls == sa.CLASS_BUDGET) and np.all(f.steps == 1)
+    f = render(backends, cases["escape_below_camera"])
+    assert np.all(f.cls == sa.CLASS_ESCAPE) and np.all(f.steps == 0)
+
+
+def test_starfield_off_gives_black_sky(backends, orc):
+    u = scene_uniforms(width=32, height=32, disc=False)
+    u.flags &= ~sa.FLAG_STARFIELD
+    f = render(backends, u)
+    assert float(f.hdr.max()) == 0.0
+    o = orc.render(u, 32)
+    assert o["hdr"].max() == 0.0
+
+
+def test_bloom_post_pass_matches_oracle(backends, orc):
+    u = config2(160, 90, 2)
+    f = render(backends, u, outputs=sa.OUT_HDR16 | sa.OUT_RGBA8, post=sa.POST_BLOOM)
+    hdr = f.hdr16[..., :3].astype(np.float32)
+    ref = orc.bloom_pipeline(hdr, u.bloom_threshold, u.bloom_strength, u.exposure, half_store=True)
+    d = np.abs(f.rgba.astype(int) - ref["rgba"].astype(int)).max(-1)
+    assert (d <= 1).mean() >= 0.999 and d.max() <= 2
+    assert f.stats.post_ms > 0.0
+
+
+def test_resize_and_capture_errors(backends):
+    be = sa.HipBackend(16, 16)
+    u = scene_uniforms(width=16, height=16)
+    be.render(u)
+    with pytest.raises(sa.SingError):
+        be.capture(sa.OUT_HDR)  # not rendered in the last frame
+    be.resize(24, 8)
+    with pytest.raises(sa.SingError):
+        be.render(u)  # size mismatch
+    u2 = scene_uniforms(width=24, height=8)
+    st = be.render(u2)
+    assert st.rays == 24 * 8
+    assert be.capture_frame().shape == (8, 24, 4)
+    be.shutdown()
+
+
+def test_full_4k_frame_properties(backends, orc):
+    """The 4K size itself: 1-spp frame, ray/step accounting, and a seeded pixel subset against the oracle."""
+    u = config3(3840, 2160, 1)
+    f = render(backends, u, sa.PREC_FP32)
+    assert f.stats.rays == 3840 * 2160
+    assert int(f.steps.sum(dtype=np.uint64)) == f.stats.steps
+    rng = np.random.default_rng(2024)
+    px = rng.choice(3840 * 2160, 4000, replace=False)
+    o = orc.render(u, 32, pixels=px)
+    assert (f.cls.ravel()[px] == o["cls"]).mean() >= 0.999
+    d = np.abs(f.rgba.reshape(-1, 4)[px].astype(int) - o["rgba"].astype(int)).max(-1)
+    assert (d <= 1).mean() >= 0.998

@@ -628,6 +628,8 @@ template <class T> struct RayOut {
     double inv0[3] = {0, 0, 0};  // Kerr: 2H, Q, - ; Schw: E, L, null residual
     double inv1[3] = {0, 0, 0};
     double E = 0, Lz = 0;        // conserved parameters at init
+    double r_min = 1e30;         // smallest radius reached
+    double th_min = 1e30;        // smallest |theta - pi/2| over the states after the first
 };
 
 template <class T> inline T soft_alpha(T r, T r_in, T r_out, T w) {
@@ -821,6 +823,8 @@ inline RayOut<T> trace_kerr(const Uni<T>& u, const CamFrame<T>& f, const CameraR
         prev_r = s.r;
         s = kerr_step(s, c, u, h_cur);
         o.steps = step + 1;
+        o.r_min = std::min(o.r_min, val(s.r));
+        o.th_min = std::min(o.th_min, std::fabs(val(s.theta) - 1.5707963267948966));
     }
     return finish(CLS_BUDGET);
 }
@@ -886,6 +890,8 @@ inline RayOut<T> trace_schw(const Uni<T>& u, const CamFrame<T>& f, const CameraR
         prev_uphi = s.uphi;
         s = schw_rk4(s, u.h_step, u.rs);
         o.steps = step + 1;
+        o.r_min = std::min(o.r_min, val(s.r));
+        o.th_min = std::min(o.th_min, std::fabs(val(s.theta) - 1.5707963267948966));
     }
     return finish(CLS_BUDGET);
 }

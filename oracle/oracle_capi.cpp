@@ -78,6 +78,8 @@ template <class T> void ray_impl(const void* raw, const OrcExtras* ex, std::uint
     }
     out[12] = r.E;
     out[13] = r.Lz;
+    out[14] = r.r_min;
+    out[15] = r.th_min;
 }
 
 // Scene parameters of the differentiable renderer -> Uniforms in T.
@@ -133,9 +135,9 @@ int orc_render(const void* uniforms, int precision, const OrcExtras* ex, const s
 }
 
 int orc_trace_ray(const void* uniforms, int precision, const OrcExtras* ex, std::uint32_t gx, std::uint32_t gy,
-                  double ox, double oy, double* out14) {
-    if (precision == 32) ray_impl<float>(uniforms, ex, gx, gy, ox, oy, out14);
-    else if (precision == 64) ray_impl<double>(uniforms, ex, gx, gy, ox, oy, out14);
+                  double ox, double oy, double* out16) {
+    if (precision == 32) ray_impl<float>(uniforms, ex, gx, gy, ox, oy, out16);
+    else if (precision == 64) ray_impl<double>(uniforms, ex, gx, gy, ox, oy, out16);
     else return -1;
     return 0;
 }

@@ -316,6 +316,35 @@ template <class R> __device__ __forceinline__ void shade_escape(const SceneConst
     rgb[2] = b * R(0.6);
 }
 
+// True when the escape direction lies within `eps` cells of a starfield cell
+// boundary and a star sits on either side of it: FP32 and FP64 trajectories
+// may then land in different cells (geodesic_kernel.cu:135-153 grid).
+__device__ __forceinline__ bool starfield_ambiguous(float th, float ph, float eps) {
+    float st, ct, sp, cp;
+    dsincos(th, &st, &ct);
+    dsincos(ph, &sp, &cp);
+    const float x = st * cp, y = st * sp, z = ct;
+    const float len = sqrtf(x * x + y * y + z * z);
+    if (len < 1e-6f) return false;
+    const float il = 1.0f / len;
+    const float pi = 3.14159265358979323846f;
+    const float fx = (atan2f(y * il, x * il) + pi) / (2.0f * pi) * 768.0f;
+    const float fy = acosf(fminf(fmaxf(z * il, -1.0f), 1.0f)) / pi * 384.0f;
+    const int ix = int(fx) % 768, iy = int(fy) % 384;
+    const float dx = fx - floorf(fx), dy = fy - floorf(fy);
+    auto star = [](int cx, int cy) {
+        cx = (cx % 768 + 768) % 768;
+        cy = (cy % 384 + 384) % 384;
+        return (mix32(std::uint32_t(cx), std::uint32_t(cy)) & 0xFFu) < 4u;
+    };
+    const bool here = star(ix, iy);
+    if (dx < eps && (here || star(ix - 1, iy))) return true;
+    if (dx > 1.0f - eps && (here || star(ix + 1, iy))) return true;
+    if (dy < eps && (here || star(ix, iy - 1))) return true;
+    if (dy > 1.0f - eps && (here || star(ix, iy + 1))) return true;
+    return false;
+}
+
 // Disc emission; kerr: cos psi = clamp(L_z/r) and the Kerr Keplerian rate
 // (geodesic_kernel.cu:366-394); schwarzschild: cos psi = clamp(-u^phi r)
 // (geodesic_kernel.cu:264-289).
@@ -392,18 +421,19 @@ __device__ __forceinline__ void camera_ray(const SceneConst<R>& sc, std::uint32_
 // A ray is near-critical (it winds around the photon shell and amplifies
 // rounding exponentially) when the local minimum of R outside the horizon is
 // close to zero.  Returns |R(r*)| / r*^4 at the largest stationary point r*,
-// or a large value when R has no such minimum outside r+.
+// or -1 when R has no stationary point outside r+ (no turning point: an
+// ingoing ray is captured, an outgoing one escapes directly).
 template <class R> __device__ __forceinline__ R photon_margin(R xi, R eta, R a, R M, R r_plus) {
     const R C = (xi - a) * (xi - a) + eta;
     const R P = R(0.5) * (a * a - xi * xi - eta);  // R'/4 = r^3 + P r + Qc
     const R Qc = R(0.5) * M * C;
-    if (!(P < R(0))) return R(1e30);
+    if (!(P < R(0))) return R(-1);
     const R disc = -(R(4) * P * P * P + R(27) * Qc * Qc);
-    if (!(disc > R(0))) return R(1e30);
+    if (!(disc > R(0))) return R(-1);
     const R m = R(2) * dsqrt(-P / R(3));
     const R arg = dclamp((R(3) * Qc / (R(2) * P)) * dsqrt(R(-3) / P), R(-1), R(1));
     const R rs = m * dcos(dacos(arg) / R(3));
-    if (!(rs > r_plus)) return R(1e30);
+    if (!(rs > r_plus)) return R(-1);
     const R r2 = rs * rs;
     const R Rv = r2 * r2 + (a * a - xi * xi - eta) * r2 + R(2) * M * C * rs - a * a * eta;
     const R v = Rv / (r2 * r2);

@@ -30,8 +30,17 @@ template <class S> struct RayState {
     S v0, v1, v2, v3;  // Kerr: p_r, p_theta, E, L_z.  Schwarzschild: u^t, u^r, u^theta, u^phi
     S prev_r, prev_th, prev_uph;
     S aux0, aux1;      // disc crossing radius, u^phi at the crossing (Schwarzschild)
+    S near_eq;         // mixed precision: min |theta - pi/2| over the steps (0 = ambiguous crossing)
     std::uint32_t steps;
 };
+
+// Mixed-precision deferral thresholds (see the FP32/FP64 disagreement study in
+// the design notes): a state within kGrazeTheta of the equator, a crossing
+// within kEdgeRel of an annulus edge, or an escape within kStarCell cells of a
+// starfield boundary next to a star makes the FP32 outcome ambiguous.
+constexpr float kGrazeTheta = 2e-6f;
+constexpr float kEdgeRel = 1e-5f;
+constexpr float kStarCell = 1e-3f;
 
 __device__ __forceinline__ std::uint64_t shfl_u64(std::uint64_t v, int src) {
     const std::uint32_t lo = __shfl(std::uint32_t(v), src);
@@ -41,20 +50,25 @@ __device__ __forceinline__ std::uint64_t shfl_u64(std::uint64_t v, int src) {
 
 // Termination checks in the reference's order (geodesic_kernel.cu:351-401)
 // followed by one RK4 step.  Arithmetic in R, storage in S.
-template <class R, class S, bool KERR>
+template <class R, class S, bool KERR, bool TRACK = false>
 __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneConst<R>& sc) {
     if (y.steps >= sc.max_steps) return kTermBudget;
     R r = R(y.r), th = R(y.th);
     if (!__builtin_isfinite(r) || r < sc.cut) return kTermHorizon;
     if (r > sc.escape_r) return kTermEscape;
+    const R mid = R(1.5707963267948966192313216916397514);
     if (sc.flags & SING_FLAG_DISC_ON) {
-        const R mid = R(1.5707963267948966192313216916397514);
         const R pth = R(y.prev_th);
         const R dp = pth - mid, dc = th - mid;
         if (dp * dc < R(0)) {
             const R frac = (mid - pth) / (th - pth);
             const R pr0 = R(y.prev_r);
             const R rc = pr0 + frac * (r - pr0);
+            if (TRACK) {
+                const R tol = R(kEdgeRel) * rc;
+                const R d_in = rc - sc.r_in, d_out = rc - sc.r_out;
+                if ((d_in < tol && d_in > -tol) || (d_out < tol && d_out > -tol)) y.near_eq = S(0);
+            }
             if (rc >= sc.r_in && rc <= sc.r_out) {
                 y.aux0 = S(rc);
                 if (!KERR) {
@@ -87,6 +101,11 @@ __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneCons
     y.th = S(th);
     y.ph = S(ph);
     y.steps += 1;
+    if (TRACK) {
+        const R d = th - mid;
+        const R ad = d < R(0) ? -d : d;
+        if (ad < R(y.near_eq)) y.near_eq = S(ad);
+    }
     return kTermLive;
 }
 
@@ -104,6 +123,7 @@ __device__ __forceinline__ bool init_ray(RayState<S>& y, const SceneConst<R>& sc
     y.prev_r = y.r;
     y.prev_th = y.th;
     y.steps = 0;
+    y.near_eq = S(1e30);
     if (KERR) {
         const R Bq = R(2) * sc.g_tp * uph;
         const R Cq = sc.g_rr * ur * ur + sc.g_thth * uth * uth + sc.g_pp * uph * uph;
@@ -120,7 +140,9 @@ __device__ __forceinline__ bool init_ray(RayState<S>& y, const SceneConst<R>& sc
         if (margin_out) {
             const R Q = pt * pt + sc.q_cos2 * (L * L * sc.q_inv_s2 - sc.a2 * E * E);
             const R rp = sc.M + dsqrt(dmax(sc.M * sc.M - sc.a2, R(0)));
-            *margin_out = photon_margin<R>(L / E, Q / (E * E), sc.a, sc.M, rp);
+            const R m = photon_margin<R>(L / E, Q / (E * E), sc.a, sc.M, rp);
+            // no turning point: ingoing rays are captured (defer), outgoing escape directly
+            *margin_out = (m < R(0)) ? (ur < R(0) ? R(-1) : R(1e30)) : m;
         }
     } else {
         const R cr = sc.cam_r;
@@ -134,7 +156,8 @@ __device__ __forceinline__ bool init_ray(RayState<S>& y, const SceneConst<R>& sc
         if (margin_out) {
             const R E = sc.f_cam * ut;
             const R L2 = cr * cr * cr * cr * (uth * uth + sc.sin_th * sc.sin_th * uph * uph);
-            *margin_out = photon_margin<R>(R(0), L2 / (E * E), R(0), sc.M, sc.twoM);
+            const R m = photon_margin<R>(R(0), L2 / (E * E), R(0), sc.M, sc.twoM);
+            *margin_out = (m < R(0)) ? (ur < R(0) ? R(-1) : R(1e30)) : m;
         }
     }
     return true;
@@ -260,7 +283,8 @@ __global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneC
         std::uint32_t term = kTermLive;
         for (std::uint32_t k = 0; k < step_batch; ++k) {
             if (live) {
-                const std::uint32_t t = F64 ? advance<double, S, KERR>(y, sd) : advance<float, S, KERR>(y, sf);
+                const std::uint32_t t = F64 ? advance<double, S, KERR>(y, sd)
+                                            : advance<float, S, KERR, PASS == 1>(y, sf);
                 if (t != kTermLive) {
                     live = false;
                     term = t;
@@ -268,11 +292,18 @@ __global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneC
             }
         }
 
-        // (3) emit finished rays
+        // (3) emit finished rays; in the mixed pass, hand ambiguous outcomes to FP64
         if (term != kTermLive) {
-            ch.rec[rec_idx] = make_rec<S, KERR>(y, term);
-            lane_steps += y.steps;
-            lane_rays += 1;
+            bool defer = false;
+            if (PASS == 1) defer = float(y.near_eq) < kGrazeTheta;
+            if (defer) {
+                const std::uint32_t slot = atomicAdd(&ctl->band_count, 1u);
+                ch.band[slot] = rec_idx;
+            } else {
+                ch.rec[rec_idx] = make_rec<S, KERR>(y, term);
+                lane_steps += y.steps;
+                lane_rays += 1;
+            }
             need = true;
         }
     }
@@ -285,6 +316,23 @@ __global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneC
         atomicAdd((unsigned long long*)&ctl->steps, (unsigned long long)lane_steps);
         atomicAdd((unsigned long long*)&ctl->rays, (unsigned long long)lane_rays);
         if (F64) atomicAdd((unsigned long long*)&ctl->rays64, (unsigned long long)lane_rays);
+    }
+}
+
+// Mixed precision: escape records whose direction sits on a starfield cell
+// boundary next to a star are handed to the FP64 pass (kept out of the trace
+// kernel so its step loop stays register-lean).
+__global__ __launch_bounds__(256) void audit_kernel(SceneConst<float> sf, Chunk ch, QueueCtl* __restrict__ ctl) {
+    const std::uint32_t n = ch.S * ch.P;
+    for (std::uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+        const Rec r = ch.rec[i];
+        const std::uint32_t term = r.meta & 7u;
+        if (term == kTermEscape && starfield_ambiguous(r.a, r.b, kStarCell)) {
+            const std::uint32_t slot = atomicAdd(&ctl->band_count, 1u);
+            ch.band[slot] = i;
+            atomicSub((unsigned long long*)&ctl->rays, 1ull);
+            atomicSub((unsigned long long*)&ctl->steps, (unsigned long long)(r.meta >> 4));
+        }
     }
 }
 
@@ -377,6 +425,8 @@ hipError_t launch_trace_chunk(const SceneConst<float>& sf, const SceneConst<doub
         if (kerr) SING_TRACE(1, 2, shape.blocks64); else SING_TRACE(0, 2, shape.blocks64);
     } else {
         if (kerr) SING_TRACE(1, 1, shape.blocks); else SING_TRACE(0, 1, shape.blocks);
+        if (sf.flags & SING_FLAG_STARFIELD_ON)
+            hipLaunchKernelGGL(audit_kernel, dim3(shape.blocks), block, 0, stream, sf, ch, ctl);
         if (kerr) SING_TRACE(1, 3, shape.blocks64); else SING_TRACE(0, 3, shape.blocks64);
     }
 #undef SING_TRACE

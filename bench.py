@@ -141,7 +141,8 @@ def main() -> int:
         },
         "roofline": {
             "bound": "valu", "achieved": round(achieved, 3), "peak": FL.PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / FL.PEAK_FP32_TFLOPS, 4), "traffic": None,
+            "frac": round(achieved / FL.PEAK_FP32_TFLOPS, 4),
+            "traffic": round(FL.TRACE_HBM_BYTES_PER_RAY * rays_per_frame / max(1, int(stats[-1].chunks))),
             "kernel": "trace (FP32 pass + FP64 band pass)",
             "flops_per_step": FL.KERR_STEP_FLOPS, "steps_per_launch": int(steps_total),
             "frac_of_nonpacked_fp32": round(achieved / FL.PEAK_FP32_NONPACKED_TFLOPS, 4),

@@ -17,9 +17,10 @@
 // float instantiation is bit-identical to the reference headers built with
 // -ffp-contract=off (checked against oracle/_ref by tests).
 //
-// Parity status per function (how each is pinned) is documented in the
-// oracle section of the design notes; functions with no pin say
-// "parity unpinned" here: none at present.
+// Every function here is pinned by a test (tests/test_oracle_vs_reference.py,
+// tests/test_oracle_math.py, tests/test_golden_images.py) except the bloom
+// post passes, whose parity is unpinned: the reference's realtime backends
+// that own them are not buildable here and commit no bloom fixtures.
 
 #pragma once
 

@@ -34,10 +34,12 @@ template <class S> struct RayState {
     std::uint32_t steps;
 };
 
-// Mixed-precision deferral thresholds (see the FP32/FP64 disagreement study in
-// the design notes): a state within kGrazeTheta of the equator, a crossing
-// within kEdgeRel of an annulus edge, or an escape within kStarCell cells of a
-// starfield boundary next to a star makes the FP32 outcome ambiguous.
+// Mixed-precision deferral thresholds.  Comparing FP32 and FP64 traces of the
+// same rays, outcomes differ only for rays near a discontinuity of the
+// discrete algorithm: a state within kGrazeTheta of the equator (the crossing
+// test flips), a crossing within kEdgeRel of an annulus edge, or an escape
+// within kStarCell cells of a starfield boundary next to a star.  Such rays
+// are re-traced in FP64.
 constexpr float kGrazeTheta = 2e-6f;
 constexpr float kEdgeRel = 1e-5f;
 constexpr float kStarCell = 1e-3f;

@@ -24,3 +24,8 @@ PEAK_HBM_GBS = 8000.0
 # half-res RGBA16F bloom ping-pong, RGBA8 out):
 #   extract 8 read + 2 write, blur_h 2 + 2, blur_v 2 + 2, composite 8 + 2 read + 4 write
 POST_BYTES_PER_PIXEL = 32
+
+# Measured HBM traffic of one FP32 trace dispatch (rocprofv3 FETCH_SIZE x2.0 +
+# WRITE_SIZE x1.0, calibrated on a plain copy kernel), per traced ray:
+# 16-byte termination record + deferred-ray indices.
+TRACE_HBM_BYTES_PER_RAY = (575061.8 * 1024 + 93.8 * 1024 * 2.0) / 33177600.0

@@ -225,6 +225,55 @@ void sing_pack_uniforms(const SingScene* scene, const SingCamera* cam, uint32_t 
  * 5 photon orbit retrograde. */
 double sing_kerr_radius(int which, double mass, double spin_a);
 
+/* --- differentiable render (discrete adjoint) ------------------------------
+ * No counterpart in the reference (nothing there is differentiable).  The scene
+ * is Kerr with parameters p = (a/M, inclination, r_in/M, r_out/M, peak-T
+ * scale); the camera is the orbital camera at elevation pi/2 - inclination;
+ * the disc has soft annulus edges of width soft_edge (M) so the image is
+ * differentiable in r_in and r_out.  One sample per pixel at the pixel centre.
+ * `base` supplies resolution (must equal the context), mass, h_step,
+ * max_steps, escape_r, flags, peak T, turbulence and time; its camera and
+ * spin/disc fields are ignored. */
+typedef struct SingDiffParams {
+    double spin;      /* a/M */
+    double incl;      /* inclination (rad): camera polar angle */
+    double r_in;      /* disc inner radius / M */
+    double r_out;     /* disc outer radius / M */
+    double t_scale;   /* multiplies base->disc_peak_T */
+} SingDiffParams;
+
+typedef struct SingDiffCamera {
+    double distance;  /* M */
+    double azimuth;   /* rad */
+    double fov_y_deg;
+} SingDiffCamera;
+
+typedef struct SingDiffOpts {
+    uint32_t precision;        /* SING_PREC_*: FP32, FP64, or mixed (FP64 for band rays) */
+    uint32_t checkpoint_every; /* RK4 steps between stored states (0 = 64) */
+    double soft_edge;          /* annulus edge half-width in M (0 = 0.5) */
+    uint32_t reserved[6];
+} SingDiffOpts;
+
+typedef struct SingDiffStats {
+    double forward_ms;
+    double backward_ms;        /* adjoint sweep + reduction */
+    uint64_t rays;
+    uint64_t steps;            /* forward RK4 steps */
+    uint64_t fp64_rays;
+    uint64_t reserved[3];
+} SingDiffStats;
+
+/* Forward only: radiance (W*H*3 float, host) of the differentiable model. */
+int sing_diff_forward(SingContext* ctx, const SingUniforms* base, const SingDiffParams* p, const SingDiffCamera* cam,
+                      const SingDiffOpts* opts, float* hdr_out, SingDiffStats* stats);
+/* Loss = sum_pixels w_p |I_p - target_p|^2 (target W*H*3 float, weights W*H
+ * float or NULL for 1) and its gradient wrt the 5 parameters (grad[5]);
+ * hdr_out (optional) receives the radiance. */
+int sing_diff_loss_grad(SingContext* ctx, const SingUniforms* base, const SingDiffParams* p,
+                        const SingDiffCamera* cam, const SingDiffOpts* opts, const float* target,
+                        const float* weights, double* loss, double* grad, float* hdr_out, SingDiffStats* stats);
+
 #ifdef __cplusplus
 }
 #endif

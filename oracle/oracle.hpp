@@ -631,6 +631,7 @@ template <class T> struct RayOut {
     double E = 0, Lz = 0;        // conserved parameters at init
     double r_min = 1e30;         // smallest radius reached
     double th_min = 1e30;        // smallest |theta - pi/2| over the states after the first
+    int n_soft = 0;              // crossings with nonzero opacity (soft-edge model)
 };
 
 template <class T> inline T soft_alpha(T r, T r_in, T r_out, T w) {
@@ -712,12 +713,15 @@ inline bool disc_hit(const Uni<T>& u, T r_cross, V3<T>& acc, T& trans, RayOut<T>
     if (!(alpha > T(0.0))) return false;
     const T Tk = disc_temperature(r_cross, u.disc_r_inner, peak);
     if (Tk > T(0.0)) acc = acc + emit(Tk) * (trans * alpha);
-    trans = trans * (T(1.0) - alpha);
     o.r_cross = r_cross;
-    if (!(val(alpha) < 1.0)) {
+    o.n_soft += 1;
+    // opaque crossing, or the 8th crossing of the soft model: the ray ends here
+    if (!(val(alpha) < 1.0) || o.n_soft == 8) {
         o.cls = CLS_DISC;
+        trans = T(0.0);
         return true;
     }
+    trans = trans * (T(1.0) - alpha);
     return false;
 }
 

@@ -383,3 +383,18 @@ double orc_loss_f64(const void* base, const OrcExtras* ex, const double* params,
 }
 
 }  // extern "C"
+
+extern "C" {
+// Exact Jacobian (forward-mode dual numbers) of one damped Kerr RK4 step of the
+// reference algorithm (kerr_hamilton.h:165-178) in double:
+// J[i*8 + j] = d y'_i / d x_j, y = (r, theta, phi, p_r, p_theta), x = (y, E, L_z, a).
+void orc_kerr_step_jacobian(const double* y, double E, double L, double a, double M, double h, double* J) {
+    using D = Dual<8>;
+    const KState<D> s{D(0.0), D::var(y[0], 0), D::var(y[1], 1), D::var(y[2], 2), D::var(y[3], 3), D::var(y[4], 4)};
+    const KConst<D> c{D::var(E, 5), D::var(L, 6), D(0.0), D::var(a, 7), D(M)};
+    const KState<D> o = kerr_rk4(s, D(h), c);
+    const D* out[5] = {&o.r, &o.theta, &o.phi, &o.p_r, &o.p_theta};
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 8; ++j) J[i * 8 + j] = out[i]->d[j];
+}
+}

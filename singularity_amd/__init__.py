@@ -324,3 +324,105 @@ class HipBackend:
         f = self.capture(outputs | (OUT_RGBA8 if o.post else 0))
         f.stats = st
         return f
+
+
+# --- differentiable render (include/sing_abi.h: sing_diff_*) -------------------
+class DiffParams(C.Structure):
+    _fields_ = [("spin", C.c_double), ("incl", C.c_double), ("r_in", C.c_double), ("r_out", C.c_double),
+                ("t_scale", C.c_double)]
+
+    def as_array(self) -> np.ndarray:
+        return np.array([self.spin, self.incl, self.r_in, self.r_out, self.t_scale])
+
+    @staticmethod
+    def from_array(a) -> "DiffParams":
+        return DiffParams(*[float(v) for v in a])
+
+
+class DiffCamera(C.Structure):
+    _fields_ = [("distance", C.c_double), ("azimuth", C.c_double), ("fov_y_deg", C.c_double)]
+
+
+class DiffOpts(C.Structure):
+    _fields_ = [("precision", C.c_uint32), ("checkpoint_every", C.c_uint32), ("soft_edge", C.c_double),
+                ("reserved", C.c_uint32 * 6)]
+
+
+class DiffStats(C.Structure):
+    _fields_ = [("forward_ms", C.c_double), ("backward_ms", C.c_double), ("rays", C.c_uint64),
+                ("steps", C.c_uint64), ("fp64_rays", C.c_uint64), ("reserved", C.c_uint64 * 3)]
+
+
+_SIGS.update({
+    "sing_diff_forward": (C.c_int, [C.c_void_p, C.POINTER(Uniforms), C.POINTER(DiffParams), C.POINTER(DiffCamera),
+                                    C.POINTER(DiffOpts), C.c_void_p, C.POINTER(DiffStats)]),
+    "sing_diff_loss_grad": (C.c_int, [C.c_void_p, C.POINTER(Uniforms), C.POINTER(DiffParams),
+                                      C.POINTER(DiffCamera), C.POINTER(DiffOpts), C.c_void_p, C.c_void_p,
+                                      C.POINTER(C.c_double), C.c_void_p, C.c_void_p, C.POINTER(DiffStats)]),
+})
+
+
+def diff_base_uniforms(width: int, height: int, starfield: bool = False, peak_T: float = 4500.0,
+                       h_step: float = 0.4, max_steps: int = 2500) -> Uniforms:
+    """Base block of the differentiable Kerr scene: resolution, integrator and shading settings."""
+    s = scene_defaults()
+    s.metric = METRIC_KERR
+    s.disc_peak_T_K = peak_T
+    s.h_step = h_step
+    s.max_steps = max_steps
+    u = pack_uniforms(s, orbital_camera(), width, height, 0.0)
+    if not starfield:
+        u.flags &= ~FLAG_STARFIELD
+    return u
+
+
+def _diff_call(be: "HipBackend", fn: str, base: Uniforms, params: DiffParams, cam: DiffCamera,
+               opts: DiffOpts | None, *rest):
+    L = lib()
+    f = getattr(L, fn)
+    o = opts if opts is not None else DiffOpts()
+    st = DiffStats()
+    be._check(f(be._ctx, C.byref(base), C.byref(params), C.byref(cam), C.byref(o), *rest, C.byref(st)))
+    return st
+
+
+def diff_forward(be: "HipBackend", base: Uniforms, params: DiffParams, cam: DiffCamera,
+                 opts: DiffOpts | None = None) -> tuple[np.ndarray, DiffStats]:
+    hdr = np.empty((be.height, be.width, 3), np.float32)
+    st = _diff_call(be, "sing_diff_forward", base, params, cam, opts, hdr.ctypes.data)
+    return hdr, st
+
+
+def diff_loss_grad(be: "HipBackend", base: Uniforms, params: DiffParams, cam: DiffCamera, target: np.ndarray,
+                   weights: np.ndarray | None = None, opts: DiffOpts | None = None):
+    """Loss = sum w |I - target|^2 and d loss / d(a/M, incl, r_in/M, r_out/M, T-scale)."""
+    t = np.ascontiguousarray(target, dtype=np.float32)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    loss = C.c_double(0.0)
+    grad = np.zeros(5, np.float64)
+    hdr = np.empty((be.height, be.width, 3), np.float32)
+    st = _diff_call(be, "sing_diff_loss_grad", base, params, cam, opts, t.ctypes.data,
+                    None if w is None else w.ctypes.data, C.byref(loss), grad.ctypes.data, hdr.ctypes.data)
+    return loss.value, grad, hdr, st
+
+
+def adam_fit(be: "HipBackend", base: Uniforms, start: np.ndarray, target: np.ndarray, cam: DiffCamera,
+             steps: int = 200, lr: np.ndarray | None = None, free: np.ndarray | None = None,
+             opts: DiffOpts | None = None, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-12):
+    """Adam on the 5 scene parameters (entries with free == False stay fixed)."""
+    x = np.array(start, np.float64)
+    lr = np.array([0.01, 0.01, 0.05, 0.1, 0.01]) if lr is None else np.asarray(lr, np.float64)
+    free = np.ones(5, bool) if free is None else np.asarray(free, bool)
+    m = np.zeros(5)
+    v = np.zeros(5)
+    history = []
+    for k in range(1, steps + 1):
+        loss, g, _, _ = diff_loss_grad(be, base, DiffParams.from_array(x), cam, target, None, opts)
+        g = np.where(free, g, 0.0)
+        m = beta1 * m + (1 - beta1) * g
+        v = beta2 * v + (1 - beta2) * g * g
+        step = lr * (m / (1 - beta1**k)) / (np.sqrt(v / (1 - beta2**k)) + eps)
+        x = x - np.where(free, step, 0.0)
+        x[0] = min(max(x[0], 0.0), 0.999)
+        history.append((loss, x.copy()))
+    return x, history

@@ -1,0 +1,615 @@
+// singularity_amd/csrc/adjoint.hip — differentiable render: forward pass with
+// checkpoints and a hand-written discrete adjoint of the RK4 march.
+//
+// Forward (one thread per pixel, 1 sample at the pixel centre): initialise the
+// ray, march with the damped Kerr RK4 (kerr_hamilton.h:165-178), store the
+// state every `C` steps, and record every equator crossing that meets the
+// soft annulus (step index, opacity, emission, crossing radius).  Radiance is
+// the front-to-back composite of the crossings (plus the escape colour behind
+// them); the loss is sum_pixels w_p |I_p - T_p|^2.
+//
+// Backward (one thread per pixel): the crossing gradients follow from the
+// composite; emission and opacity Jacobians come from forward-mode duals; the
+// state adjoint is swept back through the march segment by segment, each
+// segment recomputed from its checkpoint and reversed with kerr_step_vjp.  The
+// camera initialisation's dependence on (a, inclination) is differentiated in
+// forward mode.  Per-pixel gradients are reduced in a fixed order.
+//
+// Mixed precision: the FP32 pass hands captured, photon-band and grazing rays
+// to an FP64 pass, exactly as the render path does; each ray's backward runs in
+// the precision of its forward.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <type_traits>
+
+#include "../../include/sing_abi.h"
+#include "adjoint.hpp"
+#include "context.hpp"
+#include "diff_scene.hpp"
+#include "physics.hpp"
+
+namespace sing {
+
+namespace {
+
+constexpr float kGrazeThetaDiff = 2e-6f;
+constexpr std::uint32_t kMetaPrec64 = 1u;  // meta[3]: precision of the pixel's forward
+
+template <class R> __device__ __forceinline__ const DiffScene<R>& scene_of(const DiffSet* ds);
+template <> __device__ __forceinline__ const DiffScene<float>& scene_of<float>(const DiffSet* ds) { return ds->sf; }
+template <> __device__ __forceinline__ const DiffScene<double>& scene_of<double>(const DiffSet* ds) { return ds->sd; }
+template <class R> __device__ __forceinline__ const DiffScene<Dual<2, R>>& tangent_scene_of(const DiffSet* ds);
+template <> __device__ __forceinline__ const DiffScene<Dual<2, float>>& tangent_scene_of<float>(const DiffSet* ds) {
+    return ds->tf;
+}
+template <> __device__ __forceinline__ const DiffScene<Dual<2, double>>& tangent_scene_of<double>(const DiffSet* ds) {
+    return ds->td;
+}
+
+template <class R> __device__ __forceinline__ void store_state(double* p, const KY<R>& y) {
+    p[0] = double(y.r);
+    p[1] = double(y.th);
+    p[2] = double(y.ph);
+    p[3] = double(y.pr);
+    p[4] = double(y.pt);
+}
+template <class R> __device__ __forceinline__ KY<R> load_state(const double* p) {
+    return {R(p[0]), R(p[1]), R(p[2]), R(p[3]), R(p[4])};
+}
+
+// MODE 0: FP32 over all pixels; 1: FP32 with FP64 deferral; 2: FP64 over all
+// pixels; 3: FP64 over the deferred list.
+template <class R, int MODE>
+__global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict__ ds, DiffBuffers b) {
+    const std::uint32_t idx = blockIdx.x * 256u + threadIdx.x;
+    std::uint32_t pix;
+    if (MODE == 3) {
+        if (idx >= *b.count) return;
+        pix = b.list[idx];
+    } else {
+        if (idx >= b.P) return;
+        pix = idx;
+    }
+    const DiffScene<R>& sc = scene_of<R>(ds);
+    const std::uint32_t gx = pix % sc.width, gy = pix / sc.width;
+    std::uint32_t* meta = b.meta + 4u * pix;
+    double* cp = b.cp + std::size_t(pix) * b.ncp * 5u;
+    double* cr = b.cross + std::size_t(pix) * kMaxCross * 7u;
+    KY<R> y;
+    R E, L;
+    const bool ok = diff_init(sc, gx, gy, y, E, L);
+    meta[3] = (MODE >= 2) ? kMetaPrec64 : 0u;
+    if (ok && MODE == 1) {
+        // photon-orbit margin of the ray (as in the render path)
+        float st, ct;
+        dsincos(float(sc.cam_theta), &st, &ct);
+        const float Ef = float(E), Lf = float(L), ptf = float(y.pt);
+        const float Q = ptf * ptf + ct * ct * (Lf * Lf / fmaxf(st * st, 1e-6f) - float(sc.a * sc.a) * Ef * Ef);
+        const float Mf = float(sc.M), af = float(sc.a);
+        const float rp = Mf + sqrtf(fmaxf(Mf * Mf - af * af, 0.0f));
+        const float m = photon_margin<float>(Lf / Ef, Q / (Ef * Ef), af, Mf, rp);
+        const bool defer = (m < 0.0f) ? (y.pr < R(0)) : (m < 0.4f);
+        if (defer) {
+            meta[3] = kMetaPrec64;
+            b.list[atomicAdd(b.count, 1u)] = pix;
+            return;
+        }
+    }
+    R acc[3] = {R(0), R(0), R(0)};
+    R tau = R(1);
+    std::uint32_t ncross = 0, step = 0, term = kTermInitFail;
+    double sky[3] = {0.0, 0.0, 0.0};
+    R near_eq = R(1e30);
+    if (ok) {
+        const KC<R> c{E, L, sc.a, sc.M};
+        store_state(cp, y);
+        R prev_r = y.r, prev_th = y.th;
+        const R mid = R(1.5707963267948966192313216916397514);
+        for (;;) {
+            if (step >= sc.max_steps) {
+                term = kTermBudget;
+                break;
+            }
+            if (!__builtin_isfinite(y.r) || y.r < sc.cut) {
+                term = kTermHorizon;
+                break;
+            }
+            if (y.r > sc.escape_r) {
+                term = kTermEscape;
+                break;
+            }
+            if (sc.flags & SING_FLAG_DISC_ON) {
+                const R dp = prev_th - mid, dc = y.th - mid;
+                if (dp * dc < R(0)) {
+                    const R frac = (mid - prev_th) / (y.th - prev_th);
+                    const R rc = prev_r + frac * (y.r - prev_r);
+                    const R alpha = soft_alpha(rc, sc.r_in, sc.r_out, sc.w);
+                    if (alpha > R(0)) {
+                        R col[3];
+                        diff_emit(rc, y.ph, L, sc.a, sc.M, sc.r_in, sc.peak_T, sc.f_obs, sc.turbulence, sc.time_sec,
+                                  sc.flags, col);
+                        double* rec = cr + 7u * ncross;
+                        rec[0] = double(step);
+                        rec[1] = double(alpha);
+                        rec[2] = double(col[0]);
+                        rec[3] = double(col[1]);
+                        rec[4] = double(col[2]);
+                        rec[5] = double(rc);
+                        rec[6] = double(y.ph);
+                        for (int k = 0; k < 3; ++k) acc[k] = acc[k] + tau * alpha * col[k];
+                        ++ncross;
+                        if (!(alpha < R(1)) || ncross == std::uint32_t(kMaxCross)) {
+                            term = kTermDisc;
+                            break;
+                        }
+                        tau = tau * (R(1) - alpha);
+                    }
+                }
+            }
+            prev_th = y.th;
+            prev_r = y.r;
+            y = kerr_step(y, c, sc.h);
+            ++step;
+            if (step % b.C == 0u) store_state(cp + 5u * (step / b.C), y);
+            if (MODE == 1) {
+                const R d = y.th - mid;
+                const R ad = d < R(0) ? -d : d;
+                if (ad < near_eq) near_eq = ad;
+            }
+        }
+        if (MODE == 1 && float(near_eq) < kGrazeThetaDiff) {
+            meta[3] = kMetaPrec64;
+            b.list[atomicAdd(b.count, 1u)] = pix;
+            return;
+        }
+        if (term == kTermEscape && (sc.flags & SING_FLAG_STARFIELD_ON)) {
+            SceneConst<float> sk{};
+            sk.flags = sc.flags;
+            float c3[3];
+            shade_escape<float>(sk, float(y.th), float(y.ph), c3);
+            for (int k = 0; k < 3; ++k) {
+                sky[k] = c3[k];
+                acc[k] = acc[k] + tau * R(c3[k]);
+            }
+        }
+    }
+    meta[0] = step;
+    meta[1] = ncross;
+    meta[2] = term;
+    double* skp = b.sky + 3u * pix;
+    const float* tgt = b.target + 3u * pix;
+    const double w = b.weight ? double(b.weight[pix]) : 1.0;
+    double l = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        skp[k] = sky[k];
+        b.hdr[3u * pix + k] = float(acc[k]);
+        const double diff = double(acc[k]) - double(tgt[k]);
+        b.gI[3u * pix + k] = 2.0 * w * diff;
+        l += w * diff * diff;
+    }
+    b.loss[pix] = l;
+    b.steps[pix] = step;
+}
+
+template <class R, int MODE>
+__global__ __launch_bounds__(256) void diff_bwd_kernel(const DiffSet* __restrict__ ds, DiffBuffers b) {
+    const std::uint32_t idx = blockIdx.x * 256u + threadIdx.x;
+    std::uint32_t pix;
+    if (MODE == 1) {
+        if (idx >= *b.count) return;
+        pix = b.list[idx];
+    } else {
+        if (idx >= b.P) return;
+        pix = idx;
+    }
+    const std::uint32_t* meta = b.meta + 4u * pix;
+    double* g = b.grad + 5u * pix;
+    const bool want64 = std::is_same<R, double>::value;
+    if ((meta[3] == kMetaPrec64) != want64) return;
+    for (int k = 0; k < 5; ++k) g[k] = 0.0;
+    const DiffScene<R>& sc = scene_of<R>(ds);
+    const std::uint32_t gx = pix % sc.width, gy = pix / sc.width;
+    // camera initialisation and its tangents wrt (a/M, inclination)
+    KY<Dual<2, R>> y0;
+    Dual<2, R> Ed, Ld;
+    if (!diff_init(tangent_scene_of<R>(ds), gx, gy, y0, Ed, Ld)) return;
+    const std::uint32_t n = meta[0], J = meta[1], term = meta[2];
+    if (J == 0u) return;  // no crossing: the radiance has no parameter dependence
+    const KC<R> c{Ed.v, Ld.v, sc.a, sc.M};
+    const double* gI = b.gI + 3u * pix;
+    const double* cr = b.cross + std::size_t(pix) * kMaxCross * 7u;
+
+    // ---- crossing gradients (front-to-back composite)
+    R tau[kMaxCross];
+    R t = R(1);
+    for (std::uint32_t j = 0; j < J; ++j) {
+        tau[j] = t;
+        t = t * (R(1) - R(cr[7u * j + 1]));
+    }
+    const bool sky_behind = (term == kTermEscape);
+    R behind[3] = {R(0), R(0), R(0)};
+    if (sky_behind)
+        for (int k = 0; k < 3; ++k) behind[k] = R(b.sky[3u * pix + k]);
+    R rc_bar[kMaxCross], ph_bar[kMaxCross];
+    R gL = R(0), g_a = R(0), g_rin = R(0), g_rout = R(0), g_ts = R(0);
+    const R Ts = R(ds->tscale), T0 = R(ds->T0);
+    for (int j = int(J) - 1; j >= 0; --j) {
+        const double* rec = cr + 7u * j;
+        const R alpha = R(rec[1]);
+        const R col[3] = {R(rec[2]), R(rec[3]), R(rec[4])};
+        R cbar[3], abar = R(0);
+        for (int k = 0; k < 3; ++k) {
+            cbar[k] = R(gI[k]) * tau[j] * alpha;
+            abar = abar + R(gI[k]) * tau[j] * (col[k] - behind[k]);
+        }
+        for (int k = 0; k < 3; ++k) behind[k] = alpha * col[k] + (R(1) - alpha) * behind[k];
+        // emission Jacobian wrt (rc, phi, L, a, r_in, Ts) at the recorded crossing
+        using D6 = Dual<6, R>;
+        const R rc = R(rec[5]);
+        const R phi_k = R(rec[6]);
+        D6 rgb[3];
+        diff_emit<D6>(D6::var(rc, 0), D6::var(phi_k, 1), D6::var(Ld.v, 2), D6::var(sc.a, 3), D6(sc.M),
+                      D6::var(sc.r_in, 4), D6(T0) * D6::var(Ts, 5), D6(sc.f_obs), D6(sc.turbulence), D6(sc.time_sec),
+                      sc.flags, rgb);
+        using D3 = Dual<3, R>;
+        const D3 al = soft_alpha(D3::var(rc, 0), D3::var(sc.r_in, 1), D3::var(sc.r_out, 2), D3(sc.w));
+        R grc = abar * al.d[0], gph = R(0);
+        for (int k = 0; k < 3; ++k) {
+            grc = grc + cbar[k] * rgb[k].d[0];
+            gph = gph + cbar[k] * rgb[k].d[1];
+            gL = gL + cbar[k] * rgb[k].d[2];
+            g_a = g_a + cbar[k] * rgb[k].d[3];
+            g_rin = g_rin + cbar[k] * rgb[k].d[4];
+            g_ts = g_ts + cbar[k] * rgb[k].d[5];
+        }
+        g_rin = g_rin + abar * al.d[1];
+        g_rout = g_rout + abar * al.d[2];
+        rc_bar[j] = grc;
+        ph_bar[j] = gph;
+    }
+
+    // ---- adjoint sweep through the march
+    KY<R> lam = {R(0), R(0), R(0), R(0), R(0)};
+    KG<R> gc = {R(0), R(0), R(0)};
+    gc.L = gL;
+    const double* cp = b.cp + std::size_t(pix) * b.ncp * 5u;
+    double* scr = b.scratch + std::size_t(pix) * (b.C + 1u) * 5u;
+    int jn = int(J) - 1;  // next crossing (from the end) whose seeds are pending
+    if (n > 0) {
+        const std::uint32_t nseg = (n + b.C - 1u) / b.C;
+        for (int seg = int(nseg) - 1; seg >= 0; --seg) {
+            const std::uint32_t k0 = std::uint32_t(seg) * b.C;
+            const std::uint32_t k1 = (k0 + b.C < n) ? k0 + b.C : n;
+            KY<R> ys = load_state<R>(cp + 5u * seg);
+            store_state(scr, ys);
+            for (std::uint32_t i = 1; i <= k1 - k0; ++i) {
+                ys = kerr_step(ys, c, sc.h);
+                store_state(scr + 5u * i, ys);
+            }
+            for (std::uint32_t k = k1; k > k0; --k) {
+                const KY<R> yk = load_state<R>(scr + 5u * (k - k0));
+                const KY<R> ykm = load_state<R>(scr + 5u * (k - 1u - k0));
+                R dth = R(0), seed_prev_r = R(0), seed_prev_th = R(0);
+                bool has = false;
+                if (jn >= 0 && std::uint32_t(cr[7u * jn]) == k) {
+                    has = true;
+                    const R mid = R(1.5707963267948966192313216916397514);
+                    dth = yk.th - ykm.th;
+                    const R f = (mid - ykm.th) / dth;
+                    const R dr = yk.r - ykm.r;
+                    const R gb = rc_bar[jn];
+                    lam.r = lam.r + gb * f;
+                    lam.th = lam.th + gb * dr * (-f / dth);
+                    lam.ph = lam.ph + ph_bar[jn];
+                    seed_prev_r = gb * (R(1) - f);
+                    seed_prev_th = gb * dr * (f - R(1)) / dth;
+                    --jn;
+                }
+                lam = kerr_step_vjp(ykm, c, sc.h, lam, gc);
+                if (has) {
+                    lam.r = lam.r + seed_prev_r;
+                    lam.th = lam.th + seed_prev_th;
+                }
+            }
+        }
+    }
+
+    // ---- camera initialisation: dJ/dp = lam . dy0/dp + gE dE/dp + gL dL/dp (+ direct terms)
+    const R M = sc.M;
+    for (int p = 0; p < 2; ++p) {
+        R v = lam.r * y0.r.d[p] + lam.th * y0.th.d[p] + lam.ph * y0.ph.d[p] + lam.pr * y0.pr.d[p] +
+              lam.pt * y0.pt.d[p] + gc.E * Ed.d[p] + gc.L * Ld.d[p];
+        g[p] = double(v);
+    }
+    g[0] += double((gc.a + g_a) * M);  // a = (a/M) M
+    g[2] = double(g_rin * M);
+    g[3] = double(g_rout * M);
+    g[4] = double(g_ts);
+}
+
+// Fixed-order reduction of per-pixel loss (1) and gradients (5) into out[6].
+__global__ __launch_bounds__(1024) void diff_reduce_kernel(const double* __restrict__ loss,
+                                                           const double* __restrict__ grad, std::uint32_t P,
+                                                           double* __restrict__ out) {
+    __shared__ double sh[6][1024];
+    const std::uint32_t t = threadIdx.x;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (std::uint32_t i = t; i < P; i += 1024u) {
+        s[0] += loss[i];
+        for (int k = 0; k < 5; ++k) s[1 + k] += grad[5u * i + k];
+    }
+    for (int k = 0; k < 6; ++k) sh[k][t] = s[k];
+    __syncthreads();
+    for (std::uint32_t w = 512; w > 0; w >>= 1) {
+        if (t < w)
+            for (int k = 0; k < 6; ++k) sh[k][t] += sh[k][t + w];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int k = 0; k < 6; ++k) out[k] = sh[k][0];
+}
+
+}  // namespace
+
+hipError_t launch_diff(const DiffSet* ds_dev, const DiffBuffers& b, std::uint32_t precision, bool backward,
+                       double* out6, hipStream_t stream, hipEvent_t ev_mid) {
+    const dim3 block(256), grid((b.P + 255u) / 256u);
+    if (precision == SING_PREC_FP32) {
+        hipLaunchKernelGGL((diff_fwd_kernel<float, 0>), grid, block, 0, stream, ds_dev, b);
+    } else if (precision == SING_PREC_FP64) {
+        hipLaunchKernelGGL((diff_fwd_kernel<double, 2>), grid, block, 0, stream, ds_dev, b);
+    } else {
+        hipLaunchKernelGGL((diff_fwd_kernel<float, 1>), grid, block, 0, stream, ds_dev, b);
+        hipLaunchKernelGGL((diff_fwd_kernel<double, 3>), grid, block, 0, stream, ds_dev, b);
+    }
+    if (ev_mid) (void)hipEventRecord(ev_mid, stream);
+    if (backward) {
+        if (precision == SING_PREC_FP32) {
+            hipLaunchKernelGGL((diff_bwd_kernel<float, 0>), grid, block, 0, stream, ds_dev, b);
+        } else if (precision == SING_PREC_FP64) {
+            hipLaunchKernelGGL((diff_bwd_kernel<double, 0>), grid, block, 0, stream, ds_dev, b);
+        } else {
+            hipLaunchKernelGGL((diff_bwd_kernel<float, 0>), grid, block, 0, stream, ds_dev, b);
+            hipLaunchKernelGGL((diff_bwd_kernel<double, 1>), grid, block, 0, stream, ds_dev, b);
+        }
+    }
+    hipLaunchKernelGGL(diff_reduce_kernel, dim3(1), dim3(1024), 0, stream, b.loss, b.grad, b.P, out6);
+    return hipGetLastError();
+}
+
+}  // namespace sing
+
+// ============================================================= host side
+void SingContext::free_diff() {
+    (void)hipFree(d_dset);
+    (void)hipFree(diff.cp);
+    (void)hipFree(diff.cross);
+    (void)hipFree(diff.meta);
+    (void)hipFree(diff.sky);
+    (void)hipFree(diff.gI);
+    (void)hipFree(diff.hdr);
+    (void)hipFree(diff.loss);
+    (void)hipFree(diff.grad);
+    (void)hipFree(diff.scratch);
+    (void)hipFree(diff.list);
+    (void)hipFree(diff.count);
+    (void)hipFree(diff.steps);
+    (void)hipFree(d_target);
+    (void)hipFree(d_weight);
+    (void)hipFree(d_out6);
+    d_dset = nullptr;
+    diff = sing::DiffBuffers{};
+    d_target = d_weight = nullptr;
+    d_out6 = nullptr;
+    diff_P = diff_ncp = diff_C = 0;
+}
+
+int SingContext::ensure_diff(std::size_t P, std::size_t ncp, std::size_t C) {
+    if (P == diff_P && ncp <= diff_ncp && C == diff_C && d_dset) return SING_OK;
+    free_diff();
+    hipError_t e = hipSuccess;
+    auto A = [&](void** p, std::size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(p, bytes);
+    };
+    A((void**)&d_dset, sizeof(sing::DiffSet));
+    A((void**)&diff.cp, P * ncp * 5 * sizeof(double));
+    A((void**)&diff.cross, P * sing::kMaxCross * 7 * sizeof(double));
+    A((void**)&diff.meta, P * 4 * sizeof(std::uint32_t));
+    A((void**)&diff.sky, P * 3 * sizeof(double));
+    A((void**)&diff.gI, P * 3 * sizeof(double));
+    A((void**)&diff.hdr, P * 3 * sizeof(float));
+    A((void**)&diff.loss, P * sizeof(double));
+    A((void**)&diff.grad, P * 5 * sizeof(double));
+    A((void**)&diff.scratch, P * (C + 1) * 5 * sizeof(double));
+    A((void**)&diff.list, P * sizeof(std::uint32_t));
+    A((void**)&diff.count, sizeof(std::uint32_t));
+    A((void**)&diff.steps, P * sizeof(std::uint32_t));
+    A((void**)&d_target, P * 3 * sizeof(float));
+    A((void**)&d_weight, P * sizeof(float));
+    A((void**)&d_out6, 6 * sizeof(double));
+    if (e != hipSuccess) {
+        free_diff();
+        return fail(SING_ERR_NOMEM, "hipMalloc of differentiable-render buffers failed: %s", hipGetErrorString(e));
+    }
+    diff.P = std::uint32_t(P);
+    diff.ncp = std::uint32_t(ncp);
+    diff.C = std::uint32_t(C);
+    diff_P = P;
+    diff_ncp = ncp;
+    diff_C = C;
+    return SING_OK;
+}
+
+namespace {
+
+int diff_run(SingContext* c, const SingUniforms* base, const SingDiffParams* p, const SingDiffCamera* cam,
+             const SingDiffOpts* opts_in, const float* target, const float* weights, bool backward, double* loss,
+             double* grad, float* hdr_out, SingDiffStats* stats) {
+    if (!c) return SING_ERR_ARG;
+    if (!base || !p || !cam) return c->fail(SING_ERR_ARG, "sing_diff: null argument");
+    if (base->width != c->width || base->height != c->height)
+        return c->fail(SING_ERR_ARG, "sing_diff: base size %ux%u != context %ux%u", base->width, base->height,
+                       c->width, c->height);
+    SingDiffOpts o{};
+    if (opts_in) o = *opts_in;
+    if (o.precision > SING_PREC_MIXED) return c->fail(SING_ERR_ARG, "sing_diff: unknown precision %u", o.precision);
+    const double w = o.soft_edge > 0.0 ? o.soft_edge : 0.5;
+    const std::size_t C = o.checkpoint_every ? o.checkpoint_every : 64u;
+    if (C > 4096) return c->fail(SING_ERR_ARG, "sing_diff: checkpoint_every too large");
+    if (!(p->spin >= 0.0 && p->spin < 1.0)) return c->fail(SING_ERR_ARG, "sing_diff: spin must be in [0, 1)");
+    if (!(p->r_out > p->r_in) || !(p->t_scale > 0.0) || !(cam->distance > 0.0))
+        return c->fail(SING_ERR_ARG, "sing_diff: need r_out > r_in, t_scale > 0, distance > 0");
+    if (backward && !target) return c->fail(SING_ERR_ARG, "sing_diff: null target");
+    const std::size_t P = std::size_t(c->width) * c->height;
+    const std::size_t ncp = base->max_steps / C + 1;
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return c->hip_fail(e, "hipSetDevice");
+    int rc = c->ensure_diff(P, ncp, C);
+    if (rc != SING_OK) return rc;
+    if ((rc = c->ensure_events(4)) != SING_OK) return rc;
+
+    // scene in all precisions (host, double duals for the tangents)
+    sing::DiffSet ds{};
+    using D2 = sing::Dual<2, double>;
+    const sing::DiffScene<D2> td = sing::make_diff_scene<D2>(*base, D2::var(p->spin, 0), D2::var(p->incl, 1),
+                                                             D2(p->r_in), D2(p->r_out), D2(p->t_scale),
+                                                             cam->distance, cam->azimuth, cam->fov_y_deg, w);
+    ds.td = td;
+    ds.sd = sing::make_diff_scene<double>(*base, p->spin, p->incl, p->r_in, p->r_out, p->t_scale, cam->distance,
+                                          cam->azimuth, cam->fov_y_deg, w);
+    // float copies are the double values rounded (same discrete model)
+    auto to_f = [](const sing::DiffScene<double>& d) {
+        sing::DiffScene<float> f{};
+        for (int i = 0; i < 3; ++i) {
+            f.fwd[i] = float(d.fwd[i]); f.right[i] = float(d.right[i]); f.up[i] = float(d.up[i]);
+            f.rhat[i] = float(d.rhat[i]); f.that[i] = float(d.that[i]); f.phat[i] = float(d.phat[i]);
+        }
+        f.tan_half_fov = float(d.tan_half_fov); f.aspect = float(d.aspect); f.width_f = float(d.width_f);
+        f.height_f = float(d.height_f); f.cam_r = float(d.cam_r); f.cam_theta = float(d.cam_theta);
+        f.cam_phi = float(d.cam_phi); f.uphi_den = float(d.uphi_den); f.g_tt = float(d.g_tt); f.g_tp = float(d.g_tp);
+        f.g_rr = float(d.g_rr); f.g_thth = float(d.g_thth); f.g_pp = float(d.g_pp); f.M = float(d.M); f.a = float(d.a);
+        f.r_in = float(d.r_in); f.r_out = float(d.r_out); f.peak_T = float(d.peak_T); f.f_obs = float(d.f_obs);
+        f.turbulence = float(d.turbulence); f.time_sec = float(d.time_sec); f.w = float(d.w); f.h = float(d.h);
+        f.escape_r = float(d.escape_r); f.cut = float(d.cut);
+        f.width = d.width; f.height = d.height; f.flags = d.flags; f.max_steps = d.max_steps;
+        return f;
+    };
+    ds.sf = to_f(ds.sd);
+    {
+        // Dual<2, float> scene: values and tangents rounded from the double duals
+        const D2* src = &td.fwd[0];
+        sing::Dual<2, float>* dst = &ds.tf.fwd[0];
+        constexpr std::size_t n_fields = (offsetof(sing::DiffScene<D2>, width)) / sizeof(D2);
+        for (std::size_t i = 0; i < n_fields; ++i) {
+            dst[i].v = float(src[i].v);
+            dst[i].d[0] = float(src[i].d[0]);
+            dst[i].d[1] = float(src[i].d[1]);
+        }
+        ds.tf.width = td.width; ds.tf.height = td.height; ds.tf.flags = td.flags; ds.tf.max_steps = td.max_steps;
+    }
+    ds.tscale = p->t_scale;
+    ds.T0 = base->disc_peak_T;
+
+    if ((e = hipMemcpyAsync(c->d_dset, &ds, sizeof(ds), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+        return c->hip_fail(e, "hipMemcpyAsync(scene)");
+    if (target) {
+        if ((e = hipMemcpyAsync(c->d_target, target, P * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream)) !=
+            hipSuccess)
+            return c->hip_fail(e, "hipMemcpyAsync(target)");
+    } else if ((e = hipMemsetAsync(c->d_target, 0, P * 3 * sizeof(float), c->stream)) != hipSuccess) {
+        return c->hip_fail(e, "hipMemsetAsync(target)");
+    }
+    if (weights &&
+        (e = hipMemcpyAsync(c->d_weight, weights, P * sizeof(float), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+        return c->hip_fail(e, "hipMemcpyAsync(weights)");
+    if ((e = hipMemsetAsync(c->diff.count, 0, sizeof(std::uint32_t), c->stream)) != hipSuccess)
+        return c->hip_fail(e, "hipMemsetAsync(count)");
+    if (!backward && (e = hipMemsetAsync(c->diff.grad, 0, P * 5 * sizeof(double), c->stream)) != hipSuccess)
+        return c->hip_fail(e, "hipMemsetAsync(grad)");
+    sing::DiffBuffers b = c->diff;
+    b.target = c->d_target;
+    b.weight = weights ? c->d_weight : nullptr;
+    (void)hipEventRecord(c->events[0], c->stream);
+    if ((e = sing::launch_diff(c->d_dset, b, o.precision, backward, c->d_out6, c->stream, c->events[1])) !=
+        hipSuccess)
+        return c->hip_fail(e, "differentiable render launch");
+    (void)hipEventRecord(c->events[2], c->stream);
+    double out6[6] = {0, 0, 0, 0, 0, 0};
+    std::uint32_t n64 = 0;
+    if ((e = hipMemcpyAsync(out6, c->d_out6, sizeof(out6), hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(&n64, c->diff.count, sizeof(n64), hipMemcpyDeviceToHost, c->stream)) != hipSuccess)
+        return c->hip_fail(e, "hipMemcpyAsync(result)");
+    if (hdr_out &&
+        (e = hipMemcpyAsync(hdr_out, c->diff.hdr, P * 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream)) !=
+            hipSuccess)
+        return c->hip_fail(e, "hipMemcpyAsync(hdr)");
+    std::vector<std::uint32_t> steps;
+    if (stats) {
+        steps.resize(P);
+        if ((e = hipMemcpyAsync(steps.data(), c->diff.steps, P * sizeof(std::uint32_t), hipMemcpyDeviceToHost,
+                                c->stream)) != hipSuccess)
+            return c->hip_fail(e, "hipMemcpyAsync(steps)");
+    }
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return c->hip_fail(e, "hipStreamSynchronize");
+    if (loss) *loss = out6[0];
+    if (grad)
+        for (int k = 0; k < 5; ++k) grad[k] = out6[1 + k];
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        float a = 0.0f, bms = 0.0f;
+        (void)hipEventElapsedTime(&a, c->events[0], c->events[1]);
+        (void)hipEventElapsedTime(&bms, c->events[1], c->events[2]);
+        stats->forward_ms = a;
+        stats->backward_ms = bms;
+        stats->rays = P;
+        std::uint64_t s = 0;
+        for (std::uint32_t v : steps) s += v;
+        stats->steps = s;
+        stats->fp64_rays = (o.precision == SING_PREC_FP64) ? P : (o.precision == SING_PREC_MIXED ? n64 : 0);
+    }
+    return SING_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sing_diff_forward(SingContext* ctx, const SingUniforms* base, const SingDiffParams* p, const SingDiffCamera* cam,
+                      const SingDiffOpts* opts, float* hdr_out, SingDiffStats* stats) {
+    return diff_run(ctx, base, p, cam, opts, nullptr, nullptr, false, nullptr, nullptr, hdr_out, stats);
+}
+
+int sing_diff_loss_grad(SingContext* ctx, const SingUniforms* base, const SingDiffParams* p,
+                        const SingDiffCamera* cam, const SingDiffOpts* opts, const float* target,
+                        const float* weights, double* loss, double* grad, float* hdr_out, SingDiffStats* stats) {
+    return diff_run(ctx, base, p, cam, opts, target, weights, true, loss, grad, hdr_out, stats);
+}
+
+// ---- diagnostics: the adjoint math evaluated on the host
+void sing_diag_kerr_rhs(const double* y, const double* c, double* out) {
+    const sing::KY<double> d = sing::kerr_rhs<double>({y[0], y[1], y[2], y[3], y[4]}, {c[0], c[1], c[2], c[3]});
+    out[0] = d.r; out[1] = d.th; out[2] = d.ph; out[3] = d.pr; out[4] = d.pt;
+}
+void sing_diag_kerr_rhs_vjp(const double* y, const double* c, const double* v, double* gy, double* gc) {
+    sing::KY<double> g = {0, 0, 0, 0, 0};
+    sing::KG<double> k = {0, 0, 0};
+    sing::kerr_rhs_vjp<double>({y[0], y[1], y[2], y[3], y[4]}, {c[0], c[1], c[2], c[3]}, {v[0], v[1], v[2], v[3], v[4]},
+                               g, k);
+    gy[0] = g.r; gy[1] = g.th; gy[2] = g.ph; gy[3] = g.pr; gy[4] = g.pt;
+    gc[0] = k.E; gc[1] = k.L; gc[2] = k.a;
+}
+void sing_diag_kerr_step(const double* y, const double* c, double h, double* out) {
+    const sing::KY<double> d = sing::kerr_step<double>({y[0], y[1], y[2], y[3], y[4]}, {c[0], c[1], c[2], c[3]}, h);
+    out[0] = d.r; out[1] = d.th; out[2] = d.ph; out[3] = d.pr; out[4] = d.pt;
+}
+void sing_diag_kerr_step_vjp(const double* y, const double* c, double h, const double* lam, double* gy, double* gc) {
+    sing::KG<double> k = {0, 0, 0};
+    const sing::KY<double> g = sing::kerr_step_vjp<double>({y[0], y[1], y[2], y[3], y[4]}, {c[0], c[1], c[2], c[3]},
+                                                          h, {lam[0], lam[1], lam[2], lam[3], lam[4]}, k);
+    gy[0] = g.r; gy[1] = g.th; gy[2] = g.ph; gy[3] = g.pr; gy[4] = g.pt;
+    gc[0] = k.E; gc[1] = k.L; gc[2] = k.a;
+}
+
+}  // extern "C"

@@ -1,0 +1,45 @@
+// singularity_amd/csrc/adjoint.hpp — host-visible declarations of the
+// differentiable render (forward with checkpoints + discrete adjoint).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "diff_scene.hpp"
+
+namespace sing {
+
+// Scene in every precision the kernels need; lives in device memory.
+struct DiffSet {
+    DiffScene<float> sf;
+    DiffScene<double> sd;
+    DiffScene<Dual<2, float>> tf;   // tangents wrt (a/M, inclination)
+    DiffScene<Dual<2, double>> td;
+    double tscale;                  // peak-temperature scale parameter
+    double T0;                      // unscaled peak temperature (K)
+};
+
+// Per-pixel device buffers of one differentiable frame.
+struct DiffBuffers {
+    double* cp;             // [P][ncp][5] states every C steps (double storage)
+    double* cross;          // [P][8][7] step, alpha, emission rgb, r_cross, phi
+    std::uint32_t* meta;    // [P][4] steps, crossings, termination, precision (1 = FP64)
+    double* sky;            // [P][3] escape colour behind the crossings
+    double* gI;             // [P][3] dLoss/dRadiance
+    float* hdr;             // [P][3] radiance
+    double* loss;           // [P]
+    double* grad;           // [P][5]
+    double* scratch;        // [P][C+1][5] recomputed segment states
+    const float* target;    // [P][3]
+    const float* weight;    // [P] or null
+    std::uint32_t* list;    // pixels of the FP64 pass
+    std::uint32_t* count;   // length of list
+    std::uint32_t* steps;   // [P] RK4 steps of the forward
+    std::uint32_t P, ncp, C;
+};
+
+hipError_t launch_diff(const DiffSet* ds_dev, const DiffBuffers& b, std::uint32_t precision, bool backward,
+                       double* out6, hipStream_t stream, hipEvent_t ev_mid);
+
+}  // namespace sing

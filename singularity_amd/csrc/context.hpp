@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/sing_abi.h"
+#include "adjoint.hpp"
 #include "trace.hpp"
 
 void sing_set_global_error(const std::string& m);
@@ -37,6 +38,13 @@ struct SingContext {
     float2* d_halton = nullptr;
     std::uint32_t outputs_valid = 0;  // SING_OUT_* of the last frame
     std::string err;
+    // differentiable render
+    sing::DiffSet* d_dset = nullptr;
+    sing::DiffBuffers diff{};
+    float* d_target = nullptr;
+    float* d_weight = nullptr;
+    double* d_out6 = nullptr;
+    std::size_t diff_P = 0, diff_ncp = 0, diff_C = 0;
 
     int fail(int code, const char* fmt, ...);
     int hip_fail(hipError_t e, const char* where);
@@ -46,4 +54,6 @@ struct SingContext {
     int ensure_events(std::size_t n);
     std::uint32_t chunk_samples(std::uint32_t spp) const;
     sing::LaunchShape launch_shape(const SingRenderOpts& o) const;
+    int ensure_diff(std::size_t P, std::size_t ncp, std::size_t C);
+    void free_diff();
 };

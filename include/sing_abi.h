@@ -251,8 +251,10 @@ typedef struct SingDiffCamera {
 typedef struct SingDiffOpts {
     uint32_t precision;        /* SING_PREC_*: FP32, FP64, or mixed (FP64 for band rays) */
     uint32_t checkpoint_every; /* RK4 steps between stored states (0 = 64) */
-    double soft_edge;          /* annulus edge half-width in M (0 = 0.5) */
-    uint32_t reserved[6];
+    double soft_edge;          /* annulus edge ramp width in M (0 = 1.0) */
+    uint32_t r_in_isco;        /* 1: r_in = prograde ISCO of a/M (p->r_in ignored); grad[0] is then the
+                                  total derivative through r_in(a), grad[2] the partial dL/d(r_in/M) */
+    uint32_t reserved[5];
 } SingDiffOpts;
 
 typedef struct SingDiffStats {

@@ -635,10 +635,12 @@ template <class T> struct RayOut {
 };
 
 template <class T> inline T soft_alpha(T r, T r_in, T r_out, T w) {
-    // Opacity of the differentiable disc model: 1 inside [r_in+w, r_out-w] ramps
-    // linearly to 0 at r_in-w / r_out+w.  w = 0 is never evaluated (hard edges).
-    const T lo = (r - (r_in - w)) / (T(2.0) * w);
-    const T hi = ((r_out + w) - r) / (T(2.0) * w);
+    // Opacity of the differentiable disc model: 1 inside [r_in+w, r_out-w], ramping
+    // linearly to 0 at r_in and r_out (zero outside the annulus, so the emission's
+    // jump at r_in — the 1000 K floor of the blackbody — is multiplied by 0).
+    // w = 0 is never evaluated (hard edges).
+    const T lo = (r - r_in) / w;
+    const T hi = (r_out - r) / w;
     return vclamp(xfmin(lo, hi), T(0.0), T(1.0));
 }
 
@@ -797,7 +799,7 @@ inline RayOut<T> trace_kerr(const Uni<T>& u, const CamFrame<T>& f, const CameraR
     o.Lz = val(c.L_z);
     o.inv0[0] = val(kerr_two_H(s.r, s.theta, s.p_r, s.p_theta, c));
     o.inv0[1] = val(kerr_carter_Q(s.theta, s.p_theta, c));
-    T prev_theta = s.theta, prev_r = s.r;
+    T prev_theta = s.theta, prev_r = s.r, prev_phi = s.phi;
     const T cut = T(1.02) * (M + xsqrt(xfmax(M * M - a * a, T(0.0))));
     V3<T> acc = v3(T(0.0), T(0.0), T(0.0));
     T trans = T(1.0);
@@ -818,7 +820,9 @@ inline RayOut<T> trace_kerr(const Uni<T>& u, const CamFrame<T>& f, const CameraR
         const Crossing<T> x = detect_crossing(prev_theta, s.theta);
         if ((u.flags & FLAG_DISC) && x.crossed) {
             const T r_cross = lerp_scalar(prev_r, s.r, x.frac);
-            const T phi_cross = s.phi;
+            // the reference shades at phi of the step after the crossing; the soft
+            // (differentiable) model interpolates it like r
+            const T phi_cross = (val(u.ext.soft_edge) > 0.0) ? lerp_scalar(prev_phi, s.phi, x.frac) : s.phi;
             const bool stop = disc_hit(u, r_cross, acc, trans, o, [&](T Tk) {
                 return kerr_disc_emit(u, r_cross, phi_cross, cam_r, c.L_z, Tk);
             });
@@ -826,6 +830,7 @@ inline RayOut<T> trace_kerr(const Uni<T>& u, const CamFrame<T>& f, const CameraR
         }
         prev_theta = s.theta;
         prev_r = s.r;
+        prev_phi = s.phi;
         s = kerr_step(s, c, u, h_cur);
         o.steps = step + 1;
         o.r_min = std::min(o.r_min, val(s.r));

@@ -345,7 +345,7 @@ class DiffCamera(C.Structure):
 
 class DiffOpts(C.Structure):
     _fields_ = [("precision", C.c_uint32), ("checkpoint_every", C.c_uint32), ("soft_edge", C.c_double),
-                ("reserved", C.c_uint32 * 6)]
+                ("r_in_isco", C.c_uint32), ("reserved", C.c_uint32 * 5)]
 
 
 class DiffStats(C.Structure):
@@ -363,13 +363,14 @@ _SIGS.update({
 
 
 def diff_base_uniforms(width: int, height: int, starfield: bool = False, peak_T: float = 4500.0,
-                       h_step: float = 0.4, max_steps: int = 2500) -> Uniforms:
+                       h_step: float = 0.4, max_steps: int = 2500, escape_M: float = 100.0) -> Uniforms:
     """Base block of the differentiable Kerr scene: resolution, integrator and shading settings."""
     s = scene_defaults()
     s.metric = METRIC_KERR
     s.disc_peak_T_K = peak_T
     s.h_step = h_step
     s.max_steps = max_steps
+    s.escape_r_M = escape_M
     u = pack_uniforms(s, orbital_camera(), width, height, 0.0)
     if not starfield:
         u.flags &= ~FLAG_STARFIELD
@@ -406,23 +407,39 @@ def diff_loss_grad(be: "HipBackend", base: Uniforms, params: DiffParams, cam: Di
     return loss.value, grad, hdr, st
 
 
-def adam_fit(be: "HipBackend", base: Uniforms, start: np.ndarray, target: np.ndarray, cam: DiffCamera,
-             steps: int = 200, lr: np.ndarray | None = None, free: np.ndarray | None = None,
-             opts: DiffOpts | None = None, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-12):
-    """Adam on the 5 scene parameters (entries with free == False stay fixed)."""
+def adam_fit(be: "HipBackend", base: Uniforms, start, target: np.ndarray, cam: DiffCamera, steps: int = 200,
+             lr=None, free=None, opts: DiffOpts | None = None, beta1: float = 0.9, beta2: float = 0.999,
+             clip: float = 3.0, lr_floor: float = 0.05, callback=None):
+    """Adam over the 5 scene parameters (entries with free == False stay fixed).
+
+    The learning rate follows a cosine decay from lr to lr_floor * lr.  After 5
+    steps each gradient entry is clipped to clip x its running RMS: pixels whose
+    rays pass near the photon orbit have steep but finite derivatives, and one
+    such spike would otherwise inflate Adam's second moment for hundreds of
+    steps.  Returns (params, history of (loss, params, grad)).
+    """
     x = np.array(start, np.float64)
-    lr = np.array([0.01, 0.01, 0.05, 0.1, 0.01]) if lr is None else np.asarray(lr, np.float64)
-    free = np.ones(5, bool) if free is None else np.asarray(free, bool)
+    lr = np.array([0.02, 0.01, 0.05, 0.2, 0.01]) if lr is None else np.asarray(lr, np.float64)
+    tied = opts is not None and opts.r_in_isco
+    if free is None:
+        free = np.array([True, True, not tied, True, True])
+    free = np.asarray(free, bool)
     m = np.zeros(5)
     v = np.zeros(5)
     history = []
     for k in range(1, steps + 1):
         loss, g, _, _ = diff_loss_grad(be, base, DiffParams.from_array(x), cam, target, None, opts)
         g = np.where(free, g, 0.0)
+        history.append((loss, x.copy(), g.copy()))
+        if callback is not None:
+            callback(k, loss, x, g)
+        if k > 5:
+            lim = clip * np.sqrt(v / (1 - beta2 ** (k - 1)))
+            g = np.clip(g, -lim, lim)
         m = beta1 * m + (1 - beta1) * g
         v = beta2 * v + (1 - beta2) * g * g
-        step = lr * (m / (1 - beta1**k)) / (np.sqrt(v / (1 - beta2**k)) + eps)
+        sched = lr_floor + (1 - lr_floor) * 0.5 * (1 + np.cos(np.pi * k / steps))
+        step = sched * lr * (m / (1 - beta1**k)) / (np.sqrt(v / (1 - beta2**k)) + 1e-300)
         x = x - np.where(free, step, 0.0)
-        x[0] = min(max(x[0], 0.0), 0.999)
-        history.append((loss, x.copy()))
+        x[0] = min(max(x[0], 0.0), 0.99)
     return x, history

@@ -48,15 +48,18 @@ template <> __device__ __forceinline__ const DiffScene<Dual<2, double>>& tangent
     return ds->td;
 }
 
-template <class R> __device__ __forceinline__ void store_state(double* p, const KY<R>& y) {
+// Per-pixel arrays are slot-major with the pixel index fastest (element k of
+// slot s of pixel p at [(s * width + k) * P + p]), so a wavefront's accesses to
+// one element are contiguous.
+template <class R> __device__ __forceinline__ void store_state(double* p, std::size_t S, const KY<R>& y) {
     p[0] = double(y.r);
-    p[1] = double(y.th);
-    p[2] = double(y.ph);
-    p[3] = double(y.pr);
-    p[4] = double(y.pt);
+    p[S] = double(y.th);
+    p[2 * S] = double(y.ph);
+    p[3 * S] = double(y.pr);
+    p[4 * S] = double(y.pt);
 }
-template <class R> __device__ __forceinline__ KY<R> load_state(const double* p) {
-    return {R(p[0]), R(p[1]), R(p[2]), R(p[3]), R(p[4])};
+template <class R> __device__ __forceinline__ KY<R> load_state(const double* p, std::size_t S) {
+    return {R(p[0]), R(p[S]), R(p[2 * S]), R(p[3 * S]), R(p[4 * S])};
 }
 
 // MODE 0: FP32 over all pixels; 1: FP32 with FP64 deferral; 2: FP64 over all
@@ -74,9 +77,10 @@ __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict
     }
     const DiffScene<R>& sc = scene_of<R>(ds);
     const std::uint32_t gx = pix % sc.width, gy = pix / sc.width;
+    const std::size_t S = b.P;
     std::uint32_t* meta = b.meta + 4u * pix;
-    double* cp = b.cp + std::size_t(pix) * b.ncp * 5u;
-    double* cr = b.cross + std::size_t(pix) * kMaxCross * 7u;
+    double* cp = b.cp + pix;
+    double* cr = b.cross + pix;
     KY<R> y;
     R E, L;
     const bool ok = diff_init(sc, gx, gy, y, E, L);
@@ -104,8 +108,10 @@ __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict
     R near_eq = R(1e30);
     if (ok) {
         const KC<R> c{E, L, sc.a, sc.M};
-        store_state(cp, y);
-        R prev_r = y.r, prev_th = y.th;
+        b.EL[pix] = double(E);
+        b.EL[S + pix] = double(L);
+        store_state(cp, S, y);
+        R prev_r = y.r, prev_th = y.th, prev_ph = y.ph;
         const R mid = R(1.5707963267948966192313216916397514);
         for (;;) {
             if (step >= sc.max_steps) {
@@ -127,17 +133,18 @@ __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict
                     const R rc = prev_r + frac * (y.r - prev_r);
                     const R alpha = soft_alpha(rc, sc.r_in, sc.r_out, sc.w);
                     if (alpha > R(0)) {
+                        const R phc = prev_ph + frac * (y.ph - prev_ph);
                         R col[3];
-                        diff_emit(rc, y.ph, L, sc.a, sc.M, sc.r_in, sc.peak_T, sc.f_obs, sc.turbulence, sc.time_sec,
+                        diff_emit(rc, phc, L, sc.a, sc.M, sc.r_in, sc.peak_T, sc.f_obs, sc.turbulence, sc.time_sec,
                                   sc.flags, col);
-                        double* rec = cr + 7u * ncross;
+                        double* rec = cr + 7u * ncross * S;
                         rec[0] = double(step);
-                        rec[1] = double(alpha);
-                        rec[2] = double(col[0]);
-                        rec[3] = double(col[1]);
-                        rec[4] = double(col[2]);
-                        rec[5] = double(rc);
-                        rec[6] = double(y.ph);
+                        rec[S] = double(alpha);
+                        rec[2 * S] = double(col[0]);
+                        rec[3 * S] = double(col[1]);
+                        rec[4 * S] = double(col[2]);
+                        rec[5 * S] = double(rc);
+                        rec[6 * S] = double(phc);
                         for (int k = 0; k < 3; ++k) acc[k] = acc[k] + tau * alpha * col[k];
                         ++ncross;
                         if (!(alpha < R(1)) || ncross == std::uint32_t(kMaxCross)) {
@@ -150,9 +157,10 @@ __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict
             }
             prev_th = y.th;
             prev_r = y.r;
+            prev_ph = y.ph;
             y = kerr_step(y, c, sc.h);
             ++step;
-            if (step % b.C == 0u) store_state(cp + 5u * (step / b.C), y);
+            if (step % b.C == 0u) store_state(cp + 5u * (step / b.C) * S, S, y);
             if (MODE == 1) {
                 const R d = y.th - mid;
                 const R ad = d < R(0) ? -d : d;
@@ -217,16 +225,18 @@ __global__ __launch_bounds__(256) void diff_bwd_kernel(const DiffSet* __restrict
     if (!diff_init(tangent_scene_of<R>(ds), gx, gy, y0, Ed, Ld)) return;
     const std::uint32_t n = meta[0], J = meta[1], term = meta[2];
     if (J == 0u) return;  // no crossing: the radiance has no parameter dependence
-    const KC<R> c{Ed.v, Ld.v, sc.a, sc.M};
+    const std::size_t S = b.P;
+    // the forward's own constants, so the recomputed segments repeat it exactly
+    const KC<R> c{R(b.EL[pix]), R(b.EL[S + pix]), sc.a, sc.M};
     const double* gI = b.gI + 3u * pix;
-    const double* cr = b.cross + std::size_t(pix) * kMaxCross * 7u;
+    const double* cr = b.cross + pix;
 
     // ---- crossing gradients (front-to-back composite)
     R tau[kMaxCross];
     R t = R(1);
     for (std::uint32_t j = 0; j < J; ++j) {
         tau[j] = t;
-        t = t * (R(1) - R(cr[7u * j + 1]));
+        t = t * (R(1) - R(cr[(7u * j + 1u) * S]));
     }
     const bool sky_behind = (term == kTermEscape);
     R behind[3] = {R(0), R(0), R(0)};
@@ -236,9 +246,9 @@ __global__ __launch_bounds__(256) void diff_bwd_kernel(const DiffSet* __restrict
     R gL = R(0), g_a = R(0), g_rin = R(0), g_rout = R(0), g_ts = R(0);
     const R Ts = R(ds->tscale), T0 = R(ds->T0);
     for (int j = int(J) - 1; j >= 0; --j) {
-        const double* rec = cr + 7u * j;
-        const R alpha = R(rec[1]);
-        const R col[3] = {R(rec[2]), R(rec[3]), R(rec[4])};
+        const double* rec = cr + 7u * std::size_t(j) * S;
+        const R alpha = R(rec[S]);
+        const R col[3] = {R(rec[2 * S]), R(rec[3 * S]), R(rec[4 * S])};
         R cbar[3], abar = R(0);
         for (int k = 0; k < 3; ++k) {
             cbar[k] = R(gI[k]) * tau[j] * alpha;
@@ -247,10 +257,10 @@ __global__ __launch_bounds__(256) void diff_bwd_kernel(const DiffSet* __restrict
         for (int k = 0; k < 3; ++k) behind[k] = alpha * col[k] + (R(1) - alpha) * behind[k];
         // emission Jacobian wrt (rc, phi, L, a, r_in, Ts) at the recorded crossing
         using D6 = Dual<6, R>;
-        const R rc = R(rec[5]);
-        const R phi_k = R(rec[6]);
+        const R rc = R(rec[5 * S]);
+        const R phi_k = R(rec[6 * S]);
         D6 rgb[3];
-        diff_emit<D6>(D6::var(rc, 0), D6::var(phi_k, 1), D6::var(Ld.v, 2), D6::var(sc.a, 3), D6(sc.M),
+        diff_emit<D6>(D6::var(rc, 0), D6::var(phi_k, 1), D6::var(c.L, 2), D6::var(sc.a, 3), D6(sc.M),
                       D6::var(sc.r_in, 4), D6(T0) * D6::var(Ts, 5), D6(sc.f_obs), D6(sc.turbulence), D6(sc.time_sec),
                       sc.flags, rgb);
         using D3 = Dual<3, R>;
@@ -274,43 +284,47 @@ __global__ __launch_bounds__(256) void diff_bwd_kernel(const DiffSet* __restrict
     KY<R> lam = {R(0), R(0), R(0), R(0), R(0)};
     KG<R> gc = {R(0), R(0), R(0)};
     gc.L = gL;
-    const double* cp = b.cp + std::size_t(pix) * b.ncp * 5u;
-    double* scr = b.scratch + std::size_t(pix) * (b.C + 1u) * 5u;
+    const double* cp = b.cp + pix;
+    double* scr = b.scratch + pix;
     int jn = int(J) - 1;  // next crossing (from the end) whose seeds are pending
     if (n > 0) {
         const std::uint32_t nseg = (n + b.C - 1u) / b.C;
         for (int seg = int(nseg) - 1; seg >= 0; --seg) {
             const std::uint32_t k0 = std::uint32_t(seg) * b.C;
             const std::uint32_t k1 = (k0 + b.C < n) ? k0 + b.C : n;
-            KY<R> ys = load_state<R>(cp + 5u * seg);
-            store_state(scr, ys);
+            KY<R> ys = load_state<R>(cp + 5u * std::size_t(seg) * S, S);
+            store_state(scr, S, ys);
             for (std::uint32_t i = 1; i <= k1 - k0; ++i) {
                 ys = kerr_step(ys, c, sc.h);
-                store_state(scr + 5u * i, ys);
+                store_state(scr + 5u * i * S, S, ys);
             }
             for (std::uint32_t k = k1; k > k0; --k) {
-                const KY<R> yk = load_state<R>(scr + 5u * (k - k0));
-                const KY<R> ykm = load_state<R>(scr + 5u * (k - 1u - k0));
-                R dth = R(0), seed_prev_r = R(0), seed_prev_th = R(0);
+                const KY<R> yk = load_state<R>(scr + 5u * (k - k0) * S, S);
+                const KY<R> ykm = load_state<R>(scr + 5u * (k - 1u - k0) * S, S);
+                // crossing between y_{k-1} and y_k: x_c = x_{k-1} + f (x_k - x_{k-1}) for
+                // x = r, phi with f = (pi/2 - theta_{k-1}) / (theta_k - theta_{k-1})
+                R seed_prev_r = R(0), seed_prev_th = R(0), seed_prev_ph = R(0);
                 bool has = false;
-                if (jn >= 0 && std::uint32_t(cr[7u * jn]) == k) {
+                if (jn >= 0 && std::uint32_t(cr[7u * std::size_t(jn) * S]) == k) {
                     has = true;
                     const R mid = R(1.5707963267948966192313216916397514);
-                    dth = yk.th - ykm.th;
+                    const R dth = yk.th - ykm.th;
                     const R f = (mid - ykm.th) / dth;
-                    const R dr = yk.r - ykm.r;
-                    const R gb = rc_bar[jn];
-                    lam.r = lam.r + gb * f;
-                    lam.th = lam.th + gb * dr * (-f / dth);
-                    lam.ph = lam.ph + ph_bar[jn];
-                    seed_prev_r = gb * (R(1) - f);
-                    seed_prev_th = gb * dr * (f - R(1)) / dth;
+                    const R rb = rc_bar[jn], pb = ph_bar[jn];
+                    const R dx = rb * (yk.r - ykm.r) + pb * (yk.ph - ykm.ph);  // x_bar . (x_k - x_{k-1})
+                    lam.r = lam.r + rb * f;
+                    lam.ph = lam.ph + pb * f;
+                    lam.th = lam.th + dx * (-f / dth);
+                    seed_prev_r = rb * (R(1) - f);
+                    seed_prev_ph = pb * (R(1) - f);
+                    seed_prev_th = dx * (f - R(1)) / dth;
                     --jn;
                 }
                 lam = kerr_step_vjp(ykm, c, sc.h, lam, gc);
                 if (has) {
                     lam.r = lam.r + seed_prev_r;
                     lam.th = lam.th + seed_prev_th;
+                    lam.ph = lam.ph + seed_prev_ph;
                 }
             }
         }
@@ -388,6 +402,7 @@ void SingContext::free_diff() {
     (void)hipFree(diff.cross);
     (void)hipFree(diff.meta);
     (void)hipFree(diff.sky);
+    (void)hipFree(diff.EL);
     (void)hipFree(diff.gI);
     (void)hipFree(diff.hdr);
     (void)hipFree(diff.loss);
@@ -418,6 +433,7 @@ int SingContext::ensure_diff(std::size_t P, std::size_t ncp, std::size_t C) {
     A((void**)&diff.cross, P * sing::kMaxCross * 7 * sizeof(double));
     A((void**)&diff.meta, P * 4 * sizeof(std::uint32_t));
     A((void**)&diff.sky, P * 3 * sizeof(double));
+    A((void**)&diff.EL, P * 2 * sizeof(double));
     A((void**)&diff.gI, P * 3 * sizeof(double));
     A((void**)&diff.hdr, P * 3 * sizeof(float));
     A((void**)&diff.loss, P * sizeof(double));
@@ -455,12 +471,13 @@ int diff_run(SingContext* c, const SingUniforms* base, const SingDiffParams* p, 
     SingDiffOpts o{};
     if (opts_in) o = *opts_in;
     if (o.precision > SING_PREC_MIXED) return c->fail(SING_ERR_ARG, "sing_diff: unknown precision %u", o.precision);
-    const double w = o.soft_edge > 0.0 ? o.soft_edge : 0.5;
+    const double w = o.soft_edge > 0.0 ? o.soft_edge : 1.0;
     const std::size_t C = o.checkpoint_every ? o.checkpoint_every : 64u;
     if (C > 4096) return c->fail(SING_ERR_ARG, "sing_diff: checkpoint_every too large");
     if (!(p->spin >= 0.0 && p->spin < 1.0)) return c->fail(SING_ERR_ARG, "sing_diff: spin must be in [0, 1)");
-    if (!(p->r_out > p->r_in) || !(p->t_scale > 0.0) || !(cam->distance > 0.0))
-        return c->fail(SING_ERR_ARG, "sing_diff: need r_out > r_in, t_scale > 0, distance > 0");
+    if (!(p->t_scale > 0.0) || !(cam->distance > 0.0))
+        return c->fail(SING_ERR_ARG, "sing_diff: need t_scale > 0, distance > 0");
+    if (o.r_in_isco > 1u) return c->fail(SING_ERR_ARG, "sing_diff: r_in_isco must be 0 or 1");
     if (backward && !target) return c->fail(SING_ERR_ARG, "sing_diff: null target");
     const std::size_t P = std::size_t(c->width) * c->height;
     const std::size_t ncp = base->max_steps / C + 1;
@@ -470,14 +487,23 @@ int diff_run(SingContext* c, const SingUniforms* base, const SingDiffParams* p, 
     if (rc != SING_OK) return rc;
     if ((rc = c->ensure_events(4)) != SING_OK) return rc;
 
+    // r_in: given, or the prograde ISCO of the spin with its derivative
+    double r_in = p->r_in, drin_da = 0.0;
+    if (o.r_in_isco) {
+        const sing::Dual<1, double> ri = sing::isco_prograde(sing::Dual<1, double>::var(p->spin, 0));
+        r_in = ri.v;
+        drin_da = ri.d[0];
+    }
+    if (!(p->r_out > r_in)) return c->fail(SING_ERR_ARG, "sing_diff: need r_out > r_in (r_in = %g)", r_in);
+
     // scene in all precisions (host, double duals for the tangents)
     sing::DiffSet ds{};
     using D2 = sing::Dual<2, double>;
     const sing::DiffScene<D2> td = sing::make_diff_scene<D2>(*base, D2::var(p->spin, 0), D2::var(p->incl, 1),
-                                                             D2(p->r_in), D2(p->r_out), D2(p->t_scale),
+                                                             D2(r_in), D2(p->r_out), D2(p->t_scale),
                                                              cam->distance, cam->azimuth, cam->fov_y_deg, w);
     ds.td = td;
-    ds.sd = sing::make_diff_scene<double>(*base, p->spin, p->incl, p->r_in, p->r_out, p->t_scale, cam->distance,
+    ds.sd = sing::make_diff_scene<double>(*base, p->spin, p->incl, r_in, p->r_out, p->t_scale, cam->distance,
                                           cam->azimuth, cam->fov_y_deg, w);
     // float copies are the double values rounded (same discrete model)
     auto to_f = [](const sing::DiffScene<double>& d) {
@@ -554,8 +580,10 @@ int diff_run(SingContext* c, const SingUniforms* base, const SingDiffParams* p, 
     }
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return c->hip_fail(e, "hipStreamSynchronize");
     if (loss) *loss = out6[0];
-    if (grad)
+    if (grad) {
         for (int k = 0; k < 5; ++k) grad[k] = out6[1 + k];
+        grad[0] += grad[2] * drin_da;
+    }
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         float a = 0.0f, bms = 0.0f;

@@ -22,15 +22,16 @@ struct DiffSet {
 
 // Per-pixel device buffers of one differentiable frame.
 struct DiffBuffers {
-    double* cp;             // [P][ncp][5] states every C steps (double storage)
-    double* cross;          // [P][8][7] step, alpha, emission rgb, r_cross, phi
+    double* cp;             // [ncp][5][P] states every C steps (double storage)
+    double* cross;          // [8][7][P] step, alpha, emission rgb, r_cross, phi
     std::uint32_t* meta;    // [P][4] steps, crossings, termination, precision (1 = FP64)
     double* sky;            // [P][3] escape colour behind the crossings
+    double* EL;             // [2][P] E and L_z of the forward (the backward recomputes with these)
     double* gI;             // [P][3] dLoss/dRadiance
     float* hdr;             // [P][3] radiance
     double* loss;           // [P]
     double* grad;           // [P][5]
-    double* scratch;        // [P][C+1][5] recomputed segment states
+    double* scratch;        // [C+1][5][P] recomputed segment states
     const float* target;    // [P][3]
     const float* weight;    // [P] or null
     std::uint32_t* list;    // pixels of the FP64 pass

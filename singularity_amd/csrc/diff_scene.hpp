@@ -3,9 +3,12 @@
 // Parameters p = (a/M, inclination, r_in/M, r_out/M, peak-temperature scale).
 // The camera is the reference's orbital camera (cuda_cli/main.cpp:68-109) at
 // elevation pi/2 - inclination; the disc has soft annulus edges of width w
-// (opacity ramps linearly over [r_in - w, r_in + w] and [r_out - w, r_out + w]),
-// so the image is differentiable in r_in and r_out; w -> 0 recovers the
-// reference's hard annulus (shared_shader/disc_intersection.h:53-55).
+// (opacity ramps linearly over [r_in, r_in + w] and [r_out - w, r_out]), so the
+// image is differentiable in r_in and r_out; the crossing's azimuth is
+// interpolated like its radius, so the disc pattern does not jump when a
+// crossing moves from one RK4 step to the next.  w -> 0 recovers the
+// reference's hard annulus (shared_shader/disc_intersection.h:53-55), whose
+// Kerr shading takes phi at the step after the crossing (geodesic_kernel.cu:380).
 //
 // Everything here is generic over the scalar: float / double for the march,
 // Dual<2, T> for d/d(a, inclination) of the camera initialisation, Dual<6, T>
@@ -129,10 +132,20 @@ SING_HD bool diff_init(const DiffScene<R>& sc, std::uint32_t gx, std::uint32_t g
     return true;
 }
 
-// Soft annulus opacity.
+// Prograde ISCO radius / M of spin a/M in [0, 1) (Bardeen, Press & Teukolsky 1972).
+template <class R> SING_HD R isco_prograde(R a) {
+    const R third = R(1.0 / 3.0);
+    const R z1 = R(1) + hpow(R(1) - a * a, third) * (hpow(R(1) + a, third) + hpow(R(1) - a, third));
+    const R z2 = hsqrt(R(3) * a * a + z1 * z1);
+    return R(3) + z2 - hsqrt((R(3) - z1) * (R(3) + z1 + R(2) * z2));
+}
+
+// Soft annulus opacity: 1 inside [r_in + w, r_out - w], linear to 0 at r_in and
+// r_out.  It vanishes at r_in, where the emission jumps from 0 to the colour of
+// the blackbody's 1000 K floor, so the composite stays continuous in r_cross.
 template <class R> SING_HD R soft_alpha(R rc, R r_in, R r_out, R w) {
-    const R lo = (rc - (r_in - w)) / (R(2) * w);
-    const R hi = ((r_out + w) - rc) / (R(2) * w);
+    const R lo = (rc - r_in) / w;
+    const R hi = (r_out - rc) / w;
     return hclamp(hmin(lo, hi), R(0), R(1));
 }
 

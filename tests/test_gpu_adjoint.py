@@ -1,0 +1,213 @@
+"""GPU differentiable render: forward and discrete adjoint against the oracle.
+
+Scene: Kerr, camera at 50 M, FOV 50 deg, starfield off, soft annulus ramp
+w = 1 M, one sample per pixel.  Parameters p = (a/M, incl, r_in/M, r_out/M,
+T-scale); the optimisation workload starts at (0.5, 60 deg, ISCO, 15, 0.8)
+and its target is rendered at (0.7, 70 deg, ISCO, 20, 1.0).
+
+Bars (tolerances stated here):
+  FP64 forward vs FP64 oracle          |dI| <= 1e-9 (1 + |I|) on >= 99.9 % of pixel channels
+  FP64 gradient vs oracle dual numbers |dg_i| <= 1e-6 max_j |g_j|  (same discrete model, exact derivative)
+  FP64 gradient vs central FD h=1e-3   |dg_i| <= 1e-3 |g_i| + 1e-6 max_j |g_j|, on pixels whose
+                                       FD at h and h/2 agree (no event crosses the stencil)
+  FP32 / mixed gradient vs FP64        relative L2 error <= 5e-2 / 2e-2
+  checkpoint interval                  gradient bit-identical for C = 1, 16, 64, 4096
+  determinism                          loss and gradient bit-identical on repeat
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import singularity_amd as sa
+
+pytestmark = pytest.mark.gpu
+
+W = H = 64
+CAM = dict(distance=50.0, azimuth=0.0, fov_y_deg=50.0)
+START = np.array([0.5, np.radians(60.0), 6.0, 15.0, 0.8])
+TARGET = np.array([0.7, np.radians(70.0), 6.0, 20.0, 1.0])
+SOFT = 1.0
+
+
+def cam():
+    return sa.DiffCamera(CAM["distance"], CAM["azimuth"], CAM["fov_y_deg"])
+
+
+def opts(prec=sa.PREC_FP64, checkpoint=0, isco=0):
+    o = sa.DiffOpts()
+    o.precision = prec
+    o.checkpoint_every = checkpoint
+    o.soft_edge = SOFT
+    o.r_in_isco = isco
+    return o
+
+
+@pytest.fixture(scope="module")
+def be():
+    b = sa.HipBackend(W, H)
+    yield b
+    b.shutdown()
+
+
+@pytest.fixture(scope="module")
+def base():
+    return sa.diff_base_uniforms(W, H)
+
+
+def oracle_extras(orc):
+    ex = orc.Extras()
+    ex.soft_edge = SOFT
+    ex.peak_T_scale = 1.0
+    return ex
+
+
+def oracle_render(orc, base, x):
+    """Oracle FP64 HDR (H, W, 3) of the soft-edge model at parameters x."""
+    n = W * H
+    hdr = np.zeros(n * 3)
+    x = np.ascontiguousarray(x, np.float64)
+    orc.strict().orc_loss_f64(C.byref(base), C.byref(oracle_extras(orc)), x.ctypes.data, CAM["distance"],
+                              CAM["azimuth"], CAM["fov_y_deg"], None, n, None, 8, hdr.ctypes.data)
+    return hdr.reshape(H, W, 3)
+
+
+def oracle_loss_grad(orc, base, x, target):
+    n = W * H
+    g = np.zeros(5)
+    t = np.ascontiguousarray(target, np.float64).ravel()
+    x = np.ascontiguousarray(x, np.float64)
+    loss = orc.strict().orc_loss_grad_dual(C.byref(base), C.byref(oracle_extras(orc)), x.ctypes.data,
+                                           CAM["distance"], CAM["azimuth"], CAM["fov_y_deg"], None, n,
+                                           t.ctypes.data, 8, g.ctypes.data, None)
+    return loss, g
+
+
+@pytest.fixture(scope="module")
+def target(orc, base):
+    return oracle_render(orc, base, TARGET).astype(np.float32)
+
+
+def gpu_hdr(be, base, x, prec=sa.PREC_FP64):
+    print(f"diff_forward {W}x{H} prec={prec} x={np.round(x, 4)}", flush=True)
+    hdr, _ = sa.diff_forward(be, base, sa.DiffParams.from_array(x), cam(), opts(prec))
+    return hdr.astype(np.float64)
+
+
+def gpu_grad(be, base, x, target, prec=sa.PREC_FP64, weights=None, **kw):
+    print(f"diff_loss_grad {W}x{H} prec={prec} x={np.round(x, 4)} {kw}", flush=True)
+    return sa.diff_loss_grad(be, base, sa.DiffParams.from_array(x), cam(), target, weights, opts(prec, **kw))
+
+
+def test_forward_fp64_matches_oracle(be, base, orc):
+    for x in (START, TARGET):
+        got = gpu_hdr(be, base, x)
+        want = oracle_render(orc, base, x)
+        ok = np.abs(got - want) <= 1e-9 * (1.0 + np.abs(want)) + 4e-8 * np.abs(want)  # HDR output is float32
+        assert ok.mean() >= 0.999, (ok.mean(), np.abs(got - want).max())
+        assert want.max() > 0.1  # the disc is in view
+
+
+def test_forward_fp32_and_mixed_match_oracle(be, base, orc):
+    want = oracle_render(orc, base, START)
+    for prec, frac in ((sa.PREC_FP32, 0.995), (sa.PREC_MIXED, 0.998)):
+        got = gpu_hdr(be, base, START, prec)
+        ok = np.abs(got - want) <= 1e-3 * (1.0 + np.abs(want))
+        assert ok.mean() >= frac, (prec, ok.mean())
+
+
+def test_gradient_fp64_matches_oracle_dual_numbers(be, base, orc, target):
+    for x in (START, 0.5 * (START + TARGET)):
+        loss, g, _, st = gpu_grad(be, base, x, target)
+        lo, go = oracle_loss_grad(orc, base, x, target)
+        print("gpu", loss, g, "\noracle", lo, go, flush=True)
+        assert abs(loss - lo) <= 1e-6 * lo
+        assert np.all(np.abs(g - go) <= 1e-6 * np.abs(go).max()), (g, go)
+        assert st.rays == W * H and st.steps > 0
+
+
+def smooth_pixel_mask(be, base, x, h=1e-3):
+    """Pixels whose radiance is smooth in every parameter over [x - h, x + h]:
+    the central differences at h and h/2 agree (no event inside the stencil)."""
+    mask = np.ones((H, W), bool)
+    for i in range(5):
+        e = np.zeros(5)
+        e[i] = h
+        d1 = (gpu_hdr(be, base, x + e) - gpu_hdr(be, base, x - e)) / (2 * h)
+        d2 = (gpu_hdr(be, base, x + e / 2) - gpu_hdr(be, base, x - e / 2)) / h
+        bad = np.abs(d1 - d2) > 1e-2 * (np.abs(d1) + 1e-3)
+        mask &= ~bad.any(-1)
+    return mask
+
+
+def test_gradient_fp64_matches_central_differences(be, base, target):
+    x = START.copy()
+    h = 1e-3
+    mask = smooth_pixel_mask(be, base, x, h)
+    assert mask.mean() > 0.9, mask.mean()
+    wts = mask.astype(np.float32)
+    _, g, _, _ = gpu_grad(be, base, x, target, weights=wts)
+    t = target.astype(np.float64)
+    fd = np.zeros(5)
+    for i in range(5):
+        e = np.zeros(5)
+        e[i] = h
+        lp = (wts[..., None] * (gpu_hdr(be, base, x + e) - t) ** 2).sum()
+        lm = (wts[..., None] * (gpu_hdr(be, base, x - e) - t) ** 2).sum()
+        fd[i] = (lp - lm) / (2 * h)
+    print("adjoint", g, "\nFD     ", fd, flush=True)
+    assert np.all(np.abs(g - fd) <= 1e-3 * np.abs(fd) + 1e-6 * np.abs(fd).max()), (g, fd)
+
+
+def test_gradient_fp32_and_mixed_close_to_fp64(be, base, target):
+    x = 0.5 * (START + TARGET)
+    _, g64, _, _ = gpu_grad(be, base, x, target)
+    for prec, tol in ((sa.PREC_FP32, 5e-2), (sa.PREC_MIXED, 2e-2)):
+        _, g, _, st = gpu_grad(be, base, x, target, prec)
+        err = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+        print(prec, g, "rel", err, "fp64 rays", st.fp64_rays, flush=True)
+        assert err <= tol, (prec, err, g, g64)
+        if prec == sa.PREC_MIXED:
+            assert 0 < st.fp64_rays < W * H
+
+
+def test_checkpoint_interval_does_not_change_gradient(be, base, target):
+    ref = gpu_grad(be, base, START, target, checkpoint=64)
+    for cp in (1, 16, 4096):
+        got = gpu_grad(be, base, START, target, checkpoint=cp)
+        assert got[0] == ref[0] and np.array_equal(got[1], ref[1]), (cp, got[1], ref[1])
+
+
+def test_deterministic(be, base, target):
+    for prec in (sa.PREC_FP32, sa.PREC_MIXED):
+        a = gpu_grad(be, base, START, target, prec)
+        b = gpu_grad(be, base, START, target, prec)
+        assert a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+
+
+def test_r_in_tied_to_isco(be, base, target):
+    x = START.copy()
+    rin = sa.kerr_radius(2, 1.0, x[0])
+    x_expl = x.copy()
+    x_expl[2] = rin
+    l1, g1, _, _ = gpu_grad(be, base, x_expl, target)
+    l2, g2, _, _ = gpu_grad(be, base, x, target, isco=1)
+    e = 1e-6
+    drin = (sa.kerr_radius(2, 1.0, x[0] + e) - sa.kerr_radius(2, 1.0, x[0] - e)) / (2 * e)
+    assert abs(l1 - l2) <= 1e-9 * l1
+    np.testing.assert_allclose(g2[1:], g1[1:], rtol=1e-7, atol=1e-9 * np.abs(g1).max())
+    np.testing.assert_allclose(g2[0], g1[0] + g1[2] * drin, rtol=1e-6)
+
+
+def test_argument_errors(be, base, target):
+    bad = sa.DiffParams.from_array([1.2, 1.0, 6.0, 20.0, 1.0])
+    with pytest.raises(sa.SingError):
+        sa.diff_forward(be, base, bad, cam(), opts())
+    small = sa.diff_base_uniforms(W // 2, H)
+    with pytest.raises(sa.SingError):
+        sa.diff_forward(be, small, sa.DiffParams.from_array(START), cam(), opts())
+    with pytest.raises(sa.SingError):
+        sa.diff_forward(be, base, sa.DiffParams.from_array([0.5, 1.0, 20.0, 10.0, 1.0]), cam(), opts())

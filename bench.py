@@ -89,6 +89,43 @@ def cpu_baseline(u: sa.Uniforms, budget_s: float, seed: int = 1234) -> dict:
     }
 
 
+def adjoint_leg(evals: int = 50) -> dict:
+    """grads/s of the 512x512 inverse render: one evaluation = forward with
+    checkpoints + discrete adjoint + reduction of dL/d(a/M, incl, r_in, r_out, Ts),
+    at the optimisation's start point against a target rendered at the goal."""
+    W = 512
+    be = sa.HipBackend(W, W)
+    base = sa.diff_base_uniforms(W, W)
+    cam = sa.DiffCamera(50.0, 0.0, 50.0)
+
+    def opts(prec):
+        o = sa.DiffOpts()
+        o.precision, o.soft_edge, o.r_in_isco = prec, 1.0, 1
+        return o
+
+    goal = sa.DiffParams.from_array([0.7, np.radians(70.0), 0.0, 20.0, 1.0])
+    start = sa.DiffParams.from_array([0.5, np.radians(60.0), 0.0, 15.0, 0.8])
+    tgt, _ = sa.diff_forward(be, base, goal, cam, opts(sa.PREC_FP64))
+    res = {"workload": "Kerr 512x512 1 spp, L2 over HDR RGB, start (0.5, 60 deg, 15 M, 0.8) vs target "
+                       "(0.7, 70 deg, 20 M, 1.0), r_in = ISCO(a), soft annulus 1 M, checkpoint every 64 steps"}
+    for name, prec in (("mixed", sa.PREC_MIXED), ("fp32", sa.PREC_FP32), ("fp64", sa.PREC_FP64)):
+        for _ in range(3):
+            sa.diff_loss_grad(be, base, start, cam, tgt, None, opts(prec))
+        fwd, bwd = [], []
+        t0 = time.perf_counter()
+        for _ in range(evals):
+            _, g, _, st = sa.diff_loss_grad(be, base, start, cam, tgt, None, opts(prec))
+            fwd.append(st.forward_ms)
+            bwd.append(st.backward_ms)
+        wall = time.perf_counter() - t0
+        res[name] = {"grads_per_s": round(evals / wall, 1), "forward_ms": round(float(np.mean(fwd)), 3),
+                     "adjoint_ms": round(float(np.mean(bwd)), 3), "fp64_rays": int(st.fp64_rays),
+                     "forward_steps": int(st.steps)}
+        log(f"adjoint {name}: {res[name]}")
+    be.shutdown()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +136,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-compare", action="store_true", help="skip the plain-FP32 and FP64 comparison frames")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the grads/s leg (512x512 inverse render)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; the renderer has no multi-GPU path "
@@ -144,7 +182,7 @@ def main() -> int:
             "frac": round(achieved / FL.PEAK_FP32_TFLOPS, 4),
             "traffic": round(FL.TRACE_HBM_BYTES_PER_RAY * rays_per_frame / max(1, int(stats[-1].chunks))),
             "kernel": "trace (FP32 pass + FP64 band pass)",
-            "flops_per_step": FL.KERR_STEP_FLOPS, "steps_per_launch": int(steps_total),
+            "flops_per_step": FL.KERR_STEP_FLOPS, "steps_per_frame": int(steps_total),
             "frac_of_nonpacked_fp32": round(achieved / FL.PEAK_FP32_NONPACKED_TFLOPS, 4),
         },
         "kernel_ms_per_frame": round(kern_ms, 2), "trace_ms_per_frame": round(trace_ms, 2),
@@ -168,6 +206,9 @@ def main() -> int:
         out["mixed_over_plain_time"] = round(kern_ms / st32.kernel_ms, 3)
         out["fp64_over_mixed_time"] = round(st64.kernel_ms / kern_ms, 3)
     be.shutdown()
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg()
+        out["grads_per_s"] = out["adjoint"]["mixed"]["grads_per_s"]
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(u, args.cpu_seconds)

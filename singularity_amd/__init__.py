@@ -409,7 +409,7 @@ def diff_loss_grad(be: "HipBackend", base: Uniforms, params: DiffParams, cam: Di
 
 def adam_fit(be: "HipBackend", base: Uniforms, start, target: np.ndarray, cam: DiffCamera, steps: int = 200,
              lr=None, free=None, opts: DiffOpts | None = None, beta1: float = 0.9, beta2: float = 0.999,
-             clip: float = 3.0, lr_floor: float = 0.05, callback=None):
+             clip: float = 2.0, lr_floor: float = 0.01, callback=None):
     """Adam over the 5 scene parameters (entries with free == False stay fixed).
 
     The learning rate follows a cosine decay from lr to lr_floor * lr.  After 5
@@ -419,7 +419,7 @@ def adam_fit(be: "HipBackend", base: Uniforms, start, target: np.ndarray, cam: D
     steps.  Returns (params, history of (loss, params, grad)).
     """
     x = np.array(start, np.float64)
-    lr = np.array([0.02, 0.01, 0.05, 0.2, 0.01]) if lr is None else np.asarray(lr, np.float64)
+    lr = np.array([0.03, 0.015, 0.075, 0.3, 0.015]) if lr is None else np.asarray(lr, np.float64)
     tied = opts is not None and opts.r_in_isco
     if free is None:
         free = np.array([True, True, not tied, True, True])
@@ -429,6 +429,8 @@ def adam_fit(be: "HipBackend", base: Uniforms, start, target: np.ndarray, cam: D
     history = []
     for k in range(1, steps + 1):
         loss, g, _, _ = diff_loss_grad(be, base, DiffParams.from_array(x), cam, target, None, opts)
+        if not (np.isfinite(loss) and np.all(np.isfinite(g))):
+            raise FloatingPointError(f"non-finite loss/gradient at step {k}: {loss} {g} (params {x})")
         g = np.where(free, g, 0.0)
         history.append((loss, x.copy(), g.copy()))
         if callback is not None:

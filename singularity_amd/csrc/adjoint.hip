@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void diff_bwd_kernel(const DiffSet* __restrict
     KY<Dual<2, R>> y0;
     Dual<2, R> Ed, Ld;
     if (!diff_init(tangent_scene_of<R>(ds), gx, gy, y0, Ed, Ld)) return;
-    const std::uint32_t n = meta[0], J = meta[1], term = meta[2];
+    const std::uint32_t J = meta[1], term = meta[2];
     if (J == 0u) return;  // no crossing: the radiance has no parameter dependence
     const std::size_t S = b.P;
     // the forward's own constants, so the recomputed segments repeat it exactly
@@ -287,6 +287,10 @@ __global__ __launch_bounds__(256) void diff_bwd_kernel(const DiffSet* __restrict
     const double* cp = b.cp + pix;
     double* scr = b.scratch + pix;
     int jn = int(J) - 1;  // next crossing (from the end) whose seeds are pending
+    // The state adjoint is zero after the last crossing, so the sweep starts
+    // there.  (Sweeping the remaining steps would only add 0 * J^T terms, and
+    // the final step into the horizon can have non-finite Jacobian entries.)
+    const std::uint32_t n = std::uint32_t(cr[7u * std::size_t(jn) * S]);
     if (n > 0) {
         const std::uint32_t nseg = (n + b.C - 1u) / b.C;
         for (int seg = int(nseg) - 1; seg >= 0; --seg) {

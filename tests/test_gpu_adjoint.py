@@ -6,10 +6,11 @@ T-scale); the optimisation workload starts at (0.5, 60 deg, ISCO, 15, 0.8)
 and its target is rendered at (0.7, 70 deg, ISCO, 20, 1.0).
 
 Bars (tolerances stated here):
-  FP64 forward vs FP64 oracle          |dI| <= 1e-9 (1 + |I|) on >= 99.9 % of pixel channels
+  FP64 forward vs FP64 oracle          |dI| <= 1e-9 + 6e-8 |I| (float32 output rounding) on >= 99.9 %
   FP64 gradient vs oracle dual numbers |dg_i| <= 1e-6 max_j |g_j|  (same discrete model, exact derivative)
   FP64 gradient vs central FD h=1e-3   |dg_i| <= 1e-3 |g_i| + 1e-6 max_j |g_j|, on pixels whose
-                                       FD at h and h/2 agree (no event crosses the stencil)
+                                       FD at h and h/2 agree to 3e-4 (no event inside the stencil and
+                                       the O(h^2) truncation below 4e-4)
   FP32 / mixed gradient vs FP64        relative L2 error <= 5e-2 / 2e-2
   checkpoint interval                  gradient bit-identical for C = 1, 16, 64, 4096
   determinism                          loss and gradient bit-identical on repeat
@@ -106,7 +107,7 @@ def test_forward_fp64_matches_oracle(be, base, orc):
     for x in (START, TARGET):
         got = gpu_hdr(be, base, x)
         want = oracle_render(orc, base, x)
-        ok = np.abs(got - want) <= 1e-9 * (1.0 + np.abs(want)) + 4e-8 * np.abs(want)  # HDR output is float32
+        ok = np.abs(got - want) <= 1e-9 + 6e-8 * np.abs(want)  # the HDR output is float32: half an ulp
         assert ok.mean() >= 0.999, (ok.mean(), np.abs(got - want).max())
         assert want.max() > 0.1  # the disc is in view
 
@@ -138,7 +139,7 @@ def smooth_pixel_mask(be, base, x, h=1e-3):
         e[i] = h
         d1 = (gpu_hdr(be, base, x + e) - gpu_hdr(be, base, x - e)) / (2 * h)
         d2 = (gpu_hdr(be, base, x + e / 2) - gpu_hdr(be, base, x - e / 2)) / h
-        bad = np.abs(d1 - d2) > 1e-2 * (np.abs(d1) + 1e-3)
+        bad = np.abs(d1 - d2) > 3e-4 * (np.abs(d1) + 1e-3)
         mask &= ~bad.any(-1)
     return mask
 
@@ -147,7 +148,7 @@ def test_gradient_fp64_matches_central_differences(be, base, target):
     x = START.copy()
     h = 1e-3
     mask = smooth_pixel_mask(be, base, x, h)
-    assert mask.mean() > 0.9, mask.mean()
+    assert mask.mean() > 0.8, mask.mean()
     wts = mask.astype(np.float32)
     _, g, _, _ = gpu_grad(be, base, x, target, weights=wts)
     t = target.astype(np.float64)
@@ -211,3 +212,53 @@ def test_argument_errors(be, base, target):
         sa.diff_forward(be, small, sa.DiffParams.from_array(START), cam(), opts())
     with pytest.raises(sa.SingError):
         sa.diff_forward(be, base, sa.DiffParams.from_array([0.5, 1.0, 20.0, 10.0, 1.0]), cam(), opts())
+
+
+# ---- the 512 x 512 inverse-render workload ---------------------------------
+WL = 512
+
+
+@pytest.fixture(scope="module")
+def be512():
+    b = sa.HipBackend(WL, WL)
+    yield b
+    b.shutdown()
+
+
+@pytest.fixture(scope="module")
+def workload(be512):
+    base = sa.diff_base_uniforms(WL, WL)
+    tgt, _ = sa.diff_forward(be512, base, sa.DiffParams.from_array(TARGET), cam(), opts(sa.PREC_FP64, isco=1))
+    return base, tgt
+
+
+def test_gradients_finite_and_consistent_at_512(be512, workload):
+    # regression: the sweep used to run through the final step into the horizon,
+    # whose Jacobian can overflow in FP32 (0 * inf = NaN)
+    base, tgt = workload
+    out = {}
+    for prec in (sa.PREC_FP32, sa.PREC_MIXED, sa.PREC_FP64):
+        print(f"diff_loss_grad {WL}x{WL} prec={prec}", flush=True)
+        loss, g, _, st = sa.diff_loss_grad(be512, base, sa.DiffParams.from_array(START), cam(), tgt, None,
+                                           opts(prec, isco=1))
+        assert np.isfinite(loss) and np.all(np.isfinite(g)), (prec, loss, g)
+        assert st.rays == WL * WL
+        out[prec] = g
+    g64 = out[sa.PREC_FP64]
+    assert np.linalg.norm(out[sa.PREC_MIXED] - g64) <= 2e-2 * np.linalg.norm(g64)
+    assert np.linalg.norm(out[sa.PREC_FP32] - g64) <= 5e-2 * np.linalg.norm(g64)
+
+
+def test_adam_recovers_parameters(be512, workload):
+    """200 Adam steps from (0.5, 60 deg, 15, 0.8) to the target (0.7, 70 deg, 20, 1.0),
+    r_in = ISCO(a), mixed precision.  Bars: |da| <= 0.01, |di| <= 0.5 deg,
+    |dr_out| <= 0.25 M, |dTs| <= 0.01, final loss below 0.1 % of the start."""
+    base, tgt = workload
+    print(f"adam {WL}x{WL} mixed, 200 steps", flush=True)
+    x, hist = sa.adam_fit(be512, base, START, tgt, cam(), steps=200, opts=opts(sa.PREC_MIXED, isco=1))
+    print("final", x, "loss", hist[0][0], "->", hist[-1][0], flush=True)
+    assert abs(x[0] - 0.7) <= 0.01
+    assert abs(np.degrees(x[1]) - 70.0) <= 0.5
+    assert abs(x[3] - 20.0) <= 0.25
+    assert abs(x[4] - 1.0) <= 0.01
+    assert hist[-1][0] <= 1e-3 * hist[0][0]

@@ -129,6 +129,8 @@ typedef struct SingRenderStats {
     uint64_t steps;       /* RK4 steps executed over all rays */
     uint64_t fp64_rays;   /* rays traced in FP64 */
     uint64_t chunks;      /* sample chunks the frame was split into */
+    uint64_t rejected;    /* error-controlled stepping: rejected step attempts (each is 3 RK4 steps of work;
+                             an accepted step is also 3) */
 } SingRenderStats;
 
 typedef struct SingContext SingContext;

@@ -114,6 +114,9 @@ def _type_oracle(L: C.CDLL) -> None:
     L.orc_loss_grad_dual.restype = C.c_double
     L.orc_loss_grad_dual.argtypes = [P, P, P, C.c_double, C.c_double, C.c_double, P, C.c_int64, P, C.c_int, P, P]
     L.orc_loss_f64.restype = C.c_double
+    L.orc_kerr_adaptive_step.restype = C.c_double
+    L.orc_kerr_adaptive_step.argtypes = [P, P, C.c_double, C.c_double, P, P]
+    L.orc_kerr_rk4_substeps.argtypes = [P, P, C.c_double, C.c_int, P]
     L.orc_loss_f64.argtypes = [P, P, P, C.c_double, C.c_double, C.c_double, P, C.c_int64, P, C.c_int, P]
 
 
@@ -156,16 +159,45 @@ def render(u, precision: int = 32, pixels: np.ndarray | None = None, threads: in
     return out
 
 
-def boundary_band(u, pixels: np.ndarray | None = None, eps: float = 1e-4, threads: int = 0) -> np.ndarray:
+def boundary_band(u, pixels: np.ndarray | None = None, eps: float = 1e-4, threads: int = 0,
+                  extras: Extras | None = None) -> np.ndarray:
     """Pixels whose FP64 termination class changes when the sample moves by
     +-eps pixel in x or y: the chaotic boundary band (photon ring, disc edges,
     shadow edge) where FP32 and FP64 may legitimately disagree."""
-    base = render(u, 64, pixels, threads)["cls"]
+    base = render(u, 64, pixels, threads, extras=extras)["cls"]
     band = np.zeros(base.shape, bool)
     for jx, jy in ((eps, 0.0), (-eps, 0.0), (0.0, eps), (0.0, -eps)):
-        ex = Extras(jitter_x=jx, jitter_y=jy)
+        ex = Extras()
+        if extras is not None:
+            C.memmove(C.addressof(ex), C.addressof(extras), C.sizeof(Extras))
+        ex.jitter_x, ex.jitter_y = jx, jy
         band |= render(u, 64, pixels, threads, extras=ex)["cls"] != base
     return band
+
+
+def splitmix64(seed: int, n: int) -> list[int]:
+    """First n outputs of SplitMix64 with state = seed (Steele, Lea & Flood 2014)."""
+    m = (1 << 64) - 1
+    z, out = seed & m, []
+    for _ in range(n):
+        z = (z + 0x9E3779B97F4A7C15) & m
+        x = z
+        x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & m
+        x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & m
+        out.append(x ^ (x >> 31))
+    return out
+
+
+def turbulence_extras(seed: int, adapt_tol: float = 0.0) -> Extras:
+    """Extras for a seeded disc-band pattern: phases = top 53 bits of the first two
+    SplitMix64 outputs, scaled to [0, 2 pi)."""
+    ex = Extras()
+    if seed:
+        a, b = splitmix64(seed, 2)
+        ex.band_phase1 = (a >> 11) * (1.0 / 9007199254740992.0) * 6.283185307179586476925286766559
+        ex.band_phase2 = (b >> 11) * (1.0 / 9007199254740992.0) * 6.283185307179586476925286766559
+    ex.adapt_tol = adapt_tol
+    return ex
 
 
 def trace_ray(u, gx: int, gy: int, ox: float = 0.5, oy: float = 0.5, precision: int = 32,

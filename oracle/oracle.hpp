@@ -731,7 +731,7 @@ inline bool disc_hit(const Uni<T>& u, T r_cross, V3<T>& acc, T& trans, RayOut<T>
 // doubling).  Returns the step actually taken in h_used; h_next is the
 // controller's proposal.  With adapt_tol == 0 this is the reference's fixed step.
 template <class T>
-inline KState<T> kerr_step(const KState<T>& y, const KConst<T>& c, const Uni<T>& u, T& h_cur) {
+inline KState<T> kerr_step(const KState<T>& y, const KConst<T>& c, const Uni<T>& u, T& h_cur, T* h_used = nullptr) {
     if (!(val(u.ext.adapt_tol) > 0.0)) return kerr_rk4(y, u.h_step, c);
     // Step doubling on the damped step: one h step vs two h/2 steps.
     const T tol = u.ext.adapt_tol;
@@ -751,9 +751,43 @@ inline KState<T> kerr_step(const KState<T>& y, const KConst<T>& c, const Uni<T>&
         T grow = (err > T(0.0)) ? T(0.9) * xpow(tol / err, T(0.2)) : T(4.0);
         grow = vclamp(grow, T(0.5), T(4.0));
         h_cur = xfmin(h_cur * grow, u.h_step * T(16.0));
+        if (h_used) *h_used = h;
         return half;
     }
     return kerr_rk4(y, h_cur, c);
+}
+
+// Equator crossing of an error-controlled step (no counterpart in the
+// reference, whose fixed steps interpolate linearly): cubic Hermite dense
+// output on [y0, y1] from the states and their RHS derivatives (step h), three
+// Newton iterations on theta(s) = pi/2 from the linear estimate, then r and phi
+// at that s.  Accurate to O(h^4), where the linear form is O(h^2): error-
+// controlled steps are long (up to 16 h_step) far from the hole.
+template <class T> struct DenseCross {
+    T r, phi;
+};
+template <class T> inline T hermite(T s, T x0, T d0, T x1, T d1) {
+    const T s2 = s * s, s3 = s2 * s;
+    return (T(2.0) * s3 - T(3.0) * s2 + T(1.0)) * x0 + (s3 - T(2.0) * s2 + s) * d0 + (T(3.0) * s2 - T(2.0) * s3) * x1 +
+           (s3 - s2) * d1;
+}
+template <class T> inline T hermite_ds(T s, T x0, T d0, T x1, T d1) {
+    const T s2 = s * s;
+    return (T(6.0) * s2 - T(6.0) * s) * (x0 - x1) + (T(3.0) * s2 - T(4.0) * s + T(1.0)) * d0 + (T(3.0) * s2 - T(2.0) * s) * d1;
+}
+template <class T>
+inline DenseCross<T> kerr_dense_crossing(const KState<T>& y0, const KState<T>& y1, T h, T frac, const KConst<T>& c) {
+    const KState<T> f0 = kerr_rhs(y0, c), f1 = kerr_rhs(y1, c);
+    const T mid = T(1.5707963267948966192313216916397514);
+    const T dt0 = h * f0.theta, dt1 = h * f1.theta;
+    T s = frac;
+    for (int it = 0; it < 3; ++it) {
+        const T g = hermite(s, y0.theta, dt0, y1.theta, dt1) - mid;
+        const T dg = hermite_ds(s, y0.theta, dt0, y1.theta, dt1);
+        if (val(dg) != 0.0) s = s - g / dg;
+        s = vclamp(s, T(0.0), T(1.0));
+    }
+    return {hermite(s, y0.r, h * f0.r, y1.r, h * f1.r), hermite(s, y0.phi, h * f0.phi, y1.phi, h * f1.phi)};
 }
 
 // trace_kerr (geodesic_kernel.cu:303-403) with invariants recorded.
@@ -800,6 +834,9 @@ inline RayOut<T> trace_kerr(const Uni<T>& u, const CamFrame<T>& f, const CameraR
     o.inv0[0] = val(kerr_two_H(s.r, s.theta, s.p_r, s.p_theta, c));
     o.inv0[1] = val(kerr_carter_Q(s.theta, s.p_theta, c));
     T prev_theta = s.theta, prev_r = s.r, prev_phi = s.phi;
+    KState<T> prev = s;  // error-controlled stepping: full state before the step, and its size
+    T h_used = u.h_step;
+    const bool adaptive = val(u.ext.adapt_tol) > 0.0;
     const T cut = T(1.02) * (M + xsqrt(xfmax(M * M - a * a, T(0.0))));
     V3<T> acc = v3(T(0.0), T(0.0), T(0.0));
     T trans = T(1.0);
@@ -819,10 +856,16 @@ inline RayOut<T> trace_kerr(const Uni<T>& u, const CamFrame<T>& f, const CameraR
         }
         const Crossing<T> x = detect_crossing(prev_theta, s.theta);
         if ((u.flags & FLAG_DISC) && x.crossed) {
-            const T r_cross = lerp_scalar(prev_r, s.r, x.frac);
+            T r_cross = lerp_scalar(prev_r, s.r, x.frac);
             // the reference shades at phi of the step after the crossing; the soft
-            // (differentiable) model interpolates it like r
-            const T phi_cross = (val(u.ext.soft_edge) > 0.0) ? lerp_scalar(prev_phi, s.phi, x.frac) : s.phi;
+            // (differentiable) model interpolates it like r; error-controlled steps
+            // use the dense output for both
+            T phi_cross = (val(u.ext.soft_edge) > 0.0) ? lerp_scalar(prev_phi, s.phi, x.frac) : s.phi;
+            if (adaptive) {
+                const DenseCross<T> dc = kerr_dense_crossing(prev, s, h_used, x.frac, c);
+                r_cross = dc.r;
+                phi_cross = dc.phi;
+            }
             const bool stop = disc_hit(u, r_cross, acc, trans, o, [&](T Tk) {
                 return kerr_disc_emit(u, r_cross, phi_cross, cam_r, c.L_z, Tk);
             });
@@ -831,7 +874,8 @@ inline RayOut<T> trace_kerr(const Uni<T>& u, const CamFrame<T>& f, const CameraR
         prev_theta = s.theta;
         prev_r = s.r;
         prev_phi = s.phi;
-        s = kerr_step(s, c, u, h_cur);
+        prev = s;
+        s = kerr_step(s, c, u, h_cur, &h_used);
         o.steps = step + 1;
         o.r_min = std::min(o.r_min, val(s.r));
         o.th_min = std::min(o.th_min, std::fabs(val(s.theta) - 1.5707963267948966));

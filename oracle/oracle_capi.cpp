@@ -398,3 +398,40 @@ void orc_kerr_step_jacobian(const double* y, double E, double L, double a, doubl
         for (int j = 0; j < 8; ++j) J[i * 8 + j] = out[i]->d[j];
 }
 }
+
+extern "C" {
+// One error-controlled step (the oracle's kerr_step with adapt_tol = tol) in
+// double.  y[5] = (r, theta, phi, p_r, p_theta); c[4] = (E, L_z, a, M).
+// h_cur is the controller state (in/out).  out[5] = accepted state;
+// returns the step size actually taken (h_cur at acceptance x pole scale).
+double orc_kerr_adaptive_step(const double* y, const double* c, double h_step, double tol, double* h_cur,
+                              double* out) {
+    Uni<double> u{};
+    u.h_step = h_step;
+    u.ext.adapt_tol = tol;
+    const KState<double> s{0.0, y[0], y[1], y[2], y[3], y[4]};
+    const KConst<double> k{c[0], c[1], 0.0, c[2], c[3]};
+    double h = *h_cur;
+    const KState<double> o = kerr_step(s, k, u, h);
+    // the step taken: replay the controller's halvings to find h at acceptance
+    double hc = *h_cur;
+    for (int attempt = 0; attempt < 12; ++attempt) {
+        const double hh = hc * kerr_pole_scale(s.theta);
+        const KState<double> half = kerr_rk4_raw(kerr_rk4_raw(s, 0.5 * hh, k), 0.5 * hh, k);
+        if (half.r == o.r && half.theta == o.theta && half.phi == o.phi && half.p_r == o.p_r &&
+            half.p_theta == o.p_theta)
+            break;
+        hc *= 0.5;
+    }
+    *h_cur = h;
+    out[0] = o.r; out[1] = o.theta; out[2] = o.phi; out[3] = o.p_r; out[4] = o.p_theta;
+    return hc * kerr_pole_scale(s.theta);
+}
+// Reference solution for the same step: n classical RK4 substeps of h/n.
+void orc_kerr_rk4_substeps(const double* y, const double* c, double h, int n, double* out) {
+    KState<double> s{0.0, y[0], y[1], y[2], y[3], y[4]};
+    const KConst<double> k{c[0], c[1], 0.0, c[2], c[3]};
+    for (int i = 0; i < n; ++i) s = kerr_rk4_raw(s, h / n, k);
+    out[0] = s.r; out[1] = s.theta; out[2] = s.phi; out[3] = s.p_r; out[4] = s.p_theta;
+}
+}

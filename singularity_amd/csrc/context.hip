@@ -222,7 +222,10 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
     if (o.post > SING_POST_BLOOM) return c->fail(SING_ERR_ARG, "sing_render: unknown post pass %u", o.post);
     if (u->metric_type > SING_METRIC_KERR)
         return c->fail(SING_ERR_ARG, "sing_render: unknown metric %u", u->metric_type);
-    if (o.adapt_tol > 0.0f) return c->fail(SING_ERR_ARG, "sing_render: adapt_tol needs sing_render_adaptive");
+    if (!(o.adapt_tol >= 0.0f && o.adapt_tol <= 0.1f))
+        return c->fail(SING_ERR_ARG, "sing_render: adapt_tol must be in [0, 0.1]");
+    if (o.adapt_tol > 0.0f && u->metric_type != SING_METRIC_KERR)
+        return c->fail(SING_ERR_ARG, "sing_render: error-controlled stepping (adapt_tol) is Kerr-only");
     if (o.post == SING_POST_BLOOM && (c->width < 2 || c->height < 2))
         return c->fail(SING_ERR_ARG, "sing_render: bloom needs at least 2x2 pixels");
     std::uint32_t outs = o.outputs ? o.outputs : SING_OUT_RGBA8;
@@ -314,6 +317,7 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
         stats->steps = ctl.steps;
         stats->fp64_rays = ctl.rays64;
         stats->chunks = n_chunks;
+        stats->rejected = ctl.rejected;
     }
     const std::uint64_t expect = std::uint64_t(P) * spp;
     if (ctl.rays != expect)

@@ -151,16 +151,17 @@ __device__ __forceinline__ void kerr_rhs_sc(R r, R pr, R pth, R st, R ct, const 
     dpth = R(-0.5) * dHdth;
 }
 
-// Classical RK4 with the reference's pole damping h*clamp(5|sin theta|, 0.05, 1)
-// (kerr_hamilton.h:165-178).  hs is the undamped step.
+// Pole damping factor clamp(5|sin theta|, 0.05, 1) (kerr_hamilton.h:166-169).
+template <class R> __device__ __forceinline__ R kerr_pole_scale(R st) {
+    const R ps = (st < R(0) ? -st : st) * R(5);
+    return (ps > R(1)) ? R(1) : ((ps < R(0.05)) ? R(0.05) : ps);
+}
+
+// Classical RK4 step of size h (kerr_hamilton.h:171-177); (st, ct) = sin/cos of
+// the starting theta, reused by the first stage.
 template <class R>
-__device__ __forceinline__ void kerr_rk4(R& r, R& th, R& ph, R& pr, R& pth, R hs, const SceneConst<R>& sc,
-                                         const KerrK<R>& k) {
-    R st, ct;
-    dsincos(th, &st, &ct);
-    R ps = (st < R(0) ? -st : st) * R(5);
-    ps = (ps > R(1)) ? R(1) : ((ps < R(0.05)) ? R(0.05) : ps);
-    const R h = hs * ps;
+__device__ __forceinline__ void kerr_rk4_raw(R& r, R& th, R& ph, R& pr, R& pth, R h, R st, R ct,
+                                             const SceneConst<R>& sc, const KerrK<R>& k) {
     const R hh = R(0.5) * h;
     R k_r, k_th, k_ph, k_pr, k_pth;
     kerr_rhs_sc(r, pr, pth, st, ct, sc, k, k_r, k_th, k_ph, k_pr, k_pth);
@@ -183,6 +184,97 @@ __device__ __forceinline__ void kerr_rk4(R& r, R& th, R& ph, R& pr, R& pth, R hs
     ph += h6 * a_ph;
     pr += h6 * a_pr;
     pth += h6 * a_pth;
+}
+
+// The reference's damped step (kerr_hamilton.h:165-178): RK4 with h = hs * pole scale.
+template <class R>
+__device__ __forceinline__ void kerr_rk4(R& r, R& th, R& ph, R& pr, R& pth, R hs, const SceneConst<R>& sc,
+                                         const KerrK<R>& k) {
+    R st, ct;
+    dsincos(th, &st, &ct);
+    kerr_rk4_raw(r, th, ph, pr, pth, hs * kerr_pole_scale(st), st, ct, sc, k);
+}
+
+// Error-controlled step by step doubling: one step of h against two of h/2
+// (h = h_cur * pole scale).  The error norm is max(|dr|/(1+|r|), |dtheta|,
+// |dphi|)/15.  Rejected (err > tol, fewer than 11 retries, h_cur above
+// 1e-3 h_step): h_cur halves and the state is unchanged.  Accepted: the state
+// becomes the two-half-step result and h_cur grows by clamp(0.9 (tol/err)^0.2,
+// 0.5, 4), capped at 16 h_step.  Returns true when accepted (h_used = the
+// step taken).
+template <class R> __device__ __forceinline__ R pow_fifth(R x);
+template <> __device__ __forceinline__ float pow_fifth<float>(float x) { return exp2f(0.2f * log2f(x)); }
+template <> __device__ __forceinline__ double pow_fifth<double>(double x) { return pow(x, 0.2); }
+
+template <class R>
+__device__ __forceinline__ bool kerr_adaptive_step(R& r, R& th, R& ph, R& pr, R& pth, R& h_cur, std::uint32_t& tries,
+                                                   R tol, const SceneConst<R>& sc, const KerrK<R>& k, R& h_used) {
+    R st, ct;
+    dsincos(th, &st, &ct);
+    const R h = h_cur * kerr_pole_scale(st);
+    R br = r, bth = th, bph = ph, bpr = pr, bpt = pth;
+    kerr_rk4_raw(br, bth, bph, bpr, bpt, h, st, ct, sc, k);
+    R hr = r, hth = th, hph = ph, hpr = pr, hpt = pth;
+    kerr_rk4_raw(hr, hth, hph, hpr, hpt, R(0.5) * h, st, ct, sc, k);
+    dsincos(hth, &st, &ct);
+    kerr_rk4_raw(hr, hth, hph, hpr, hpt, R(0.5) * h, st, ct, sc, k);
+    const R er = (hr - br) / (R(1) + (hr < R(0) ? -hr : hr));
+    const R et = hth - bth, ep = hph - bph;
+    const R aer = er < R(0) ? -er : er, aet = et < R(0) ? -et : et, aep = ep < R(0) ? -ep : ep;
+    const R err = dmax(dmax(aer, aet), aep) / R(15);
+    if (!(err <= tol) && tries < 11u && h_cur > sc.h * R(1e-3)) {
+        h_cur = h_cur * R(0.5);
+        tries += 1u;
+        return false;
+    }
+    R grow = (err > R(0)) ? R(0.9) * pow_fifth(tol / err) : R(4);
+    grow = (grow > R(4)) ? R(4) : ((grow < R(0.5)) ? R(0.5) : grow);
+    const R hn = h_cur * grow, cap = sc.h * R(16);
+    h_cur = (hn < cap) ? hn : cap;
+    tries = 0u;
+    h_used = h;
+    r = hr;
+    th = hth;
+    ph = hph;
+    pr = hpr;
+    pth = hpt;
+    return true;
+}
+
+// Equator crossing of an error-controlled step: cubic Hermite dense output on
+// the step from the end states and their RHS derivatives (step h); three Newton
+// iterations on theta(s) = pi/2 from the linear estimate `frac`, then r and phi
+// at s.  O(h^4) where the linear interpolation of the fixed-step algorithm is
+// O(h^2); error-controlled steps reach 16 h_step far from the hole.
+template <class R> __device__ __forceinline__ R hermite(R s, R x0, R d0, R x1, R d1) {
+    const R s2 = s * s, s3 = s2 * s;
+    return (R(2) * s3 - R(3) * s2 + R(1)) * x0 + (s3 - R(2) * s2 + s) * d0 + (R(3) * s2 - R(2) * s3) * x1 +
+           (s3 - s2) * d1;
+}
+template <class R> __device__ __forceinline__ R hermite_ds(R s, R x0, R d0, R x1, R d1) {
+    const R s2 = s * s;
+    return (R(6) * s2 - R(6) * s) * (x0 - x1) + (R(3) * s2 - R(4) * s + R(1)) * d0 + (R(3) * s2 - R(2) * s) * d1;
+}
+template <class R>
+__device__ __forceinline__ void kerr_dense_crossing(R r0, R th0, R ph0, R pr0, R pt0, R r1, R th1, R ph1, R pr1,
+                                                    R pt1, R h, R frac, const SceneConst<R>& sc, const KerrK<R>& k,
+                                                    R& rc, R& phc) {
+    R st, ct, fr0, fth0, fph0, fr1, fth1, fph1, dpr, dpt;
+    dsincos(th0, &st, &ct);
+    kerr_rhs_sc(r0, pr0, pt0, st, ct, sc, k, fr0, fth0, fph0, dpr, dpt);
+    dsincos(th1, &st, &ct);
+    kerr_rhs_sc(r1, pr1, pt1, st, ct, sc, k, fr1, fth1, fph1, dpr, dpt);
+    const R mid = R(1.5707963267948966192313216916397514);
+    const R dt0 = h * fth0, dt1 = h * fth1;
+    R s = frac;
+    for (int it = 0; it < 3; ++it) {
+        const R g = hermite(s, th0, dt0, th1, dt1) - mid;
+        const R dg = hermite_ds(s, th0, dt0, th1, dt1);
+        if (dg != R(0)) s = s - g / dg;
+        s = (s < R(0)) ? R(0) : ((s > R(1)) ? R(1) : s);
+    }
+    rc = hermite(s, r0, h * fr0, r1, h * fr1);
+    phc = hermite(s, ph0, h * fph0, ph1, h * fph1);
 }
 
 // ---- Schwarzschild geodesic RHS (geodesic_math.h:101-123) ----------------------

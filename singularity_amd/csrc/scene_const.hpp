@@ -39,6 +39,8 @@ template <class R> struct SceneConst {
     R r_in, r_out, peak_T, turbulence, time_sec, exposure, ph1, ph2;
     // mixed precision: rays whose photon-orbit margin |R(r*)|/r*^4 < band go FP64
     R band;
+    // error-controlled stepping (Kerr): per-step tolerance, 0 = the reference's fixed step
+    R tol;
     std::uint32_t width, height, metric, flags, max_steps, spp;
 };
 
@@ -110,8 +112,8 @@ template <class R> inline SceneConst<R> make_scene_const(const SingUniforms& u, 
     s.ph1 = R(0);
     s.ph2 = R(0);
     if (o.turb_seed != 0) {
-        // SplitMix64 of the seed -> two phases in [0, 2 pi)
-        std::uint64_t z = std::uint64_t(o.turb_seed) + 0x9E3779B97F4A7C15ull;
+        // SplitMix64 (state = seed): the first two outputs' top 53 bits -> two phases in [0, 2 pi)
+        std::uint64_t z = std::uint64_t(o.turb_seed);
         auto next = [&z]() {
             std::uint64_t x = (z += 0x9E3779B97F4A7C15ull);
             x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -123,6 +125,7 @@ template <class R> inline SceneConst<R> make_scene_const(const SingUniforms& u, 
         s.ph2 = R(double(next() >> 11) * (1.0 / 9007199254740992.0) * twopi);
     }
     s.band = o.band_width > 0.0f ? R(o.band_width) : R(0.4);
+    s.tol = o.adapt_tol > 0.0f ? R(o.adapt_tol) : R(0);
     s.width = u.width;
     s.height = u.height;
     s.metric = u.metric_type;

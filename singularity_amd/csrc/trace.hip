@@ -28,10 +28,13 @@ namespace {
 template <class S> struct RayState {
     S r, th, ph;
     S v0, v1, v2, v3;  // Kerr: p_r, p_theta, E, L_z.  Schwarzschild: u^t, u^r, u^theta, u^phi
-    S prev_r, prev_th, prev_uph;
-    S aux0, aux1;      // disc crossing radius, u^phi at the crossing (Schwarzschild)
+    S prev_r, prev_th, prev_uph;  // prev_uph: Schwarzschild u^phi; error-controlled Kerr: phi
+    S prev_v0, prev_v1, hprev;    // error-controlled Kerr: p_r, p_theta before the step, its size
+    S aux0, aux1;      // disc crossing radius; u^phi (Schwarzschild) or phi (error-controlled Kerr) at it
     S near_eq;         // mixed precision: min |theta - pi/2| over the steps (0 = ambiguous crossing)
+    S hcur;            // error-controlled stepping: current step size
     std::uint32_t steps;
+    std::uint32_t tries;  // error-controlled stepping: rejections of the current step
 };
 
 // Mixed-precision deferral thresholds.  Comparing FP32 and FP64 traces of the
@@ -51,9 +54,13 @@ __device__ __forceinline__ std::uint64_t shfl_u64(std::uint64_t v, int src) {
 }
 
 // Termination checks in the reference's order (geodesic_kernel.cu:351-401)
-// followed by one RK4 step.  Arithmetic in R, storage in S.
-template <class R, class S, bool KERR, bool TRACK = false>
-__device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneConst<R>& sc) {
+// followed by one RK4 step.  Arithmetic in R, storage in S.  With ADAPT (Kerr)
+// the step is one attempt of the error-controlled step; a rejected attempt
+// leaves the state unchanged (the checks it repeats are idempotent) and
+// counts in *rejected.
+template <class R, class S, bool KERR, bool TRACK = false, bool ADAPT = false>
+__device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneConst<R>& sc,
+                                                 std::uint32_t* rejected = nullptr) {
     if (y.steps >= sc.max_steps) return kTermBudget;
     R r = R(y.r), th = R(y.th);
     if (!__builtin_isfinite(r) || r < sc.cut) return kTermHorizon;
@@ -65,7 +72,13 @@ __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneCons
         if (dp * dc < R(0)) {
             const R frac = (mid - pth) / (th - pth);
             const R pr0 = R(y.prev_r);
-            const R rc = pr0 + frac * (r - pr0);
+            R rc = pr0 + frac * (r - pr0);
+            R phc = R(0);
+            if (ADAPT) {
+                const KerrK<R> k = kerr_k(sc, R(y.v2), R(y.v3));
+                kerr_dense_crossing(pr0, pth, R(y.prev_uph), R(y.prev_v0), R(y.prev_v1), r, th, R(y.ph), R(y.v0),
+                                    R(y.v1), R(y.hprev), frac, sc, k, rc, phc);
+            }
             if (TRACK) {
                 const R tol = R(kEdgeRel) * rc;
                 const R d_in = rc - sc.r_in, d_out = rc - sc.r_out;
@@ -73,6 +86,7 @@ __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneCons
             }
             if (rc >= sc.r_in && rc <= sc.r_out) {
                 y.aux0 = S(rc);
+                if (ADAPT) y.aux1 = S(phc);
                 if (!KERR) {
                     const R pu = R(y.prev_uph);
                     y.aux1 = S(pu + frac * (R(y.v3) - pu));
@@ -87,7 +101,21 @@ __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneCons
     if (KERR) {
         R pr = R(y.v0), pt = R(y.v1);
         const KerrK<R> k = kerr_k(sc, R(y.v2), R(y.v3));
-        kerr_rk4(r, th, ph, pr, pt, sc.h, sc, k);
+        if (ADAPT) {
+            R hc = R(y.hcur), hused;
+            y.prev_uph = y.ph;
+            y.prev_v0 = y.v0;
+            y.prev_v1 = y.v1;
+            const bool acc = kerr_adaptive_step(r, th, ph, pr, pt, hc, y.tries, sc.tol, sc, k, hused);
+            y.hcur = S(hc);
+            if (!acc) {
+                *rejected += 1u;
+                return kTermLive;
+            }
+            y.hprev = S(hused);
+        } else {
+            kerr_rk4(r, th, ph, pr, pt, sc.h, sc, k);
+        }
         y.v0 = S(pr);
         y.v1 = S(pt);
     } else {
@@ -126,6 +154,8 @@ __device__ __forceinline__ bool init_ray(RayState<S>& y, const SceneConst<R>& sc
     y.prev_th = y.th;
     y.steps = 0;
     y.near_eq = S(1e30);
+    y.hcur = S(sc.h);
+    y.tries = 0u;
     if (KERR) {
         const R Bq = R(2) * sc.g_tp * uph;
         const R Cq = sc.g_rr * ur * ur + sc.g_thth * uth * uth + sc.g_pp * uph * uph;
@@ -165,13 +195,13 @@ __device__ __forceinline__ bool init_ray(RayState<S>& y, const SceneConst<R>& sc
     return true;
 }
 
-template <class S, bool KERR>
+template <class S, bool KERR, bool ADAPT = false>
 __device__ __forceinline__ Rec make_rec(const RayState<S>& y, std::uint32_t term) {
     Rec rc;
     rc.meta = term | (y.steps << 4);
     if (term == kTermDisc) {
         rc.a = float(y.aux0);
-        rc.b = float(y.ph);
+        rc.b = ADAPT ? float(y.aux1) : float(y.ph);
         rc.c = KERR ? float(y.v3) : float(y.aux1);
     } else {
         rc.a = float(y.th);
@@ -197,13 +227,15 @@ __device__ __forceinline__ bool item_to_ray(std::uint64_t item, const Chunk& ch,
     return gx < W && gy < H;
 }
 
+// METRIC 0: Schwarzschild; 1: Kerr; 2: Kerr with error-controlled steps.
 // PASS 0: FP32 trace of every ray; 1: FP32 trace deferring photon-band rays;
 // 2: FP64 trace of every ray; 3: FP64 trace of the deferred list.
 template <int METRIC, int PASS>
 __global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneConst<double> sd,
                                                     const float2* __restrict__ halton, Chunk ch,
                                                     QueueCtl* __restrict__ ctl, std::uint32_t step_batch) {
-    constexpr bool KERR = (METRIC == 1);
+    constexpr bool KERR = (METRIC >= 1);
+    constexpr bool ADAPT = (METRIC == 2);
     constexpr bool F64 = (PASS >= 2);
     using S = typename std::conditional<F64, double, float>::type;
     const int lane = __lane_id();
@@ -215,6 +247,7 @@ __global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneC
     bool retired = false, live = false, need = true;
     std::uint32_t rec_idx = 0;
     std::uint64_t lane_steps = 0, lane_rays = 0;
+    std::uint32_t lane_rej = 0;
     RayState<S> y{};
 
     for (;;) {
@@ -285,8 +318,8 @@ __global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneC
         std::uint32_t term = kTermLive;
         for (std::uint32_t k = 0; k < step_batch; ++k) {
             if (live) {
-                const std::uint32_t t = F64 ? advance<double, S, KERR>(y, sd)
-                                            : advance<float, S, KERR, PASS == 1>(y, sf);
+                const std::uint32_t t = F64 ? advance<double, S, KERR, false, ADAPT>(y, sd, &lane_rej)
+                                            : advance<float, S, KERR, PASS == 1, ADAPT>(y, sf, &lane_rej);
                 if (t != kTermLive) {
                     live = false;
                     term = t;
@@ -302,7 +335,7 @@ __global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneC
                 const std::uint32_t slot = atomicAdd(&ctl->band_count, 1u);
                 ch.band[slot] = rec_idx;
             } else {
-                ch.rec[rec_idx] = make_rec<S, KERR>(y, term);
+                ch.rec[rec_idx] = make_rec<S, KERR, ADAPT>(y, term);
                 lane_steps += y.steps;
                 lane_rays += 1;
             }
@@ -310,10 +343,13 @@ __global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneC
         }
     }
 
+    std::uint64_t rej = lane_rej;
     for (int off = 32; off > 0; off >>= 1) {
         lane_steps += shfl_u64(lane_steps, lane ^ off);
         lane_rays += shfl_u64(lane_rays, lane ^ off);
+        if (ADAPT) rej += shfl_u64(rej, lane ^ off);
     }
+    if (ADAPT && lane == 0 && rej) atomicAdd((unsigned long long*)&ctl->rejected, (unsigned long long)rej);
     if (lane == 0 && lane_rays) {
         atomicAdd((unsigned long long*)&ctl->steps, (unsigned long long)lane_steps);
         atomicAdd((unsigned long long*)&ctl->rays, (unsigned long long)lane_rays);
@@ -419,17 +455,25 @@ hipError_t launch_trace_chunk(const SceneConst<float>& sf, const SceneConst<doub
     const dim3 block(256);
     const std::uint32_t sb = shape.step_batch ? shape.step_batch : 8u;
     const bool kerr = sf.metric == SING_METRIC_KERR;
-#define SING_TRACE(M, P, G) \
-    hipLaunchKernelGGL((trace_kernel<M, P>), dim3(G), block, 0, stream, sf, sd, halton, ch, ctl, sb)
+    const bool adapt = kerr && sf.tol > 0.0f;
+#define SING_TRACE(P, G)                                                                                  \
+    do {                                                                                                  \
+        if (adapt)                                                                                        \
+            hipLaunchKernelGGL((trace_kernel<2, P>), dim3(G), block, 0, stream, sf, sd, halton, ch, ctl, sb); \
+        else if (kerr)                                                                                    \
+            hipLaunchKernelGGL((trace_kernel<1, P>), dim3(G), block, 0, stream, sf, sd, halton, ch, ctl, sb); \
+        else                                                                                              \
+            hipLaunchKernelGGL((trace_kernel<0, P>), dim3(G), block, 0, stream, sf, sd, halton, ch, ctl, sb); \
+    } while (0)
     if (mode == SING_PREC_FP32) {
-        if (kerr) SING_TRACE(1, 0, shape.blocks); else SING_TRACE(0, 0, shape.blocks);
+        SING_TRACE(0, shape.blocks);
     } else if (mode == SING_PREC_FP64) {
-        if (kerr) SING_TRACE(1, 2, shape.blocks64); else SING_TRACE(0, 2, shape.blocks64);
+        SING_TRACE(2, shape.blocks64);
     } else {
-        if (kerr) SING_TRACE(1, 1, shape.blocks); else SING_TRACE(0, 1, shape.blocks);
+        SING_TRACE(1, shape.blocks);
         if (sf.flags & SING_FLAG_STARFIELD_ON)
             hipLaunchKernelGGL(audit_kernel, dim3(shape.blocks), block, 0, stream, sf, ch, ctl);
-        if (kerr) SING_TRACE(1, 3, shape.blocks64); else SING_TRACE(0, 3, shape.blocks64);
+        SING_TRACE(3, shape.blocks64);
     }
 #undef SING_TRACE
     return hipGetLastError();

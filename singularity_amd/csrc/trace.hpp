@@ -53,6 +53,7 @@ struct QueueCtl {
     std::uint64_t steps;
     std::uint64_t rays;
     std::uint64_t rays64;
+    std::uint64_t rejected;    // error-controlled stepping: rejected step attempts
 };
 
 struct Chunk {

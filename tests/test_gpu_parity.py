@@ -194,3 +194,63 @@ def test_full_4k_frame_properties(backends, orc):
     assert (f.cls.ravel()[px] == o["cls"]).mean() >= 0.999
     d = np.abs(f.rgba.reshape(-1, 4)[px].astype(int) - o["rgba"].astype(int)).max(-1)
     assert (d <= 1).mean() >= 0.998
+
+
+# ---- error-controlled stepping and seeded turbulence (the 1080p/16-spp workload) ----
+ADAPT_TOL = 1e-6
+
+
+def test_splitmix64_known_answer(orc):
+    # published first output of SplitMix64 from state 0
+    assert orc.splitmix64(0, 1)[0] == 0xE220A8397B1DCDAF
+
+
+def test_adaptive_fp32_matches_fp32_oracle(backends, orc):
+    u = config2(160, 90, 4)
+    f = render(backends, u, sa.PREC_FP32, adapt_tol=ADAPT_TOL, turb_seed=42)
+    o = orc.render(u, 32, extras=orc.turbulence_extras(42, ADAPT_TOL))
+    assert f.stats.rays == 160 * 90 * 4 and f.stats.rejected > 0
+    assert (f.cls.ravel() == o["cls"]).mean() >= 0.999
+    assert (rel_err(f.hdr, o["hdr"]) <= 1e-3).mean() >= 0.995
+    assert abs(int(f.stats.steps) - int(o["steps"].sum())) <= 1e-2 * o["steps"].sum()
+    fixed = render(backends, u, sa.PREC_FP32, turb_seed=42)
+    print("adaptive steps", f.stats.steps, "rejected", f.stats.rejected, "fixed steps", fixed.stats.steps, flush=True)
+    assert f.stats.steps < fixed.stats.steps / 3
+
+
+def test_adaptive_fp64_matches_fp64_oracle(backends, orc):
+    u = config2(160, 90, 2)
+    f = render(backends, u, sa.PREC_FP64, adapt_tol=ADAPT_TOL, turb_seed=7)
+    o = orc.render(u, 64, extras=orc.turbulence_extras(7, ADAPT_TOL))
+    assert (f.cls.ravel() == o["cls"]).mean() >= 0.9999
+    assert (rel_err(f.hdr, o["hdr"]) <= 1e-5).mean() >= 0.999
+
+
+def test_adaptive_mixed_matches_fp64_oracle_outside_band(backends, orc):
+    u = config2(160, 90, 1)
+    ex = orc.turbulence_extras(42, ADAPT_TOL)
+    f = render(backends, u, sa.PREC_MIXED, adapt_tol=ADAPT_TOL, turb_seed=42)
+    o = orc.render(u, 64, extras=ex)
+    keep = ~orc.boundary_band(u, extras=ex)
+    assert 0 < f.stats.fp64_rays < f.stats.rays
+    assert (f.cls.ravel()[keep] == o["cls"][keep]).mean() >= 0.999
+    assert (rel_err(f.hdr, o["hdr"])[keep] <= 1e-3).mean() >= 0.999
+
+
+def test_turbulence_seed_matches_oracle(backends, orc):
+    u = config2(128, 72, 1)
+    seeded = render(backends, u, sa.PREC_FP32, turb_seed=123456789)
+    plain = render(backends, u, sa.PREC_FP32)
+    o = orc.render(u, 32, extras=orc.turbulence_extras(123456789))
+    assert (rel_err(seeded.hdr, o["hdr"]) <= 1e-3).mean() >= 0.997
+    disc = seeded.cls == sa.CLASS_DISC
+    assert disc.mean() > 0.05
+    assert np.abs(seeded.hdr[disc] - plain.hdr[disc]).max() > 1e-2  # the seed moves the band pattern
+
+
+def test_adaptive_rejected_for_schwarzschild(backends):
+    u = config1(16, 16)
+    with pytest.raises(sa.SingError):
+        render(backends, u, sa.PREC_FP32, adapt_tol=1e-6)
+    with pytest.raises(sa.SingError):
+        render(backends, config2(16, 9, 1), sa.PREC_FP32, adapt_tol=-1.0)

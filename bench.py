@@ -89,6 +89,93 @@ def cpu_baseline(u: sa.Uniforms, budget_s: float, seed: int = 1234) -> dict:
     }
 
 
+def kerr_uniforms(W, H, spp, spin, r_out, dist, incl_deg, fov_deg):
+    """Kerr scene with r_in = prograde ISCO(a), Doppler + redshift, orbital camera."""
+    s = sa.scene_defaults()
+    s.metric = sa.METRIC_KERR
+    s.spin_a_over_M = spin
+    s.disc_inner_M = sa.isco_prograde(spin)
+    s.disc_outer_M = r_out
+    s.supersample = spp
+    cam = sa.orbital_camera(dist, np.radians(90.0 - incl_deg), 0.0, fov_deg)
+    return sa.pack_uniforms(s, cam, W, H, 0.0)
+
+
+def c2_leg(frames: int = 3, seed: int = 20261203) -> dict:
+    """Kerr a=0.94 1920x1080 16 spp, error-controlled RK4 (tol 1e-6), seeded disc
+    turbulence, half-float HDR -> bloom (0.8 / 0.55, 7-tap) -> exposure -> ACES -> sRGB,
+    mixed FP32/FP64; against an all-FP64 frame of the same algorithm."""
+    W, H, spp = 1920, 1080, 16
+    u = kerr_uniforms(W, H, spp, 0.94, 25.0, 40.0, 80.0, 30.0)
+    be = sa.HipBackend(W, H)
+    outs = sa.OUT_RGBA8 | sa.OUT_HDR | sa.OUT_CLASS
+
+    def opts(prec, tol=1e-6):
+        return sa.RenderOpts(precision=prec, outputs=outs, post=sa.POST_BLOOM, adapt_tol=tol, turb_seed=seed)
+
+    be.render(u, opts(sa.PREC_MIXED))
+    st = []
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        st.append(be.render(u, opts(sa.PREC_MIXED)))
+    wall = (time.perf_counter() - t0) / frames
+    fm = be.capture(outs)
+    s32 = be.render(u, opts(sa.PREC_FP32))
+    f32 = be.capture(outs)
+    s64 = be.render(u, opts(sa.PREC_FP64))
+    f64 = be.capture(outs)
+    sfix = be.render(u, opts(sa.PREC_MIXED, 0.0))
+    be.shutdown()
+    rays = W * H * spp
+    post_ms = float(np.mean([s.post_ms for s in st]))
+    kern = float(np.mean([s.kernel_ms for s in st]))
+    res = {
+        "workload": f"Kerr a=0.94 {W}x{H} {spp} spp, camera 40 M incl 80 deg FOV 30, disc [ISCO, 25 M], "
+                    f"adaptive RK4 tol 1e-6, turbulence seed {seed}, bloom post, mixed FP32/FP64",
+        "mrays_s": round(rays / wall / 1e6, 2), "kernel_ms": round(kern, 2),
+        "trace_ms": round(float(np.mean([s.trace_ms for s in st])), 2),
+        "accepted_steps": int(st[-1].steps), "rejected_steps": int(st[-1].rejected),
+        "rk4_evals_per_s": round(3 * (st[-1].steps + st[-1].rejected) / (kern * 1e-3), 1),
+        "fp64_rays": int(st[-1].fp64_rays),
+        "post_ms": round(post_ms, 4),
+        "post_hbm_gbs": round(FL.POST_BYTES_PER_PIXEL * W * H / (post_ms * 1e-3) / 1e9, 1),
+        "post_bytes_per_pixel": FL.POST_BYTES_PER_PIXEL,
+        "plain_fp32": {"kernel_ms": round(s32.kernel_ms, 2), "error_vs_fp64": err_stats(f32.hdr, f64.hdr, f32.cls, f64.cls),
+                       "rgba_max_lsb_vs_fp64": int(np.abs(f32.rgba.astype(int) - f64.rgba.astype(int)).max())},
+        "fp64": {"kernel_ms": round(s64.kernel_ms, 2)},
+        "mixed_error_vs_fp64": err_stats(fm.hdr, f64.hdr, fm.cls, f64.cls),
+        "mixed_rgba_max_lsb_vs_fp64": int(np.abs(fm.rgba.astype(int) - f64.rgba.astype(int)).max()),
+        "fixed_step_mixed": {"kernel_ms": round(sfix.kernel_ms, 2), "steps": int(sfix.steps)},
+    }
+    log(f"c2: {res['mrays_s']} Mrays/s, post {res['post_hbm_gbs']} GB/s")
+    return res
+
+
+def c3_leg() -> dict:
+    """Kerr a=0.998 3840x2160 256 spp, incl 60 deg, disc [ISCO, 30 M], cuda_cli camera
+    defaults (30 M, FOV 60), fixed h = 0.4: plain FP32, mixed and all-FP64 frames."""
+    W, H, spp = 3840, 2160, 256
+    u = kerr_uniforms(W, H, spp, 0.998, 30.0, 30.0, 60.0, 60.0)
+    be = sa.HipBackend(W, H)
+    outs = sa.OUT_HDR | sa.OUT_CLASS | sa.OUT_RGBA8
+    res = {"workload": f"Kerr a=0.998 {W}x{H} {spp} spp, incl 60 deg, disc [ISCO, 30 M], camera 30 M FOV 60, h 0.4"}
+    frames = {}
+    for name, prec in (("mixed", sa.PREC_MIXED), ("fp32", sa.PREC_FP32), ("fp64", sa.PREC_FP64)):
+        st = be.render(u, sa.RenderOpts(precision=prec, outputs=outs))
+        frames[name] = be.capture(outs)
+        res[name] = {"kernel_ms": round(st.kernel_ms, 1), "mrays_s": round(W * H * spp / st.kernel_ms / 1e3, 2),
+                     "steps": int(st.steps), "fp64_rays": int(st.fp64_rays)}
+        log(f"c3 {name}: {st.kernel_ms:.0f} ms")
+    be.shutdown()
+    f64 = frames["fp64"]
+    for name in ("mixed", "fp32"):
+        f = frames[name]
+        res[name]["error_vs_fp64"] = err_stats(f.hdr, f64.hdr, f.cls, f64.cls)
+    res["mixed_over_plain_time"] = round(res["mixed"]["kernel_ms"] / res["fp32"]["kernel_ms"], 3)
+    res["fp64_over_mixed_time"] = round(res["fp64"]["kernel_ms"] / res["mixed"]["kernel_ms"], 3)
+    return res
+
+
 def adjoint_leg(evals: int = 50) -> dict:
     """grads/s of the 512x512 inverse render: one evaluation = forward with
     checkpoints + discrete adjoint + reduction of dL/d(a/M, incl, r_in, r_out, Ts),
@@ -137,6 +224,8 @@ def main() -> int:
     ap.add_argument("--no-compare", action="store_true", help="skip the plain-FP32 and FP64 comparison frames")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the grads/s leg (512x512 inverse render)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the 1080p adaptive+bloom and the a=0.998 4K three-precision legs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; the renderer has no multi-GPU path "
@@ -206,6 +295,9 @@ def main() -> int:
         out["mixed_over_plain_time"] = round(kern_ms / st32.kernel_ms, 3)
         out["fp64_over_mixed_time"] = round(st64.kernel_ms / kern_ms, 3)
     be.shutdown()
+    if not args.no_extra:
+        out["adaptive_bloom_1080p"] = c2_leg()
+        out["kerr_a0998_4k"] = c3_leg()
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg()
         out["grads_per_s"] = out["adjoint"]["mixed"]["grads_per_s"]

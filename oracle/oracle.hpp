@@ -632,6 +632,7 @@ template <class T> struct RayOut {
     double r_min = 1e30;         // smallest radius reached
     double th_min = 1e30;        // smallest |theta - pi/2| over the states after the first
     int n_soft = 0;              // crossings with nonzero opacity (soft-edge model)
+    double esc_theta = 0, esc_phi = 0;  // direction the escape colour is looked up at
 };
 
 template <class T> inline T soft_alpha(T r, T r_in, T r_out, T w) {
@@ -775,19 +776,35 @@ template <class T> inline T hermite_ds(T s, T x0, T d0, T x1, T d1) {
     const T s2 = s * s;
     return (T(6.0) * s2 - T(6.0) * s) * (x0 - x1) + (T(3.0) * s2 - T(4.0) * s + T(1.0)) * d0 + (T(3.0) * s2 - T(2.0) * s) * d1;
 }
+// Three Newton iterations on hermite(s) = target from s, clamped to [0, 1].
+template <class T> inline T dense_root(T x0, T d0, T x1, T d1, T target, T s) {
+    for (int it = 0; it < 3; ++it) {
+        const T g = hermite(s, x0, d0, x1, d1) - target;
+        const T dg = hermite_ds(s, x0, d0, x1, d1);
+        if (val(dg) != 0.0) s = s - g / dg;
+        s = vclamp(s, T(0.0), T(1.0));
+    }
+    return s;
+}
 template <class T>
 inline DenseCross<T> kerr_dense_crossing(const KState<T>& y0, const KState<T>& y1, T h, T frac, const KConst<T>& c) {
     const KState<T> f0 = kerr_rhs(y0, c), f1 = kerr_rhs(y1, c);
     const T mid = T(1.5707963267948966192313216916397514);
-    const T dt0 = h * f0.theta, dt1 = h * f1.theta;
-    T s = frac;
-    for (int it = 0; it < 3; ++it) {
-        const T g = hermite(s, y0.theta, dt0, y1.theta, dt1) - mid;
-        const T dg = hermite_ds(s, y0.theta, dt0, y1.theta, dt1);
-        if (val(dg) != 0.0) s = s - g / dg;
-        s = vclamp(s, T(0.0), T(1.0));
-    }
+    const T s = dense_root(y0.theta, h * f0.theta, y1.theta, h * f1.theta, mid, frac);
     return {hermite(s, y0.r, h * f0.r, y1.r, h * f1.r), hermite(s, y0.phi, h * f0.phi, y1.phi, h * f1.phi)};
+}
+// Escape of an error-controlled step: the direction (theta, phi) where the dense
+// output reaches r = escape_r (the fixed-step algorithm takes the first state
+// beyond it, which moves by a whole step when the step sequence changes).
+template <class T> struct DenseEscape {
+    T theta, phi;
+};
+template <class T>
+inline DenseEscape<T> kerr_dense_escape(const KState<T>& y0, const KState<T>& y1, T h, T r_esc, const KConst<T>& c) {
+    const KState<T> f0 = kerr_rhs(y0, c), f1 = kerr_rhs(y1, c);
+    const T s = dense_root(y0.r, h * f0.r, y1.r, h * f1.r, r_esc, (r_esc - y0.r) / (y1.r - y0.r));
+    return {hermite(s, y0.theta, h * f0.theta, y1.theta, h * f1.theta),
+            hermite(s, y0.phi, h * f0.phi, y1.phi, h * f1.phi)};
 }
 
 // trace_kerr (geodesic_kernel.cu:303-403) with invariants recorded.
@@ -851,7 +868,15 @@ inline RayOut<T> trace_kerr(const Uni<T>& u, const CamFrame<T>& f, const CameraR
     for (std::uint32_t step = 0; step < u.max_steps; ++step) {
         if (!xisfinite(s.r) || s.r < cut) return finish(CLS_HORIZON);
         if (s.r > u.escape_r) {
-            acc = acc + escape_color(u, s.theta, s.phi) * trans;
+            T th_e = s.theta, ph_e = s.phi;
+            if (adaptive && o.steps > 0) {
+                const DenseEscape<T> de = kerr_dense_escape(prev, s, h_used, u.escape_r, c);
+                th_e = de.theta;
+                ph_e = de.phi;
+            }
+            o.esc_theta = val(th_e);
+            o.esc_phi = val(ph_e);
+            acc = acc + escape_color(u, th_e, ph_e) * trans;
             return finish(CLS_ESCAPE);
         }
         const Crossing<T> x = detect_crossing(prev_theta, s.theta);
@@ -926,6 +951,8 @@ inline RayOut<T> trace_schw(const Uni<T>& u, const CamFrame<T>& f, const CameraR
     for (std::uint32_t step = 0; step < u.max_steps; ++step) {
         if (!xisfinite(s.r) || s.r < u.horizon_cut) return finish(CLS_HORIZON);
         if (s.r > u.escape_r) {
+            o.esc_theta = val(s.theta);
+            o.esc_phi = val(s.phi);
             acc = acc + escape_color(u, s.theta, s.phi) * trans;
             return finish(CLS_ESCAPE);
         }

@@ -80,6 +80,8 @@ template <class T> void ray_impl(const void* raw, const OrcExtras* ex, std::uint
     out[13] = r.Lz;
     out[14] = r.r_min;
     out[15] = r.th_min;
+    out[16] = r.esc_theta;
+    out[17] = r.esc_phi;
 }
 
 // Scene parameters of the differentiable renderer -> Uniforms in T.
@@ -135,9 +137,9 @@ int orc_render(const void* uniforms, int precision, const OrcExtras* ex, const s
 }
 
 int orc_trace_ray(const void* uniforms, int precision, const OrcExtras* ex, std::uint32_t gx, std::uint32_t gy,
-                  double ox, double oy, double* out16) {
-    if (precision == 32) ray_impl<float>(uniforms, ex, gx, gy, ox, oy, out16);
-    else if (precision == 64) ray_impl<double>(uniforms, ex, gx, gy, ox, oy, out16);
+                  double ox, double oy, double* out18) {
+    if (precision == 32) ray_impl<float>(uniforms, ex, gx, gy, ox, oy, out18);
+    else if (precision == 64) ray_impl<double>(uniforms, ex, gx, gy, ox, oy, out18);
     else return -1;
     return 0;
 }

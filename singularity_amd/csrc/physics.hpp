@@ -255,26 +255,47 @@ template <class R> __device__ __forceinline__ R hermite_ds(R s, R x0, R d0, R x1
     const R s2 = s * s;
     return (R(6) * s2 - R(6) * s) * (x0 - x1) + (R(3) * s2 - R(4) * s + R(1)) * d0 + (R(3) * s2 - R(2) * s) * d1;
 }
-template <class R>
-__device__ __forceinline__ void kerr_dense_crossing(R r0, R th0, R ph0, R pr0, R pt0, R r1, R th1, R ph1, R pr1,
-                                                    R pt1, R h, R frac, const SceneConst<R>& sc, const KerrK<R>& k,
-                                                    R& rc, R& phc) {
-    R st, ct, fr0, fth0, fph0, fr1, fth1, fph1, dpr, dpt;
-    dsincos(th0, &st, &ct);
-    kerr_rhs_sc(r0, pr0, pt0, st, ct, sc, k, fr0, fth0, fph0, dpr, dpt);
-    dsincos(th1, &st, &ct);
-    kerr_rhs_sc(r1, pr1, pt1, st, ct, sc, k, fr1, fth1, fph1, dpr, dpt);
-    const R mid = R(1.5707963267948966192313216916397514);
-    const R dt0 = h * fth0, dt1 = h * fth1;
-    R s = frac;
+// Three Newton iterations on hermite(s) = target from s, clamped to [0, 1].
+template <class R> __device__ __forceinline__ R dense_root(R x0, R d0, R x1, R d1, R target, R s) {
     for (int it = 0; it < 3; ++it) {
-        const R g = hermite(s, th0, dt0, th1, dt1) - mid;
-        const R dg = hermite_ds(s, th0, dt0, th1, dt1);
+        const R g = hermite(s, x0, d0, x1, d1) - target;
+        const R dg = hermite_ds(s, x0, d0, x1, d1);
         if (dg != R(0)) s = s - g / dg;
         s = (s < R(0)) ? R(0) : ((s > R(1)) ? R(1) : s);
     }
-    rc = hermite(s, r0, h * fr0, r1, h * fr1);
-    phc = hermite(s, ph0, h * fph0, ph1, h * fph1);
+    return s;
+}
+
+// The step's end states (r, theta, phi) and their derivatives times h.
+template <class R> struct KerrDense {
+    R r0, th0, ph0, r1, th1, ph1, dr0, dth0, dph0, dr1, dth1, dph1;
+};
+template <class R>
+__device__ __forceinline__ KerrDense<R> kerr_dense(R r0, R th0, R ph0, R pr0, R pt0, R r1, R th1, R ph1, R pr1,
+                                                   R pt1, R h, const SceneConst<R>& sc, const KerrK<R>& k) {
+    KerrDense<R> d;
+    R st, ct, dpr, dpt;
+    dsincos(th0, &st, &ct);
+    kerr_rhs_sc(r0, pr0, pt0, st, ct, sc, k, d.dr0, d.dth0, d.dph0, dpr, dpt);
+    dsincos(th1, &st, &ct);
+    kerr_rhs_sc(r1, pr1, pt1, st, ct, sc, k, d.dr1, d.dth1, d.dph1, dpr, dpt);
+    d.r0 = r0; d.th0 = th0; d.ph0 = ph0; d.r1 = r1; d.th1 = th1; d.ph1 = ph1;
+    d.dr0 *= h; d.dth0 *= h; d.dph0 *= h; d.dr1 *= h; d.dth1 *= h; d.dph1 *= h;
+    return d;
+}
+template <class R> __device__ __forceinline__ void kerr_dense_crossing(const KerrDense<R>& d, R frac, R& rc, R& phc) {
+    const R mid = R(1.5707963267948966192313216916397514);
+    const R s = dense_root(d.th0, d.dth0, d.th1, d.dth1, mid, frac);
+    rc = hermite(s, d.r0, d.dr0, d.r1, d.dr1);
+    phc = hermite(s, d.ph0, d.dph0, d.ph1, d.dph1);
+}
+// Escape of an error-controlled step: the direction where the dense output
+// reaches r = r_esc (the first state beyond it moves by a whole step whenever
+// the step sequence changes, e.g. between FP32 and FP64).
+template <class R> __device__ __forceinline__ void kerr_dense_escape(const KerrDense<R>& d, R r_esc, R& th, R& ph) {
+    const R s = dense_root(d.r0, d.dr0, d.r1, d.dr1, r_esc, (r_esc - d.r0) / (d.r1 - d.r0));
+    th = hermite(s, d.th0, d.dth0, d.th1, d.dth1);
+    ph = hermite(s, d.ph0, d.dph0, d.ph1, d.dph1);
 }
 
 // ---- Schwarzschild geodesic RHS (geodesic_math.h:101-123) ----------------------

@@ -46,6 +46,10 @@ template <class S> struct RayState {
 constexpr float kGrazeTheta = 2e-6f;
 constexpr float kEdgeRel = 1e-5f;
 constexpr float kStarCell = 1e-3f;
+// With error-controlled steps FP32 and FP64 take different step sequences, so
+// their exit directions differ by the integration error (~1e-5 rad), not by
+// rounding: the starfield guard widens to 1e-2 cell.
+constexpr float kStarCellAdaptive = 1e-2f;
 
 __device__ __forceinline__ std::uint64_t shfl_u64(std::uint64_t v, int src) {
     const std::uint32_t lo = __shfl(std::uint32_t(v), src);
@@ -64,7 +68,20 @@ __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneCons
     if (y.steps >= sc.max_steps) return kTermBudget;
     R r = R(y.r), th = R(y.th);
     if (!__builtin_isfinite(r) || r < sc.cut) return kTermHorizon;
-    if (r > sc.escape_r) return kTermEscape;
+    if (r > sc.escape_r) {
+        if (ADAPT) {
+            R the = th, phe = R(y.ph);
+            if (y.steps > 0u) {
+                const KerrK<R> k = kerr_k(sc, R(y.v2), R(y.v3));
+                const KerrDense<R> d = kerr_dense(R(y.prev_r), R(y.prev_th), R(y.prev_uph), R(y.prev_v0),
+                                                  R(y.prev_v1), r, th, R(y.ph), R(y.v0), R(y.v1), R(y.hprev), sc, k);
+                kerr_dense_escape(d, sc.escape_r, the, phe);
+            }
+            y.aux0 = S(the);
+            y.aux1 = S(phe);
+        }
+        return kTermEscape;
+    }
     const R mid = R(1.5707963267948966192313216916397514);
     if (sc.flags & SING_FLAG_DISC_ON) {
         const R pth = R(y.prev_th);
@@ -76,8 +93,9 @@ __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneCons
             R phc = R(0);
             if (ADAPT) {
                 const KerrK<R> k = kerr_k(sc, R(y.v2), R(y.v3));
-                kerr_dense_crossing(pr0, pth, R(y.prev_uph), R(y.prev_v0), R(y.prev_v1), r, th, R(y.ph), R(y.v0),
-                                    R(y.v1), R(y.hprev), frac, sc, k, rc, phc);
+                const KerrDense<R> d = kerr_dense(pr0, pth, R(y.prev_uph), R(y.prev_v0), R(y.prev_v1), r, th, R(y.ph),
+                                                  R(y.v0), R(y.v1), R(y.hprev), sc, k);
+                kerr_dense_crossing(d, frac, rc, phc);
             }
             if (TRACK) {
                 const R tol = R(kEdgeRel) * rc;
@@ -203,6 +221,10 @@ __device__ __forceinline__ Rec make_rec(const RayState<S>& y, std::uint32_t term
         rc.a = float(y.aux0);
         rc.b = ADAPT ? float(y.aux1) : float(y.ph);
         rc.c = KERR ? float(y.v3) : float(y.aux1);
+    } else if (ADAPT && term == kTermEscape) {
+        rc.a = float(y.aux0);  // dense-output direction at the escape radius
+        rc.b = float(y.aux1);
+        rc.c = 0.0f;
     } else {
         rc.a = float(y.th);
         rc.b = float(y.ph);
@@ -365,7 +387,7 @@ __global__ __launch_bounds__(256) void audit_kernel(SceneConst<float> sf, Chunk 
     for (std::uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
         const Rec r = ch.rec[i];
         const std::uint32_t term = r.meta & 7u;
-        if (term == kTermEscape && starfield_ambiguous(r.a, r.b, kStarCell)) {
+        if (term == kTermEscape && starfield_ambiguous(r.a, r.b, sf.tol > 0.0f ? kStarCellAdaptive : kStarCell)) {
             const std::uint32_t slot = atomicAdd(&ctl->band_count, 1u);
             ch.band[slot] = i;
             atomicSub((unsigned long long*)&ctl->rays, 1ull);

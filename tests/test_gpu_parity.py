@@ -254,3 +254,16 @@ def test_adaptive_rejected_for_schwarzschild(backends):
         render(backends, u, sa.PREC_FP32, adapt_tol=1e-6)
     with pytest.raises(sa.SingError):
         render(backends, config2(16, 9, 1), sa.PREC_FP32, adapt_tol=-1.0)
+
+
+def test_adaptive_mixed_close_to_fp64_with_starfield(backends):
+    """Escape directions of error-controlled steps come from the dense output at the
+    escape radius, and starfield-edge rays go to FP64: mixed agrees with all-FP64
+    (same algorithm) on >= 99.95 % of pixels within 1e-3."""
+    u = config2(480, 270, 1)
+    f64 = render(backends, u, sa.PREC_FP64, adapt_tol=ADAPT_TOL, turb_seed=42)
+    fm = render(backends, u, sa.PREC_MIXED, adapt_tol=ADAPT_TOL, turb_seed=42)
+    rel = rel_err(fm.hdr, f64.hdr)
+    print("mixed within 1e-3 of fp64:", (rel <= 1e-3).mean(), "fp64 rays", fm.stats.fp64_rays, flush=True)
+    assert (rel <= 1e-3).mean() >= 0.9995
+    assert (fm.cls == f64.cls).mean() >= 0.9999

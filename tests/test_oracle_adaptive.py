@@ -12,6 +12,10 @@ pins are mathematical:
   h = 0.004 M (same class, crossing radius within 5e-6 M: the cubic dense
   output at the crossing; the linear interpolation of the fixed-step
   algorithm would be off by ~3e-3 M on these 6 M steps);
+* an escaping ray's exit direction (dense output at r = escape radius) agrees
+  with the fine fixed-step one to 3e-6 rad (the fine run's own overshoot of one
+  0.004 M step is ~1e-6; taking the first state beyond the radius, as the
+  fixed-step algorithm does, would be off by ~1e-3 on 6 M steps);
 * the controller halves on rejection (never below 1e-3 h_step), grows 4x after an
   exact step and caps growth at 16 h_step.
 """
@@ -118,3 +122,16 @@ def test_whole_ray_converges_to_fine_fixed_step(orc):
             assert abs(a["r_cross"] - f["r_cross"]) <= 5e-6, (gx, gy, a["r_cross"], f["r_cross"])
         assert a["steps"] < f["steps"] / 5
     assert hits == 6
+
+
+def test_escape_direction_converges_to_fine_fixed_step(orc):
+    u = config2(48, 27, 1)
+    fine = scene_uniforms(spin=0.94, r_in=u.disc_r_inner, r_out=25.0, width=48, height=27, dist=40.0,
+                          elev=np.radians(10.0), fov=30.0, h=0.004, max_steps=400000)
+    ex = orc.Extras()
+    ex.adapt_tol = 1e-10
+    for gx, gy in ((0, 0), (47, 0), (0, 26), (47, 26)):
+        a = orc.trace_ray(u, gx, gy, precision=64, extras=ex)
+        f = orc.trace_ray(fine, gx, gy, precision=64)
+        assert a["cls"] == f["cls"] == 1  # escape (oracle class numbering)
+        assert abs(a["esc_theta"] - f["esc_theta"]) <= 3e-6 and abs(a["esc_phi"] - f["esc_phi"]) <= 3e-6, (a, f)

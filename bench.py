@@ -176,6 +176,70 @@ def c3_leg() -> dict:
     return res
 
 
+def batch_scenes(seed: int = 20261203, n: int = 64) -> list:
+    """64 uneven Kerr scenes from one recorded seed (numpy PCG64): resolution from
+    {128^2, 256^2, 512^2, 1024^2, 1920x1080}, a/M ~ U[0, 0.998], inclination ~
+    U[5, 89] deg, FOV ~ U[20, 60] deg, spp ~ U{1..64}, r_out ~ U[12, 40] M,
+    r_in = ISCO_pro(a), camera 30 M."""
+    rng = np.random.default_rng(seed)
+    sizes = [(128, 128), (256, 256), (512, 512), (1024, 1024), (1920, 1080)]
+    out = []
+    for _ in range(n):
+        W, H = sizes[int(rng.integers(0, len(sizes)))]
+        a = float(rng.uniform(0.0, 0.998))
+        incl = float(rng.uniform(5.0, 89.0))
+        fov = float(rng.uniform(20.0, 60.0))
+        spp = int(rng.integers(1, 65))
+        r_out = float(rng.uniform(12.0, 40.0))
+        s = sa.scene_defaults()
+        s.metric = sa.METRIC_KERR if a > 0.0 else sa.METRIC_SCHWARZSCHILD
+        s.spin_a_over_M = a
+        s.disc_inner_M = sa.isco_prograde(a)
+        s.disc_outer_M = r_out
+        s.supersample = spp
+        cam = sa.orbital_camera(30.0, np.radians(90.0 - incl), 0.0, fov)
+        out.append(sa.pack_uniforms(s, cam, W, H, 0.0))
+    return out
+
+
+def c4_leg(seed: int = 20261203, reps: int = 2) -> dict:
+    """64 uneven scenes in one persistent batched launch (recorded as a hipGraph and
+    replayed), mixed precision, launch shape autotuned once; against rendering the
+    same scenes one after another through sing_render."""
+    scenes = batch_scenes(seed)
+    rays = sum(u.width * u.height * u.supersample for u in scenes)
+    be = sa.HipBackend(128, 128)
+    opts = sa.BatchOpts(precision=sa.PREC_MIXED, use_graph=1, autotune=1)
+    _, st = sa.render_batch(be, scenes, opts)  # tunes, records the graph
+    log(f"c4: tuned step_batch={st.step_batch} waves_per_cu={st.waves_per_cu}, {st.kernel_ms:.0f} ms")
+    opts.autotune = 0
+    times, walls = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        _, st = sa.render_batch(be, scenes, opts)
+        walls.append(time.perf_counter() - t0)
+        times.append(st.kernel_ms)
+    seq = 0.0
+    for u in scenes:
+        be.resize(u.width, u.height)
+        seq += be.render(u, sa.RenderOpts(precision=sa.PREC_MIXED, outputs=sa.OUT_RGBA8,
+                                          step_batch=st.step_batch, waves_per_cu=st.waves_per_cu)).kernel_ms
+    be.shutdown()
+    kern = float(np.mean(times))
+    res = {
+        "workload": f"64 Kerr scenes (seed {seed}): res in {{128^2..1920x1080}}, a~U[0,0.998], incl~U[5,89] deg, "
+                    f"FOV~U[20,60], spp~U{{1..64}}, r_out~U[12,40], r_in=ISCO, mixed precision, {rays} rays",
+        "rays": rays, "mrays_s": round(rays / (kern * 1e-3) / 1e6, 2), "kernel_ms": round(kern, 1),
+        "wall_ms_incl_output_copies": round(float(np.mean(walls)) * 1e3, 1), "copy_ms": round(st.copy_ms, 1),
+        "steps": int(st.steps), "fp64_rays": int(st.fp64_rays), "groups": int(st.groups),
+        "graph_replayed": int(st.graph_replayed), "step_batch": int(st.step_batch),
+        "waves_per_cu": int(st.waves_per_cu),
+        "sequential_single_scene_ms": round(seq, 1), "batch_speedup_vs_sequential": round(seq / kern, 3),
+    }
+    log(f"c4: {res['mrays_s']} Mrays/s, batch {kern:.0f} ms vs sequential {seq:.0f} ms")
+    return res
+
+
 def adjoint_leg(evals: int = 50) -> dict:
     """grads/s of the 512x512 inverse render: one evaluation = forward with
     checkpoints + discrete adjoint + reduction of dL/d(a/M, incl, r_in, r_out, Ts),
@@ -298,6 +362,7 @@ def main() -> int:
     if not args.no_extra:
         out["adaptive_bloom_1080p"] = c2_leg()
         out["kerr_a0998_4k"] = c3_leg()
+        out["batch_64_scenes"] = c4_leg()
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg()
         out["grads_per_s"] = out["adjoint"]["mixed"]["grads_per_s"]

@@ -278,6 +278,52 @@ int sing_diff_loss_grad(SingContext* ctx, const SingUniforms* base, const SingDi
                         const SingDiffCamera* cam, const SingDiffOpts* opts, const float* target,
                         const float* weights, double* loss, double* grad, float* hdr_out, SingDiffStats* stats);
 
+/* --- batched render ------------------------------------------------------------
+ * No counterpart in the reference (cuda_cli renders one scene per launch,
+ * cuda_cli/main.cpp:243-299).  Renders n scenes of any sizes and sample counts
+ * (each a 160-byte SingUniforms, semantics as sing_render) with one persistent
+ * trace launch over all of them: each wavefront works on one scene at a time
+ * and moves to another scene with work left when its own runs dry.  Outputs
+ * (reference RGBA8 semantics, optional HDR and class) go to caller host buffers
+ * per scene.  The context's own frame size is unrelated to the batch's sizes. */
+typedef struct SingBatchItem {
+    const SingUniforms* uniforms;
+    uint8_t* rgba;   /* W*H*4, or NULL */
+    float* hdr;      /* W*H*3 mean radiance before exposure, or NULL */
+    uint8_t* cls;    /* W*H class of sample 0, or NULL */
+} SingBatchItem;
+
+typedef struct SingBatchOpts {
+    uint32_t precision;     /* SING_PREC_* */
+    uint32_t use_graph;     /* 1: record the batch's launches once as a hipGraph and replay it for batches
+                               of the same layout (sizes, sample counts, metrics, options) */
+    uint32_t autotune;      /* 1: time launch-shape candidates on a 1/16 sample of this batch and keep the
+                               fastest for this (metric, precision, stepping) in the context */
+    uint32_t step_batch;    /* RK4 steps between lane refills (0 = tuned or 8) */
+    uint32_t waves_per_cu;  /* persistent waves per CU (0 = tuned or 16) */
+    float adapt_tol;        /* > 0: error-controlled stepping (applies to the Kerr scenes) */
+    uint32_t turb_seed;     /* seeded disc-band phases, as SingRenderOpts */
+    uint32_t record_budget_mib; /* device memory for one group's termination records (0 = 32 GiB); scenes are
+                                   split into groups that fit, traced one group after another */
+    uint32_t reserved[4];
+} SingBatchOpts;
+
+typedef struct SingBatchStats {
+    double kernel_ms;       /* all kernels of the batch (one graph launch when use_graph) */
+    double copy_ms;         /* device -> host copies of the outputs */
+    uint64_t rays;
+    uint64_t steps;
+    uint64_t fp64_rays;
+    uint64_t rejected;
+    uint32_t step_batch;    /* launch shape used */
+    uint32_t waves_per_cu;
+    uint32_t graph_replayed;   /* 1 if an existing graph was replayed */
+    uint32_t groups;           /* record-memory groups the batch was split into */
+} SingBatchStats;
+
+int sing_render_batch(SingContext* ctx, const SingBatchItem* items, uint32_t n, const SingBatchOpts* opts,
+                      SingBatchStats* stats);
+
 #ifdef __cplusplus
 }
 #endif

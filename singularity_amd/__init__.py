@@ -154,9 +154,6 @@ def lib() -> C.CDLL:
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
-        for name in ("sing_backward", "sing_render_batch"):
-            if hasattr(L, name):
-                pass  # typed by their own modules
         _LIB = L
     return _LIB
 
@@ -446,3 +443,49 @@ def adam_fit(be: "HipBackend", base: Uniforms, start, target: np.ndarray, cam: D
         x = x - np.where(free, step, 0.0)
         x[0] = min(max(x[0], 0.0), 0.99)
     return x, history
+
+
+# --- batched render (include/sing_abi.h: sing_render_batch) -------------------
+class BatchItem(C.Structure):
+    _fields_ = [("uniforms", C.POINTER(Uniforms)), ("rgba", C.c_void_p), ("hdr", C.c_void_p), ("cls", C.c_void_p)]
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("precision", C.c_uint32), ("use_graph", C.c_uint32), ("autotune", C.c_uint32),
+                ("step_batch", C.c_uint32), ("waves_per_cu", C.c_uint32), ("adapt_tol", C.c_float),
+                ("turb_seed", C.c_uint32), ("record_budget_mib", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("copy_ms", C.c_double), ("rays", C.c_uint64), ("steps", C.c_uint64),
+                ("fp64_rays", C.c_uint64), ("rejected", C.c_uint64), ("step_batch", C.c_uint32),
+                ("waves_per_cu", C.c_uint32), ("graph_replayed", C.c_uint32), ("groups", C.c_uint32)]
+
+
+_SIGS["sing_render_batch"] = (C.c_int, [C.c_void_p, C.POINTER(BatchItem), C.c_uint32, C.POINTER(BatchOpts),
+                                        C.POINTER(BatchStats)])
+
+
+def render_batch(be: "HipBackend", scenes: list, opts: BatchOpts | None = None, hdr: bool = False,
+                 cls: bool = False) -> tuple[list[Frame], BatchStats]:
+    """Render a list of Uniforms in one batched launch; returns one Frame per scene."""
+    n = len(scenes)
+    items = (BatchItem * n)()
+    frames, keep = [], []
+    for i, u in enumerate(scenes):
+        W, H = u.width, u.height
+        f = Frame(rgba=np.empty((H, W, 4), np.uint8), hdr=np.empty((H, W, 3), np.float32) if hdr else None,
+                  cls=np.empty((H, W), np.uint8) if cls else None)
+        uc = u.copy()
+        keep.append(uc)
+        items[i].uniforms = C.pointer(uc)
+        items[i].rgba = f.rgba.ctypes.data
+        items[i].hdr = f.hdr.ctypes.data if hdr else None
+        items[i].cls = f.cls.ctypes.data if cls else None
+        frames.append(f)
+    st = BatchStats()
+    o = opts if opts is not None else BatchOpts()
+    be._check(lib().sing_render_batch(be._ctx, items, n, C.byref(o), C.byref(st)))
+    for f in frames:
+        f.stats = st
+    return frames, st

@@ -195,6 +195,7 @@ void sing_destroy(SingContext* c) {
     hipFree(c->d_rec);
     hipFree(c->d_band);
     c->free_diff();
+    c->free_batch();
     for (auto& e : c->events) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;

@@ -11,6 +11,7 @@
 
 #include "../../include/sing_abi.h"
 #include "adjoint.hpp"
+#include "batch.hpp"
 #include "trace.hpp"
 
 void sing_set_global_error(const std::string& m);
@@ -45,6 +46,23 @@ struct SingContext {
     float* d_weight = nullptr;
     double* d_out6 = nullptr;
     std::size_t diff_P = 0, diff_ncp = 0, diff_C = 0;
+    // batched render
+    sing::BatchScene* d_bscenes = nullptr;
+    std::uint64_t* d_brec_bases = nullptr;
+    std::uint64_t* d_bpix_bases = nullptr;
+    sing::SceneQueue* d_bqueues = nullptr;
+    std::size_t b_scene_cap = 0, b_rb_cap = 0, b_pb_cap = 0, b_q_cap = 0;
+    sing::Rec* d_brec = nullptr;
+    std::uint32_t* d_bband = nullptr;
+    std::size_t b_rec_cap = 0, b_band_cap = 0;
+    uchar4* d_brgba = nullptr;
+    float* d_bhdr = nullptr;
+    std::uint8_t* d_bcls = nullptr;
+    std::size_t b_rgba_cap = 0, b_hdr_cap = 0, b_cls_cap = 0;
+    hipGraph_t b_graph = nullptr;
+    hipGraphExec_t b_graph_exec = nullptr;
+    std::vector<std::uint64_t> b_graph_key;
+    std::vector<std::pair<std::uint64_t, std::uint64_t>> b_tuned;  // key -> step_batch << 32 | waves_per_cu
 
     int fail(int code, const char* fmt, ...);
     int hip_fail(hipError_t e, const char* where);
@@ -56,4 +74,6 @@ struct SingContext {
     sing::LaunchShape launch_shape(const SingRenderOpts& o) const;
     int ensure_diff(std::size_t P, std::size_t ncp, std::size_t C);
     void free_diff();
+    void free_batch();
+    void drop_batch_graph();
 };

@@ -20,234 +20,13 @@
 
 #include "physics.hpp"
 #include "trace.hpp"
+#include "trace_core.hpp"
 
 namespace sing {
 
 namespace {
 
-template <class S> struct RayState {
-    S r, th, ph;
-    S v0, v1, v2, v3;  // Kerr: p_r, p_theta, E, L_z.  Schwarzschild: u^t, u^r, u^theta, u^phi
-    S prev_r, prev_th, prev_uph;  // prev_uph: Schwarzschild u^phi; error-controlled Kerr: phi
-    S prev_v0, prev_v1, hprev;    // error-controlled Kerr: p_r, p_theta before the step, its size
-    S aux0, aux1;      // disc crossing radius; u^phi (Schwarzschild) or phi (error-controlled Kerr) at it
-    S near_eq;         // mixed precision: min |theta - pi/2| over the steps (0 = ambiguous crossing)
-    S hcur;            // error-controlled stepping: current step size
-    std::uint32_t steps;
-    std::uint32_t tries;  // error-controlled stepping: rejections of the current step
-};
-
-// Mixed-precision deferral thresholds.  Comparing FP32 and FP64 traces of the
-// same rays, outcomes differ only for rays near a discontinuity of the
-// discrete algorithm: a state within kGrazeTheta of the equator (the crossing
-// test flips), a crossing within kEdgeRel of an annulus edge, or an escape
-// within kStarCell cells of a starfield boundary next to a star.  Such rays
-// are re-traced in FP64.
-constexpr float kGrazeTheta = 2e-6f;
-constexpr float kEdgeRel = 1e-5f;
-constexpr float kStarCell = 1e-3f;
-// With error-controlled steps FP32 and FP64 take different step sequences, so
-// their exit directions differ by the integration error (~1e-5 rad), not by
-// rounding: the starfield guard widens to 1e-2 cell.
-constexpr float kStarCellAdaptive = 1e-2f;
-
-__device__ __forceinline__ std::uint64_t shfl_u64(std::uint64_t v, int src) {
-    const std::uint32_t lo = __shfl(std::uint32_t(v), src);
-    const std::uint32_t hi = __shfl(std::uint32_t(v >> 32), src);
-    return (std::uint64_t(hi) << 32) | lo;
-}
-
-// Termination checks in the reference's order (geodesic_kernel.cu:351-401)
-// followed by one RK4 step.  Arithmetic in R, storage in S.  With ADAPT (Kerr)
-// the step is one attempt of the error-controlled step; a rejected attempt
-// leaves the state unchanged (the checks it repeats are idempotent) and
-// counts in *rejected.
-template <class R, class S, bool KERR, bool TRACK = false, bool ADAPT = false>
-__device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneConst<R>& sc,
-                                                 std::uint32_t* rejected = nullptr) {
-    if (y.steps >= sc.max_steps) return kTermBudget;
-    R r = R(y.r), th = R(y.th);
-    if (!__builtin_isfinite(r) || r < sc.cut) return kTermHorizon;
-    if (r > sc.escape_r) {
-        if (ADAPT) {
-            R the = th, phe = R(y.ph);
-            if (y.steps > 0u) {
-                const KerrK<R> k = kerr_k(sc, R(y.v2), R(y.v3));
-                const KerrDense<R> d = kerr_dense(R(y.prev_r), R(y.prev_th), R(y.prev_uph), R(y.prev_v0),
-                                                  R(y.prev_v1), r, th, R(y.ph), R(y.v0), R(y.v1), R(y.hprev), sc, k);
-                kerr_dense_escape(d, sc.escape_r, the, phe);
-            }
-            y.aux0 = S(the);
-            y.aux1 = S(phe);
-        }
-        return kTermEscape;
-    }
-    const R mid = R(1.5707963267948966192313216916397514);
-    if (sc.flags & SING_FLAG_DISC_ON) {
-        const R pth = R(y.prev_th);
-        const R dp = pth - mid, dc = th - mid;
-        if (dp * dc < R(0)) {
-            const R frac = (mid - pth) / (th - pth);
-            const R pr0 = R(y.prev_r);
-            R rc = pr0 + frac * (r - pr0);
-            R phc = R(0);
-            if (ADAPT) {
-                const KerrK<R> k = kerr_k(sc, R(y.v2), R(y.v3));
-                const KerrDense<R> d = kerr_dense(pr0, pth, R(y.prev_uph), R(y.prev_v0), R(y.prev_v1), r, th, R(y.ph),
-                                                  R(y.v0), R(y.v1), R(y.hprev), sc, k);
-                kerr_dense_crossing(d, frac, rc, phc);
-            }
-            if (TRACK) {
-                const R tol = R(kEdgeRel) * rc;
-                const R d_in = rc - sc.r_in, d_out = rc - sc.r_out;
-                if ((d_in < tol && d_in > -tol) || (d_out < tol && d_out > -tol)) y.near_eq = S(0);
-            }
-            if (rc >= sc.r_in && rc <= sc.r_out) {
-                y.aux0 = S(rc);
-                if (ADAPT) y.aux1 = S(phc);
-                if (!KERR) {
-                    const R pu = R(y.prev_uph);
-                    y.aux1 = S(pu + frac * (R(y.v3) - pu));
-                }
-                return kTermDisc;
-            }
-        }
-    }
-    y.prev_th = y.th;
-    y.prev_r = y.r;
-    R ph = R(y.ph);
-    if (KERR) {
-        R pr = R(y.v0), pt = R(y.v1);
-        const KerrK<R> k = kerr_k(sc, R(y.v2), R(y.v3));
-        if (ADAPT) {
-            R hc = R(y.hcur), hused;
-            y.prev_uph = y.ph;
-            y.prev_v0 = y.v0;
-            y.prev_v1 = y.v1;
-            const bool acc = kerr_adaptive_step(r, th, ph, pr, pt, hc, y.tries, sc.tol, sc, k, hused);
-            y.hcur = S(hc);
-            if (!acc) {
-                *rejected += 1u;
-                return kTermLive;
-            }
-            y.hprev = S(hused);
-        } else {
-            kerr_rk4(r, th, ph, pr, pt, sc.h, sc, k);
-        }
-        y.v0 = S(pr);
-        y.v1 = S(pt);
-    } else {
-        y.prev_uph = y.v3;
-        R ut = R(y.v0), ur = R(y.v1), uth = R(y.v2), uph = R(y.v3);
-        schw_rk4(r, th, ph, ut, ur, uth, uph, sc.h, sc.rs);
-        y.v0 = S(ut);
-        y.v1 = S(ur);
-        y.v2 = S(uth);
-        y.v3 = S(uph);
-    }
-    y.r = S(r);
-    y.th = S(th);
-    y.ph = S(ph);
-    y.steps += 1;
-    if (TRACK) {
-        const R d = th - mid;
-        const R ad = d < R(0) ? -d : d;
-        if (ad < R(y.near_eq)) y.near_eq = S(ad);
-    }
-    return kTermLive;
-}
-
-// Ray initialisation from the camera (geodesic_kernel.cu:229-242, 303-347).
-// Returns false on a Kerr initialisation failure (no real root at the camera).
-// With margin_out, also evaluates the photon-orbit margin of the ray.
-template <class R, class S, bool KERR>
-__device__ __forceinline__ bool init_ray(RayState<S>& y, const SceneConst<R>& sc, std::uint32_t gx, std::uint32_t gy,
-                                         R ox, R oy, R* margin_out) {
-    R ur, uth, uph;
-    camera_ray(sc, gx, gy, ox, oy, ur, uth, uph);
-    y.r = S(sc.cam_r);
-    y.th = S(sc.cam_theta);
-    y.ph = S(sc.cam_phi);
-    y.prev_r = y.r;
-    y.prev_th = y.th;
-    y.steps = 0;
-    y.near_eq = S(1e30);
-    y.hcur = S(sc.h);
-    y.tries = 0u;
-    if (KERR) {
-        const R Bq = R(2) * sc.g_tp * uph;
-        const R Cq = sc.g_rr * ur * ur + sc.g_thth * uth * uth + sc.g_pp * uph * uph;
-        const R disc = Bq * Bq - R(4) * sc.g_tt * Cq;
-        if (disc < R(0)) return false;
-        const R ut = (-Bq - dsqrt(disc)) / (R(2) * sc.g_tt);
-        const R E = -(sc.g_tt * ut + sc.g_tp * uph);
-        const R L = sc.g_tp * ut + sc.g_pp * uph;
-        const R pt = sc.g_thth * uth;
-        y.v0 = S(sc.g_rr * ur);
-        y.v1 = S(pt);
-        y.v2 = S(E);
-        y.v3 = S(L);
-        if (margin_out) {
-            const R Q = pt * pt + sc.q_cos2 * (L * L * sc.q_inv_s2 - sc.a2 * E * E);
-            const R rp = sc.M + dsqrt(dmax(sc.M * sc.M - sc.a2, R(0)));
-            const R m = photon_margin<R>(L / E, Q / (E * E), sc.a, sc.M, rp);
-            // no turning point: ingoing rays are captured (defer), outgoing escape directly
-            *margin_out = (m < R(0)) ? (ur < R(0) ? R(-1) : R(1e30)) : m;
-        }
-    } else {
-        const R cr = sc.cam_r;
-        const R kin = (ur * ur) / sc.f_cam + uth * uth * cr * cr + uph * uph * cr * cr * sc.sin_th * sc.sin_th;
-        const R ut = dsqrt(dmax(kin / sc.f_cam, R(0)));
-        y.v0 = S(ut);
-        y.v1 = S(ur);
-        y.v2 = S(uth);
-        y.v3 = S(uph);
-        y.prev_uph = y.v3;
-        if (margin_out) {
-            const R E = sc.f_cam * ut;
-            const R L2 = cr * cr * cr * cr * (uth * uth + sc.sin_th * sc.sin_th * uph * uph);
-            const R m = photon_margin<R>(R(0), L2 / (E * E), R(0), sc.M, sc.twoM);
-            *margin_out = (m < R(0)) ? (ur < R(0) ? R(-1) : R(1e30)) : m;
-        }
-    }
-    return true;
-}
-
-template <class S, bool KERR, bool ADAPT = false>
-__device__ __forceinline__ Rec make_rec(const RayState<S>& y, std::uint32_t term) {
-    Rec rc;
-    rc.meta = term | (y.steps << 4);
-    if (term == kTermDisc) {
-        rc.a = float(y.aux0);
-        rc.b = ADAPT ? float(y.aux1) : float(y.ph);
-        rc.c = KERR ? float(y.v3) : float(y.aux1);
-    } else if (ADAPT && term == kTermEscape) {
-        rc.a = float(y.aux0);  // dense-output direction at the escape radius
-        rc.b = float(y.aux1);
-        rc.c = 0.0f;
-    } else {
-        rc.a = float(y.th);
-        rc.b = float(y.ph);
-        rc.c = 0.0f;
-    }
-    return rc;
-}
-
-// Work item -> (record index, pixel, sample).  Items enumerate the chunk's
-// samples sample-major, pixels in 8x8 tiles so a wave starts on a compact tile.
-__device__ __forceinline__ bool item_to_ray(std::uint64_t item, const Chunk& ch, std::uint32_t W, std::uint32_t H,
-                                            std::uint32_t& rec_idx, std::uint32_t& gx, std::uint32_t& gy,
-                                            std::uint32_t& s) {
-    const std::uint64_t per_s = std::uint64_t(ch.n_tiles) * 64u;
-    const std::uint32_t sl = std::uint32_t(item / per_s);
-    const std::uint32_t rem = std::uint32_t(item - std::uint64_t(sl) * per_s);
-    const std::uint32_t tile = rem >> 6, w = rem & 63u;
-    gx = (tile % ch.tiles_x) * 8u + (w & 7u);
-    gy = (tile / ch.tiles_x) * 8u + (w >> 3);
-    s = ch.s0 + sl;
-    rec_idx = sl * ch.P + gy * W + gx;
-    return gx < W && gy < H;
-}
+using namespace core;
 
 // METRIC 0: Schwarzschild; 1: Kerr; 2: Kerr with error-controlled steps.
 // PASS 0: FP32 trace of every ray; 1: FP32 trace deferring photon-band rays;
@@ -319,6 +98,7 @@ __global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneC
                         if (deferred) {
                             const std::uint32_t slot = atomicAdd(&ctl->band_count, 1u);
                             ch.band[slot] = rec_idx;
+                            ch.rec[rec_idx] = deferred_marker();
                             need = true;
                         } else if (!started) {
                             Rec rc;
@@ -356,6 +136,7 @@ __global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneC
             if (defer) {
                 const std::uint32_t slot = atomicAdd(&ctl->band_count, 1u);
                 ch.band[slot] = rec_idx;
+                ch.rec[rec_idx] = deferred_marker();
             } else {
                 ch.rec[rec_idx] = make_rec<S, KERR, ADAPT>(y, term);
                 lane_steps += y.steps;
@@ -396,15 +177,6 @@ __global__ __launch_bounds__(256) void audit_kernel(SceneConst<float> sf, Chunk 
     }
 }
 
-__device__ __forceinline__ std::uint8_t class_of(std::uint32_t term) {
-    switch (term) {
-        case kTermHorizon: return SING_CLASS_HORIZON;
-        case kTermEscape: return SING_CLASS_ESCAPE;
-        case kTermDisc: return SING_CLASS_DISC;
-        case kTermBudget: return SING_CLASS_BUDGET;
-        default: return SING_CLASS_INITFAIL;
-    }
-}
 
 template <int METRIC>
 __global__ __launch_bounds__(256) void shade_kernel(SceneConst<float> sc, const Rec* __restrict__ rec,

@@ -1,5 +1,5 @@
 """Host side of the C ABI: the library loads, exports every symbol that
-include/sing_abi.h declares, and the scene ingestion (scene files, Scene
+include/*.h declares, and the scene ingestion (scene files, Scene
 defaults, orbital camera, uniform packing) behaves like the reference's
 (core/include/scene/scene_config.hpp, render_backend.hpp:46-99,
 cuda_cli/main.cpp:68-109,193-218, cuda_backend.cu:45-101).  No GPU calls."""
@@ -21,8 +21,10 @@ ROOT = Path(__file__).resolve().parents[1]
 
 
 def declared_symbols() -> list[str]:
-    text = (ROOT / "include" / "sing_abi.h").read_text()
-    return sorted(set(re.findall(r"\b(sing_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for h in sorted((ROOT / "include").glob("*.h")):
+        names |= set(re.findall(r"\b(sing_[a-z0-9_]+)\s*\(", h.read_text()))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
@@ -43,6 +45,25 @@ def test_create_fails_cleanly_without_device_or_bad_size():
     assert L.sing_render(None, None, None, None) == 1
     assert L.sing_capture_rgba8(None, None, 0) == 1
     assert L.sing_resize(None, 4, 4) == 1
+    assert L.sing_render_batch(None, None, 0, None, None) == 1
+    assert L.sing_diff_forward(None, None, None, None, None, None, None) == 1
+
+
+def test_python_mirror_types_every_declared_symbol():
+    # every declared entry point has a ctypes signature (or is a host diagnostic typed by its test)
+    typed = set(sa.exported_symbols())
+    diag = set(re.findall(r"\b(sing_[a-z0-9_]+)\s*\(", (ROOT / "include" / "sing_diag.h").read_text()))
+    assert set(declared_symbols()) - diag <= typed, set(declared_symbols()) - diag - typed
+
+
+def test_struct_sizes_match_header_layout():
+    # sizes the C compiler gives the ABI structs (include/sing_abi.h), mirrored by ctypes
+    assert C.sizeof(sa.Uniforms) == 160
+    assert C.sizeof(sa.RenderOpts) == 64
+    assert C.sizeof(sa.RenderStats) == 72
+    assert C.sizeof(sa.DiffOpts) == 40 and C.sizeof(sa.DiffStats) == 64
+    assert C.sizeof(sa.DiffParams) == 40 and C.sizeof(sa.DiffCamera) == 24
+    assert C.sizeof(sa.BatchItem) == 32 and C.sizeof(sa.BatchOpts) == 48 and C.sizeof(sa.BatchStats) == 64
 
 
 def test_scene_defaults_match_reference():

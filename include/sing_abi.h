@@ -113,7 +113,7 @@ typedef struct SingRenderOpts {
     uint32_t post;        /* SING_POST_* */
     uint32_t turb_seed;   /* seeded turbulence band phases; 0 = reference bands */
     float adapt_tol;      /* > 0: error-controlled RK4 (step doubling) with this per-step tolerance */
-    float band_width;     /* mixed precision: half-width of the photon-sphere band (0 = default) */
+    float band_width;     /* mixed precision: photon-orbit margin below which a ray is traced in FP64 (0 = 0.03) */
     float peak_T_scale;   /* multiplies disc_peak_T; 0 is read as 1 */
     uint32_t step_batch;  /* RK4 steps between lane-refill checks (0 = tuned default) */
     uint32_t waves_per_cu;/* persistent grid size in waves per CU (0 = tuned default) */

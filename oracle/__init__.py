@@ -202,12 +202,12 @@ def turbulence_extras(seed: int, adapt_tol: float = 0.0) -> Extras:
 
 def trace_ray(u, gx: int, gy: int, ox: float = 0.5, oy: float = 0.5, precision: int = 32,
               extras: Extras | None = None) -> dict:
-    o = np.zeros(18, np.float64)
+    o = np.zeros(19, np.float64)
     ex = C.byref(extras) if extras is not None else None
     strict().orc_trace_ray(_ubuf(u), precision, ex, gx, gy, ox, oy, _ptr(o))
     return {"rgb": o[0:3].copy(), "cls": int(o[3]), "steps": int(o[4]), "r_cross": o[5],
             "inv0": o[6:9].copy(), "inv1": o[9:12].copy(), "E": o[12], "Lz": o[13], "r_min": o[14],
-            "th_min": o[15], "esc_theta": o[16], "esc_phi": o[17]}
+            "th_min": o[15], "esc_theta": o[16], "esc_phi": o[17], "th_max": o[18]}
 
 
 def count_step(kerr: bool = True) -> dict:

@@ -633,6 +633,7 @@ template <class T> struct RayOut {
     double th_min = 1e30;        // smallest |theta - pi/2| over the states after the first
     int n_soft = 0;              // crossings with nonzero opacity (soft-edge model)
     double esc_theta = 0, esc_phi = 0;  // direction the escape colour is looked up at
+    double th_max = 0;           // largest |theta - pi/2| over the states after the first (pole proximity)
 };
 
 template <class T> inline T soft_alpha(T r, T r_in, T r_out, T w) {
@@ -904,6 +905,7 @@ inline RayOut<T> trace_kerr(const Uni<T>& u, const CamFrame<T>& f, const CameraR
         o.steps = step + 1;
         o.r_min = std::min(o.r_min, val(s.r));
         o.th_min = std::min(o.th_min, std::fabs(val(s.theta) - 1.5707963267948966));
+        o.th_max = std::max(o.th_max, std::fabs(val(s.theta) - 1.5707963267948966));
     }
     return finish(CLS_BUDGET);
 }
@@ -973,6 +975,7 @@ inline RayOut<T> trace_schw(const Uni<T>& u, const CamFrame<T>& f, const CameraR
         o.steps = step + 1;
         o.r_min = std::min(o.r_min, val(s.r));
         o.th_min = std::min(o.th_min, std::fabs(val(s.theta) - 1.5707963267948966));
+        o.th_max = std::max(o.th_max, std::fabs(val(s.theta) - 1.5707963267948966));
     }
     return finish(CLS_BUDGET);
 }

@@ -82,6 +82,7 @@ template <class T> void ray_impl(const void* raw, const OrcExtras* ex, std::uint
     out[15] = r.th_min;
     out[16] = r.esc_theta;
     out[17] = r.esc_phi;
+    out[18] = r.th_max;
 }
 
 // Scene parameters of the differentiable renderer -> Uniforms in T.
@@ -137,9 +138,9 @@ int orc_render(const void* uniforms, int precision, const OrcExtras* ex, const s
 }
 
 int orc_trace_ray(const void* uniforms, int precision, const OrcExtras* ex, std::uint32_t gx, std::uint32_t gy,
-                  double ox, double oy, double* out18) {
-    if (precision == 32) ray_impl<float>(uniforms, ex, gx, gy, ox, oy, out18);
-    else if (precision == 64) ray_impl<double>(uniforms, ex, gx, gy, ox, oy, out18);
+                  double ox, double oy, double* out19) {
+    if (precision == 32) ray_impl<float>(uniforms, ex, gx, gy, ox, oy, out19);
+    else if (precision == 64) ray_impl<double>(uniforms, ex, gx, gy, ox, oy, out19);
     else return -1;
     return 0;
 }

@@ -34,7 +34,15 @@ namespace sing {
 
 namespace {
 
+// Mixed precision (as the render): rays that hard_ray flags (photon-orbit
+// margin below kBandDiff, reaching 2 r+, reaching 0.03 rad of the polar axis)
+// go to FP64 before the march; after it, a state within kGrazeThetaDiff of the
+// equator or 0.03 rad of the polar axis, or a ray with emission that came
+// within 2 r+ of the hole (both caught before the march unless rounding moves
+// the ray across the threshold).
 constexpr float kGrazeThetaDiff = 2e-6f;
+constexpr float kBandDiff = 0.03f;
+constexpr float kPoleGapDiff = 1.5707963267948966f - 0.03f;
 constexpr std::uint32_t kMetaPrec64 = 1u;  // meta[3]: precision of the pixel's forward
 
 template <class R> __device__ __forceinline__ const DiffScene<R>& scene_of(const DiffSet* ds);
@@ -86,15 +94,14 @@ __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict
     const bool ok = diff_init(sc, gx, gy, y, E, L);
     meta[3] = (MODE >= 2) ? kMetaPrec64 : 0u;
     if (ok && MODE == 1) {
-        // photon-orbit margin of the ray (as in the render path)
+        // hard for FP32 (as in the render path): decided from the ray's constants
         float st, ct;
         dsincos(float(sc.cam_theta), &st, &ct);
         const float Ef = float(E), Lf = float(L), ptf = float(y.pt);
         const float Q = ptf * ptf + ct * ct * (Lf * Lf / fmaxf(st * st, 1e-6f) - float(sc.a * sc.a) * Ef * Ef);
         const float Mf = float(sc.M), af = float(sc.a);
         const float rp = Mf + sqrtf(fmaxf(Mf * Mf - af * af, 0.0f));
-        const float m = photon_margin<float>(Lf / Ef, Q / (Ef * Ef), af, Mf, rp);
-        const bool defer = (m < 0.0f) ? (y.pr < R(0)) : (m < 0.4f);
+        const bool defer = hard_ray<float>(Lf / Ef, Q / (Ef * Ef), af, Mf, rp, y.pr < R(0), kBandDiff);
         if (defer) {
             meta[3] = kMetaPrec64;
             b.list[atomicAdd(b.count, 1u)] = pix;
@@ -105,7 +112,7 @@ __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict
     R tau = R(1);
     std::uint32_t ncross = 0, step = 0, term = kTermInitFail;
     double sky[3] = {0.0, 0.0, 0.0};
-    R near_eq = R(1e30);
+    R near_eq = R(1e30), rmin = R(1e30);
     if (ok) {
         const KC<R> c{E, L, sc.a, sc.M};
         b.EL[pix] = double(E);
@@ -165,9 +172,12 @@ __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict
                 const R d = y.th - mid;
                 const R ad = d < R(0) ? -d : d;
                 if (ad < near_eq) near_eq = ad;
+                if (ad > R(kPoleGapDiff)) near_eq = R(0);  // near the polar axis
+                if (y.r < rmin) rmin = y.r;
             }
         }
-        if (MODE == 1 && float(near_eq) < kGrazeThetaDiff) {
+        const R rp = sc.M + hsqrt(hmax(sc.M * sc.M - sc.a * sc.a, R(0)));
+        if (MODE == 1 && (float(near_eq) < kGrazeThetaDiff || (ncross > 0u && rmin < R(2) * rp))) {
             meta[3] = kMetaPrec64;
             b.list[atomicAdd(b.count, 1u)] = pix;
             return;

@@ -79,8 +79,11 @@ __global__ __launch_bounds__(256) void batch_reset_kernel(SceneQueue* __restrict
 // METRIC 0: Schwarzschild; 1: Kerr; 2: Kerr with error-controlled steps.
 // PASS as trace_kernel: 0 FP32, 1 FP32 with deferral, 2 FP64, 3 FP64 over the
 // scenes' deferred lists.  Scenes [first, first + count) of the table.
+// The scene table is its own read-only, non-aliased parameter so that, with a
+// wave-uniform scene index, its fields are scalar loads (as kernel arguments are).
 template <int METRIC, int PASS>
-__global__ __launch_bounds__(256) void trace_batch_kernel(BatchLaunch L, std::uint32_t first, std::uint32_t count,
+__global__ __launch_bounds__(256) void trace_batch_kernel(const BatchScene* __restrict__ scenes, BatchLaunch L,
+                                                          std::uint32_t first, std::uint32_t count,
                                                           std::uint32_t step_batch, std::uint32_t sample_div) {
     constexpr bool KERR = (METRIC >= 1);
     constexpr bool ADAPT = (METRIC == 2);
@@ -97,7 +100,7 @@ __global__ __launch_bounds__(256) void trace_batch_kernel(BatchLaunch L, std::ui
     RayState<S> y{};
 
     for (;;) {
-        const BatchScene& bs = L.scenes[sid];
+        const BatchScene& bs = scenes[sid];
         SceneQueue& q = L.queues[sid];
         // (1) claim rays of the current scene for idle lanes
         if (!dry) {
@@ -145,13 +148,13 @@ __global__ __launch_bounds__(256) void trace_batch_kernel(BatchLaunch L, std::ui
                             bool started, deferred = false;
                             if (F64) {
                                 started = init_ray<double, S, KERR>(y, bs.sd, gx, gy, double(ox), double(oy),
-                                                                    (double*)nullptr);
+                                                                    (bool*)nullptr);
                             } else if (PASS == 1) {
-                                float margin = 1e30f;
-                                started = init_ray<float, S, KERR>(y, bs.sf, gx, gy, ox, oy, &margin);
-                                deferred = started && margin < bs.sf.band;
+                                bool hard = false;
+                                started = init_ray<float, S, KERR>(y, bs.sf, gx, gy, ox, oy, &hard);
+                                deferred = started && hard;
                             } else {
-                                started = init_ray<float, S, KERR>(y, bs.sf, gx, gy, ox, oy, (float*)nullptr);
+                                started = init_ray<float, S, KERR>(y, bs.sf, gx, gy, ox, oy, (bool*)nullptr);
                             }
                             need = false;
                             if (deferred) {
@@ -182,7 +185,7 @@ __global__ __launch_bounds__(256) void trace_batch_kernel(BatchLaunch L, std::ui
                 const std::uint32_t s2 = first + (sid - first + j) % count;
                 const SceneQueue& q2 = L.queues[s2];
                 const std::uint64_t lim = (PASS == 3) ? std::uint64_t(load_relaxed(&q2.band_count))
-                                                      : item_limit(L.scenes[s2].n_items, sample_div);
+                                                      : item_limit(scenes[s2].n_items, sample_div);
                 const std::uint64_t cur = load_relaxed((PASS == 3) ? &q2.next_band : &q2.next_item);
                 if (__builtin_amdgcn_readfirstlane(cur < lim ? 1u : 0u)) {
                     sid = __builtin_amdgcn_readfirstlane(s2);
@@ -212,7 +215,7 @@ __global__ __launch_bounds__(256) void trace_batch_kernel(BatchLaunch L, std::ui
         // (4) emit finished rays; in the mixed pass, ambiguous outcomes go to FP64
         if (term != kTermLive) {
             bool defer = false;
-            if (PASS == 1) defer = float(y.near_eq) < kGrazeTheta;
+            if (PASS == 1) defer = needs_fp64(y, term);
             if (defer) {
                 const std::uint32_t slot = atomicAdd(&q.band_count, 1u);
                 L.band[bs.rec_base + slot] = rec_idx;
@@ -328,12 +331,14 @@ hipError_t launch_batch_trace(const BatchLaunch& L, const BatchGroup& g, std::ui
 #define SING_BTRACE(P, G)                                                                                       \
     do {                                                                                                        \
         if (g.n_schw)                                                                                           \
-            hipLaunchKernelGGL((trace_batch_kernel<0, P>), dim3(G), block, 0, stream, L, g.first, g.n_schw, sb,   \
-                               sample_div);                                                                     \
+            hipLaunchKernelGGL((trace_batch_kernel<0, P>), dim3(G), block, 0, stream, L.scenes, L, g.first,       \
+                               g.n_schw, sb, sample_div);                                                       \
         if (n_kerr && adaptive)                                                                                 \
-            hipLaunchKernelGGL((trace_batch_kernel<2, P>), dim3(G), block, 0, stream, L, k0, n_kerr, sb, sample_div); \
+            hipLaunchKernelGGL((trace_batch_kernel<2, P>), dim3(G), block, 0, stream, L.scenes, L, k0, n_kerr, sb, \
+                               sample_div);                                                                     \
         else if (n_kerr)                                                                                        \
-            hipLaunchKernelGGL((trace_batch_kernel<1, P>), dim3(G), block, 0, stream, L, k0, n_kerr, sb, sample_div); \
+            hipLaunchKernelGGL((trace_batch_kernel<1, P>), dim3(G), block, 0, stream, L.scenes, L, k0, n_kerr, sb, \
+                               sample_div);                                                                     \
     } while (0)
     if (mode == SING_PREC_FP32) {
         SING_BTRACE(0, shape.blocks);

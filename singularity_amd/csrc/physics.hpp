@@ -553,4 +553,27 @@ template <class R> __device__ __forceinline__ R photon_margin(R xi, R eta, R a, 
     return v < R(0) ? -v : v;
 }
 
+// Mixed precision, before the march: is this ray hard for FP32?  From its
+// constants (xi = L_z/E, eta = Q/E^2) alone: close to a photon orbit (margin
+// below band); ingoing and able to reach r = 2 r+ (the radial potential is
+// non-negative there: captured rays and rays that turn just outside the hole,
+// whose errors the near-horizon metric amplifies); or able to come within
+// 0.03 rad of the polar axis (the theta turning point u = cos^2 theta solves
+// a^2 u^2 + (eta + xi^2 - a^2) u - eta = 0; near the axis the coordinate
+// singularity amplifies rounding and the pole damping makes the ray long).
+// Such rays go straight to the FP64 pass, so no FP32 work is wasted on them.
+template <class R>
+__device__ __forceinline__ bool hard_ray(R xi, R eta, R a, R M, R r_plus, bool ingoing, R band) {
+    const R m = photon_margin<R>(xi, eta, a, M, r_plus);
+    if (m >= R(0) && m < band) return true;
+    const R C = (xi - a) * (xi - a) + eta;
+    const R r1 = R(2) * r_plus, r2 = r1 * r1;
+    const R Rv = r2 * r2 + (a * a - xi * xi - eta) * r2 + R(2) * M * C * r1 - a * a * eta;
+    if (ingoing && Rv >= R(0)) return true;
+    if (eta < R(0)) return true;  // vortical orbit: confined to a cone about an axis
+    const R B = eta + xi * xi - a * a;
+    const R umax = eta > R(0) ? R(2) * eta / (B + dsqrt(B * B + R(4) * a * a * eta)) : R(0);
+    return R(1) - umax < R(8.9973e-4);  // sin^2(0.03)
+}
+
 }  // namespace sing

@@ -124,7 +124,7 @@ template <class R> inline SceneConst<R> make_scene_const(const SingUniforms& u, 
         s.ph1 = R(double(next() >> 11) * (1.0 / 9007199254740992.0) * twopi);
         s.ph2 = R(double(next() >> 11) * (1.0 / 9007199254740992.0) * twopi);
     }
-    s.band = o.band_width > 0.0f ? R(o.band_width) : R(0.4);
+    s.band = o.band_width > 0.0f ? R(o.band_width) : R(0.03);
     s.tol = o.adapt_tol > 0.0f ? R(o.adapt_tol) : R(0);
     s.width = u.width;
     s.height = u.height;

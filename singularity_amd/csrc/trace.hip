@@ -86,13 +86,13 @@ __global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneC
                         bool started;
                         bool deferred = false;
                         if (F64) {
-                            started = init_ray<double, S, KERR>(y, sd, gx, gy, double(ox), double(oy), (double*)nullptr);
+                            started = init_ray<double, S, KERR>(y, sd, gx, gy, double(ox), double(oy), (bool*)nullptr);
                         } else if (PASS == 1) {
-                            float margin = 1e30f;
-                            started = init_ray<float, S, KERR>(y, sf, gx, gy, ox, oy, &margin);
-                            deferred = started && margin < sf.band;
+                            bool hard = false;
+                            started = init_ray<float, S, KERR>(y, sf, gx, gy, ox, oy, &hard);
+                            deferred = started && hard;
                         } else {
-                            started = init_ray<float, S, KERR>(y, sf, gx, gy, ox, oy, (float*)nullptr);
+                            started = init_ray<float, S, KERR>(y, sf, gx, gy, ox, oy, (bool*)nullptr);
                         }
                         need = false;
                         if (deferred) {
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneC
         // (3) emit finished rays; in the mixed pass, hand ambiguous outcomes to FP64
         if (term != kTermLive) {
             bool defer = false;
-            if (PASS == 1) defer = float(y.near_eq) < kGrazeTheta;
+            if (PASS == 1) defer = needs_fp64(y, term);
             if (defer) {
                 const std::uint32_t slot = atomicAdd(&ctl->band_count, 1u);
                 ch.band[slot] = rec_idx;

@@ -29,11 +29,15 @@ template <class S> struct RayState {
 };
 
 // Mixed-precision deferral thresholds.  Comparing FP32 and FP64 traces of the
-// same rays, outcomes differ only for rays near a discontinuity of the
-// discrete algorithm: a state within kGrazeTheta of the equator (the crossing
-// test flips), a crossing within kEdgeRel of an annulus edge, or an escape
-// within kStarCell cells of a starfield boundary next to a star.  Such rays
-// are re-traced in FP64.
+// same rays (57 600 rays of 64 varied Kerr scenes and the a = 0.94 4K scene),
+// outcomes or radiance differ (> 1e-3) for rays near a discontinuity of the
+// discrete algorithm — a state within kGrazeTheta of the equator (the crossing
+// test flips), a crossing within kEdgeRel of an annulus edge, an escape within
+// kStarCell cells of a starfield boundary next to a star (tested during and
+// after the FP32 march) — and for rays whose trajectory error is amplified:
+// close to a photon orbit, reaching 2 r+, or reaching 0.03 rad of the polar
+// axis (decided before the march from the ray's constants, hard_ray in
+// physics.hpp).  Such rays are traced in FP64.
 constexpr float kGrazeTheta = 2e-6f;
 constexpr float kEdgeRel = 1e-5f;
 constexpr float kStarCell = 1e-3f;
@@ -148,12 +152,21 @@ __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneCons
     return kTermLive;
 }
 
+// Mixed precision, after the FP32 march: does the outcome need FP64?  (A
+// grazing crossing or an annulus-edge crossing; the trajectory-amplification
+// cases are caught before the march by hard_ray.)
+template <class S>
+__device__ __forceinline__ bool needs_fp64(const RayState<S>& y, std::uint32_t) {
+    return float(y.near_eq) < kGrazeTheta;
+}
+
 // Ray initialisation from the camera (geodesic_kernel.cu:229-242, 303-347).
 // Returns false on a Kerr initialisation failure (no real root at the camera).
-// With margin_out, also evaluates the photon-orbit margin of the ray.
+// With hard_out, also decides from the ray's constants whether FP32 is not
+// accurate enough for it (hard_ray).
 template <class R, class S, bool KERR>
 __device__ __forceinline__ bool init_ray(RayState<S>& y, const SceneConst<R>& sc, std::uint32_t gx, std::uint32_t gy,
-                                         R ox, R oy, R* margin_out) {
+                                         R ox, R oy, bool* hard_out) {
     R ur, uth, uph;
     camera_ray(sc, gx, gy, ox, oy, ur, uth, uph);
     y.r = S(sc.cam_r);
@@ -178,12 +191,11 @@ __device__ __forceinline__ bool init_ray(RayState<S>& y, const SceneConst<R>& sc
         y.v1 = S(pt);
         y.v2 = S(E);
         y.v3 = S(L);
-        if (margin_out) {
+        if (hard_out) {
             const R Q = pt * pt + sc.q_cos2 * (L * L * sc.q_inv_s2 - sc.a2 * E * E);
             const R rp = sc.M + dsqrt(dmax(sc.M * sc.M - sc.a2, R(0)));
-            const R m = photon_margin<R>(L / E, Q / (E * E), sc.a, sc.M, rp);
-            // no turning point: ingoing rays are captured (defer), outgoing escape directly
-            *margin_out = (m < R(0)) ? (ur < R(0) ? R(-1) : R(1e30)) : m;
+            const R iE = R(1) / E;
+            *hard_out = hard_ray<R>(L * iE, Q * iE * iE, sc.a, sc.M, rp, ur < R(0), sc.band);
         }
     } else {
         const R cr = sc.cam_r;
@@ -194,11 +206,13 @@ __device__ __forceinline__ bool init_ray(RayState<S>& y, const SceneConst<R>& sc
         y.v2 = S(uth);
         y.v3 = S(uph);
         y.prev_uph = y.v3;
-        if (margin_out) {
+        if (hard_out) {
+            // Schwarzschild: xi = L_z/E, eta = (L^2 - L_z^2)/E^2 (Carter's constant at a = 0)
             const R E = sc.f_cam * ut;
             const R L2 = cr * cr * cr * cr * (uth * uth + sc.sin_th * sc.sin_th * uph * uph);
-            const R m = photon_margin<R>(R(0), L2 / (E * E), R(0), sc.M, sc.twoM);
-            *margin_out = (m < R(0)) ? (ur < R(0) ? R(-1) : R(1e30)) : m;
+            const R Lz = cr * cr * sc.sin_th * sc.sin_th * uph;
+            const R iE = R(1) / E;
+            *hard_out = hard_ray<R>(Lz * iE, (L2 - Lz * Lz) * iE * iE, R(0), sc.M, sc.twoM, ur < R(0), sc.band);
         }
     }
     return true;

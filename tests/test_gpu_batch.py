@@ -5,7 +5,9 @@ order as the single-scene pipeline; the two kernels are compiled separately,
 so the compiler's multiply-add contraction can differ and a value may round
 differently in the last bit.  Bars against single-scene renders (tolerances
 stated here): class identical on >= 99.95 % of pixels, RGBA8 within 1 LSB on
->= 99.9 %, HDR within 1e-5 relative on >= 99.9 %, ray totals exact.  Batches
+>= 99.9 %, HDR within 1e-5 relative on >= 99.9 % (1e-3 with error-controlled
+steps, whose accept/reject decisions turn last-bit differences into
+differences of the order of the tolerance), ray totals exact.  Batches
 against batches (launch shapes, record groups, graph replays, autotuning) run
 identical code and must be bit-identical.  Scenes cover both metrics, uneven
 sizes (1x1 to 300x170), 1..33 samples, error-controlled steps and seeded
@@ -58,7 +60,7 @@ def check_equal(frames, others):
         np.testing.assert_array_equal(f.cls, s.cls, err_msg=f"scene {i} class")
 
 
-def check_close(frames, singles):
+def check_close(frames, singles, hdr_tol=1e-5):
     cls = np.concatenate([(f.cls == s.cls).ravel() for f, s in zip(frames, singles)])
     lsb = np.concatenate([np.abs(f.rgba.astype(int) - s.rgba.astype(int)).max(-1).ravel()
                           for f, s in zip(frames, singles)])
@@ -66,10 +68,10 @@ def check_close(frames, singles):
                            np.maximum(np.abs(s.hdr.astype(np.float64)).max(-1), 1e-6)).ravel()
                           for f, s in zip(frames, singles)])
     print(f"batch vs single: class {cls.mean():.6f}, rgba<=1 {np.mean(lsb <= 1):.6f}, exact {np.mean(lsb == 0):.6f}, "
-          f"hdr<=1e-5 {np.mean(rel <= 1e-5):.6f}", flush=True)
+          f"hdr<={hdr_tol:g} {np.mean(rel <= hdr_tol):.6f}", flush=True)
     assert cls.mean() >= 0.9995
     assert np.mean(lsb <= 1) >= 0.999
-    assert np.mean(rel <= 1e-5) >= 0.999
+    assert np.mean(rel <= hdr_tol) >= 0.999
 
 
 @pytest.mark.parametrize("prec", [sa.PREC_FP32, sa.PREC_MIXED, sa.PREC_FP64])
@@ -94,7 +96,7 @@ def test_batch_adaptive_and_seeded_equals_single(be):
     frames, st = sa.render_batch(be, scenes, opts, hdr=True, cls=True)
     singles = [single(be, u, sa.PREC_MIXED, adapt_tol=1e-6 if u.metric_type == sa.METRIC_KERR else 0.0,
                       turb_seed=77) for u in scenes]
-    check_close(frames, singles)
+    check_close(frames, singles, hdr_tol=1e-3)
     n_rej = sum(s.stats.rejected for s in singles)
     assert n_rej > 0 and abs(st.rejected - n_rej) <= 1e-2 * n_rej
 

@@ -6,6 +6,7 @@ source tree:
 
     singularity_amd/libsing_hip.so   the C-ABI library (include/sing_abi.h)
     singularity_amd/hip_cli          flag-compatible replacement of cuda_cli
+    singularity_amd/hbm_copy         HBM counter calibration for profiling (256 MiB copy)
 
 Usage: python -m singularity_amd.build  (or import and call build()).
 """
@@ -25,6 +26,7 @@ CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libsing_hip.so"
 CLI = PKG / "hip_cli"
+COPY = PKG / "hbm_copy"
 
 ARCH = "gfx950"
 HIP_FLAGS = [
@@ -96,6 +98,11 @@ def build(verbose: bool = False) -> Path:
             ]
         )
         os.replace(tmpc, CLI)
+    copy_src = CSRC / "hbm_copy.hip"
+    if copy_src.exists():
+        tmpk = COPY.with_suffix(".tmp")
+        _run([hipcc, *HIP_FLAGS, "-o", str(tmpk), str(copy_src)])
+        os.replace(tmpk, COPY)
     return LIB
 
 

@@ -25,7 +25,8 @@ PEAK_HBM_GBS = 8000.0
 #   extract 8 read + 2 write, blur_h 2 + 2, blur_v 2 + 2, composite 8 + 2 read + 4 write
 POST_BYTES_PER_PIXEL = 32
 
-# Measured HBM traffic of one FP32 trace dispatch (rocprofv3 FETCH_SIZE x2.0 +
-# WRITE_SIZE x1.0, calibrated on a plain copy kernel), per traced ray:
-# 16-byte termination record + deferred-ray indices.
-TRACE_HBM_BYTES_PER_RAY = (575061.8 * 1024 + 93.8 * 1024 * 2.0) / 33177600.0
+# Measured HBM traffic of one mixed-precision FP32 trace dispatch at 1920x1080 x
+# 16 spp (33 177 600 rays): rocprofv3 FETCH_SIZE 104.5 KiB x 2.0 + WRITE_SIZE
+# 607141.4 KiB x 1.0 (factors from the 256 MiB copy of singularity_amd/hbm_copy),
+# per traced ray: 16-byte termination record, deferral markers and list entries.
+TRACE_HBM_BYTES_PER_RAY = (607141.4 * 1024 + 104.5 * 1024 * 2.0) / 33177600.0

@@ -70,8 +70,31 @@ template <class R> __device__ __forceinline__ KY<R> load_state(const double* p, 
     return {R(p[0]), R(p[S]), R(p[2 * S]), R(p[3 * S]), R(p[4 * S])};
 }
 
-// MODE 0: FP32 over all pixels; 1: FP32 with FP64 deferral; 2: FP64 over all
-// pixels; 3: FP64 over the deferred list.
+// Mixed precision, first: rays that hard_ray flags go to the FP64 list (and
+// meta[3] = FP64) before any march, so the FP32 and FP64 forwards can run
+// concurrently.
+__global__ __launch_bounds__(256) void diff_classify_kernel(const DiffSet* __restrict__ ds, DiffBuffers b) {
+    const std::uint32_t pix = blockIdx.x * 256u + threadIdx.x;
+    if (pix >= b.P) return;
+    const DiffScene<float>& sc = ds->sf;
+    KY<float> y;
+    float E, L;
+    const bool ok = diff_init(sc, pix % sc.width, pix / sc.width, y, E, L);
+    bool hard = false;
+    if (ok) {
+        float st, ct;
+        dsincos(sc.cam_theta, &st, &ct);
+        const float Q = y.pt * y.pt + ct * ct * (L * L / fmaxf(st * st, 1e-6f) - sc.a * sc.a * E * E);
+        const float rp = sc.M + sqrtf(fmaxf(sc.M * sc.M - sc.a * sc.a, 0.0f));
+        hard = hard_ray<float>(L / E, Q / (E * E), sc.a, sc.M, rp, y.pr < 0.0f, kBandDiff);
+    }
+    b.meta[4u * pix + 3u] = hard ? kMetaPrec64 : 0u;
+    if (hard) b.list[atomicAdd(b.count, 1u)] = pix;
+}
+
+// MODE 0: FP32 over all pixels; 1: FP32 over the pixels the classification
+// left in FP32, handing grazing rays to the late list (list2); 2: FP64 over all
+// pixels; 3: FP64 over a list (b.list / b.count).
 template <class R, int MODE>
 __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict__ ds, DiffBuffers b) {
     const std::uint32_t idx = blockIdx.x * 256u + threadIdx.x;
@@ -87,27 +110,13 @@ __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict
     const std::uint32_t gx = pix % sc.width, gy = pix / sc.width;
     const std::size_t S = b.P;
     std::uint32_t* meta = b.meta + 4u * pix;
+    if (MODE == 1 && meta[3] == kMetaPrec64) return;  // classified hard: the FP64 forward owns it
     double* cp = b.cp + pix;
     double* cr = b.cross + pix;
     KY<R> y;
     R E, L;
     const bool ok = diff_init(sc, gx, gy, y, E, L);
-    meta[3] = (MODE >= 2) ? kMetaPrec64 : 0u;
-    if (ok && MODE == 1) {
-        // hard for FP32 (as in the render path): decided from the ray's constants
-        float st, ct;
-        dsincos(float(sc.cam_theta), &st, &ct);
-        const float Ef = float(E), Lf = float(L), ptf = float(y.pt);
-        const float Q = ptf * ptf + ct * ct * (Lf * Lf / fmaxf(st * st, 1e-6f) - float(sc.a * sc.a) * Ef * Ef);
-        const float Mf = float(sc.M), af = float(sc.a);
-        const float rp = Mf + sqrtf(fmaxf(Mf * Mf - af * af, 0.0f));
-        const bool defer = hard_ray<float>(Lf / Ef, Q / (Ef * Ef), af, Mf, rp, y.pr < R(0), kBandDiff);
-        if (defer) {
-            meta[3] = kMetaPrec64;
-            b.list[atomicAdd(b.count, 1u)] = pix;
-            return;
-        }
-    }
+    if (MODE != 1) meta[3] = (MODE >= 2) ? kMetaPrec64 : 0u;
     R acc[3] = {R(0), R(0), R(0)};
     R tau = R(1);
     std::uint32_t ncross = 0, step = 0, term = kTermInitFail;
@@ -179,7 +188,7 @@ __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict
         const R rp = sc.M + hsqrt(hmax(sc.M * sc.M - sc.a * sc.a, R(0)));
         if (MODE == 1 && (float(near_eq) < kGrazeThetaDiff || (ncross > 0u && rmin < R(2) * rp))) {
             meta[3] = kMetaPrec64;
-            b.list[atomicAdd(b.count, 1u)] = pix;
+            b.list2[atomicAdd(b.count2, 1u)] = pix;
             return;
         }
         if (term == kTermEscape && (sc.flags & SING_FLAG_STARFIELD_ON)) {
@@ -382,25 +391,42 @@ __global__ __launch_bounds__(1024) void diff_reduce_kernel(const double* __restr
 }  // namespace
 
 hipError_t launch_diff(const DiffSet* ds_dev, const DiffBuffers& b, std::uint32_t precision, bool backward,
-                       double* out6, hipStream_t stream, hipEvent_t ev_mid) {
+                       double* out6, hipStream_t stream, hipStream_t stream2, const hipEvent_t* ev) {
+    // ev[0]: end of the forward (recorded on `stream`); ev[1..3]: cross-stream joins
     const dim3 block(256), grid((b.P + 255u) / 256u);
     if (precision == SING_PREC_FP32) {
         hipLaunchKernelGGL((diff_fwd_kernel<float, 0>), grid, block, 0, stream, ds_dev, b);
+        (void)hipEventRecord(ev[0], stream);
+        if (backward) hipLaunchKernelGGL((diff_bwd_kernel<float, 0>), grid, block, 0, stream, ds_dev, b);
     } else if (precision == SING_PREC_FP64) {
         hipLaunchKernelGGL((diff_fwd_kernel<double, 2>), grid, block, 0, stream, ds_dev, b);
+        (void)hipEventRecord(ev[0], stream);
+        if (backward) hipLaunchKernelGGL((diff_bwd_kernel<double, 0>), grid, block, 0, stream, ds_dev, b);
     } else {
+        // Mixed: the hard rays (classified first) run in FP64 on stream2 while the
+        // rest run in FP32 on stream; grazing rays found by the FP32 march follow
+        // in FP64 (late list).  Each launch is latency-bound at these sizes (one
+        // ray per thread), so running the two precisions side by side hides the
+        // FP64 rays' long tail instead of adding it.
+        DiffBuffers late = b;
+        late.list = b.list2;
+        late.count = b.count2;
+        hipLaunchKernelGGL(diff_classify_kernel, grid, block, 0, stream, ds_dev, b);
+        (void)hipEventRecord(ev[1], stream);
+        (void)hipStreamWaitEvent(stream2, ev[1], 0);
+        hipLaunchKernelGGL((diff_fwd_kernel<double, 3>), grid, block, 0, stream2, ds_dev, b);
+        (void)hipEventRecord(ev[2], stream2);
         hipLaunchKernelGGL((diff_fwd_kernel<float, 1>), grid, block, 0, stream, ds_dev, b);
-        hipLaunchKernelGGL((diff_fwd_kernel<double, 3>), grid, block, 0, stream, ds_dev, b);
-    }
-    if (ev_mid) (void)hipEventRecord(ev_mid, stream);
-    if (backward) {
-        if (precision == SING_PREC_FP32) {
+        hipLaunchKernelGGL((diff_fwd_kernel<double, 3>), grid, block, 0, stream, ds_dev, late);
+        (void)hipStreamWaitEvent(stream, ev[2], 0);
+        (void)hipEventRecord(ev[0], stream);
+        if (backward) {
+            (void)hipStreamWaitEvent(stream2, ev[0], 0);
+            hipLaunchKernelGGL((diff_bwd_kernel<double, 1>), grid, block, 0, stream2, ds_dev, b);
+            hipLaunchKernelGGL((diff_bwd_kernel<double, 1>), grid, block, 0, stream2, ds_dev, late);
+            (void)hipEventRecord(ev[3], stream2);
             hipLaunchKernelGGL((diff_bwd_kernel<float, 0>), grid, block, 0, stream, ds_dev, b);
-        } else if (precision == SING_PREC_FP64) {
-            hipLaunchKernelGGL((diff_bwd_kernel<double, 0>), grid, block, 0, stream, ds_dev, b);
-        } else {
-            hipLaunchKernelGGL((diff_bwd_kernel<float, 0>), grid, block, 0, stream, ds_dev, b);
-            hipLaunchKernelGGL((diff_bwd_kernel<double, 1>), grid, block, 0, stream, ds_dev, b);
+            (void)hipStreamWaitEvent(stream, ev[3], 0);
         }
     }
     hipLaunchKernelGGL(diff_reduce_kernel, dim3(1), dim3(1024), 0, stream, b.loss, b.grad, b.P, out6);
@@ -424,6 +450,8 @@ void SingContext::free_diff() {
     (void)hipFree(diff.scratch);
     (void)hipFree(diff.list);
     (void)hipFree(diff.count);
+    (void)hipFree(diff.list2);
+    (void)hipFree(diff.count2);
     (void)hipFree(diff.steps);
     (void)hipFree(d_target);
     (void)hipFree(d_weight);
@@ -455,6 +483,8 @@ int SingContext::ensure_diff(std::size_t P, std::size_t ncp, std::size_t C) {
     A((void**)&diff.scratch, P * (C + 1) * 5 * sizeof(double));
     A((void**)&diff.list, P * sizeof(std::uint32_t));
     A((void**)&diff.count, sizeof(std::uint32_t));
+    A((void**)&diff.list2, P * sizeof(std::uint32_t));
+    A((void**)&diff.count2, sizeof(std::uint32_t));
     A((void**)&diff.steps, P * sizeof(std::uint32_t));
     A((void**)&d_target, P * 3 * sizeof(float));
     A((void**)&d_weight, P * sizeof(float));
@@ -499,7 +529,7 @@ int diff_run(SingContext* c, const SingUniforms* base, const SingDiffParams* p, 
     if (e != hipSuccess) return c->hip_fail(e, "hipSetDevice");
     int rc = c->ensure_diff(P, ncp, C);
     if (rc != SING_OK) return rc;
-    if ((rc = c->ensure_events(4)) != SING_OK) return rc;
+    if ((rc = c->ensure_events(6)) != SING_OK) return rc;
 
     // r_in: given, or the prograde ISCO of the spin with its derivative
     double r_in = p->r_in, drin_da = 0.0;
@@ -564,6 +594,8 @@ int diff_run(SingContext* c, const SingUniforms* base, const SingDiffParams* p, 
     if (weights &&
         (e = hipMemcpyAsync(c->d_weight, weights, P * sizeof(float), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
         return c->hip_fail(e, "hipMemcpyAsync(weights)");
+    if ((e = hipMemsetAsync(c->diff.count2, 0, sizeof(std::uint32_t), c->stream)) != hipSuccess)
+        return c->hip_fail(e, "hipMemsetAsync(count2)");
     if ((e = hipMemsetAsync(c->diff.count, 0, sizeof(std::uint32_t), c->stream)) != hipSuccess)
         return c->hip_fail(e, "hipMemsetAsync(count)");
     if (!backward && (e = hipMemsetAsync(c->diff.grad, 0, P * 5 * sizeof(double), c->stream)) != hipSuccess)
@@ -572,10 +604,11 @@ int diff_run(SingContext* c, const SingUniforms* base, const SingDiffParams* p, 
     b.target = c->d_target;
     b.weight = weights ? c->d_weight : nullptr;
     (void)hipEventRecord(c->events[0], c->stream);
-    if ((e = sing::launch_diff(c->d_dset, b, o.precision, backward, c->d_out6, c->stream, c->events[1])) !=
+    if ((e = sing::launch_diff(c->d_dset, b, o.precision, backward, c->d_out6, c->stream, c->stream2,
+                               &c->events[1])) !=
         hipSuccess)
         return c->hip_fail(e, "differentiable render launch");
-    (void)hipEventRecord(c->events[2], c->stream);
+    (void)hipEventRecord(c->events[5], c->stream);
     double out6[6] = {0, 0, 0, 0, 0, 0};
     std::uint32_t n64 = 0;
     if ((e = hipMemcpyAsync(out6, c->d_out6, sizeof(out6), hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
@@ -602,7 +635,7 @@ int diff_run(SingContext* c, const SingUniforms* base, const SingDiffParams* p, 
         std::memset(stats, 0, sizeof(*stats));
         float a = 0.0f, bms = 0.0f;
         (void)hipEventElapsedTime(&a, c->events[0], c->events[1]);
-        (void)hipEventElapsedTime(&bms, c->events[1], c->events[2]);
+        (void)hipEventElapsedTime(&bms, c->events[1], c->events[5]);
         stats->forward_ms = a;
         stats->backward_ms = bms;
         stats->rays = P;

@@ -34,13 +34,15 @@ struct DiffBuffers {
     double* scratch;        // [C+1][5][P] recomputed segment states
     const float* target;    // [P][3]
     const float* weight;    // [P] or null
-    std::uint32_t* list;    // pixels of the FP64 pass
+    std::uint32_t* list;    // pixels of the FP64 pass (classified hard before the march)
     std::uint32_t* count;   // length of list
+    std::uint32_t* list2;   // pixels handed to FP64 after the FP32 march (grazing, near-hole emission)
+    std::uint32_t* count2;  // length of list2
     std::uint32_t* steps;   // [P] RK4 steps of the forward
     std::uint32_t P, ncp, C;
 };
 
 hipError_t launch_diff(const DiffSet* ds_dev, const DiffBuffers& b, std::uint32_t precision, bool backward,
-                       double* out6, hipStream_t stream, hipEvent_t ev_mid);
+                       double* out6, hipStream_t stream, hipStream_t stream2, const hipEvent_t* ev);
 
 }  // namespace sing

@@ -165,6 +165,7 @@ SingContext* sing_create(uint32_t width, uint32_t height, int device) {
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->d_ctl, sizeof(sing::QueueCtl));
     if (e == hipSuccess) e = hipMalloc(&c->d_halton, 1024 * sizeof(float2));
     if (e != hipSuccess) {
@@ -197,6 +198,10 @@ void sing_destroy(SingContext* c) {
     c->free_diff();
     c->free_batch();
     for (auto& e : c->events) hipEventDestroy(e);
+    if (c->stream2) {
+        hipStreamSynchronize(c->stream2);
+        hipStreamDestroy(c->stream2);
+    }
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }

@@ -21,6 +21,7 @@ struct SingContext {
     int num_cu = 256;
     std::uint32_t width = 0, height = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // second stream (default priority) for concurrent passes
     std::vector<hipEvent_t> events;
     // frame outputs
     uchar4* d_rgba = nullptr;

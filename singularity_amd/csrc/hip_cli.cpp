@@ -140,6 +140,7 @@ void print_help() {
               "  --escape-radius M    Escape radius in M (default 200)\n"
               "  --peak-temp-scale S  Multiplies the disc peak temperature (default 1)\n"
               "  --turb-seed N        Seeded turbulence band phases (0 = reference bands)\n"
+              "  --adapt-tol T        Error-controlled RK4 with per-step tolerance T (Kerr scenes)\n"
               "  --stats              Print per-frame timing and step counts.\n");
 }
 
@@ -221,6 +222,7 @@ int main(int argc, char** argv) {
         else if (a == "--escape-radius" && has) escape_r = std::strtof(argv[++i], nullptr);
         else if (a == "--peak-temp-scale" && has) temp_scale = std::strtof(argv[++i], nullptr);
         else if (a == "--turb-seed" && has) turb_seed = std::uint32_t(std::strtoul(argv[++i], nullptr, 10));
+        else if (a == "--adapt-tol" && has) opts.adapt_tol = std::strtof(argv[++i], nullptr);
         else {
             std::fprintf(stderr, "hip_cli: unknown argument: %.*s\n", int(a.size()), a.data());
             print_help();

@@ -30,3 +30,30 @@ def test_goldens_distinguish_scenes(orc):
     imgs = {n: orc.render(golden_scene(n), 32)["rgba"].reshape(256, 256, 4) for n in NAMES}
     assert phash_distance(imgs["schw.png"], load_png(GOLDEN / "cuda" / "kerr_a094.png")) > 4
     assert np.any(imgs["kerr_a05.png"] != imgs["kerr_a094.png"])
+
+
+# ---- the reference's Vulkan goldens (verification/golden/vulkan/*.png, copied as
+# data to tests/golden/vulkan/): the realtime pipeline (bloom + ACES blit) at
+# camera (30 M, 0.15 rad), 256x256, 1 spp; the reference gates the offline CUDA
+# renderer against them at pHash <= 20 (verification/test_backend_equivalence.py:207-256).
+def vulkan_scene(name):
+    import singularity_amd as sa
+
+    if name == "schw.png":  # Schwarzschild, Scene defaults otherwise (README of the golden set)
+        s = sa.scene_defaults()
+        s.metric = sa.METRIC_SCHWARZSCHILD
+        s.spin_a_over_M = 0.0
+    else:
+        s = sa.scene_from_config(sa.load_scene_config(GOLDEN / name.replace(".png", ".scene")))
+    return sa.pack_uniforms(s, sa.orbital_camera(30.0, 0.15, 0.0, 60.0), 256, 256, 0.0)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_render_matches_vulkan_golden_structure(orc, name):
+    img = orc.render(vulkan_scene(name), 32)["rgba"].reshape(256, 256, 4)
+    assert phash_distance(img, load_png(GOLDEN / "vulkan" / name)) <= 20
+
+
+def test_vulkan_goldens_distinguish_metric(orc):
+    img = orc.render(vulkan_scene("schw.png"), 32)["rgba"].reshape(256, 256, 4)
+    assert phash_distance(img, load_png(GOLDEN / "vulkan" / "kerr_a094.png")) > 20

@@ -90,9 +90,14 @@ def test_mixed_matches_fp64_oracle_outside_band(backends, orc, name):
 
 
 def test_golden_structure_from_gpu(backends):
+    from test_golden_images import vulkan_scene
+
     for name in ("schw.png", "kerr_a05.png", "kerr_a094.png"):
         f = render(backends, golden_scene(name), outputs=sa.OUT_RGBA8)
         assert phash_distance(f.rgba, load_png(GOLDEN / "cuda" / name)) <= 4
+        # the reference's gate of the offline renderer against its realtime (Vulkan) goldens
+        f = render(backends, vulkan_scene(name), outputs=sa.OUT_RGBA8)
+        assert phash_distance(f.rgba, load_png(GOLDEN / "vulkan" / name)) <= 20
 
 
 def test_frames_are_bit_reproducible(backends):

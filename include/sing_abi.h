@@ -105,6 +105,12 @@ typedef struct SingUniforms {
 #define SING_POST_NONE 0u
 #define SING_POST_BLOOM 1u /* half HDR -> extract -> 7-tap H/V -> composite+exposure -> ACES -> sRGB into RGBA8 */
 
+/* --- sub-pixel sampler ----------------------------------------------------- */
+#define SING_SAMPLER_HALTON 0u /* supersample Halton(2,3) offsets, mean of per-sample ACES
+                                  (render/cuda/kernels/geodesic_kernel.cu:482-502) */
+#define SING_SAMPLER_GRID 1u   /* ss x ss grid, ss = clamp(supersample, 1, 4), ACES of the mean radiance
+                                  (render/metal/shaders/geodesic_kernel.metal:416-433, blit.metal:54-68) */
+
 /* Options beyond the 160-byte block.  Zero-initialised = reference semantics,
  * FP32, RGBA8 only. */
 typedef struct SingRenderOpts {
@@ -117,7 +123,8 @@ typedef struct SingRenderOpts {
     float peak_T_scale;   /* multiplies disc_peak_T; 0 is read as 1 */
     uint32_t step_batch;  /* RK4 steps between lane-refill checks (0 = tuned default) */
     uint32_t waves_per_cu;/* persistent grid size in waves per CU (0 = tuned default) */
-    uint32_t reserved[7];
+    uint32_t sampler;     /* SING_SAMPLER_* */
+    uint32_t reserved[6];
 } SingRenderOpts;
 
 typedef struct SingRenderStats {

@@ -27,7 +27,7 @@ _LIBS: dict[str, C.CDLL] = {}
 class Extras(C.Structure):
     _fields_ = [("soft_edge", C.c_double), ("band_phase1", C.c_double), ("band_phase2", C.c_double),
                 ("adapt_tol", C.c_double), ("peak_T_scale", C.c_double), ("jitter_x", C.c_double),
-                ("jitter_y", C.c_double)]
+                ("jitter_y", C.c_double), ("grid_sampler", C.c_double)]
 
 
 def build() -> None:

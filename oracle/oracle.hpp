@@ -243,6 +243,7 @@ template <class T> struct Extras {
     T peak_T_scale = T(1.0);
     T jitter_x = T(0.0);    // added to every sample offset (boundary-band probes)
     T jitter_y = T(0.0);
+    bool grid = false;      // realtime sampler: ss x ss grid, ss = clamp(spp, 1, 4), tone map of the mean
 };
 
 template <class T> struct Uni {
@@ -1008,11 +1009,19 @@ inline PixelOut<T> render_pixel(const Uni<T>& u, const CamFrame<T>& f, std::uint
     PixelOut<T> p{};
     std::uint32_t spp = u.supersample;
     spp = spp < 1u ? 1u : (spp > 1024u ? 1024u : spp);
+    // realtime backends' sampler (render/metal/shaders/geodesic_kernel.metal:416-434):
+    // ss x ss grid at ((sx + 0.5) / ss, (sy + 0.5) / ss), mean radiance, then tone map
+    const std::uint32_t ss = spp < 4u ? spp : 4u;
+    if (u.ext.grid) spp = ss * ss;
+    const float sub_w = 1.0f / float(ss);
     V3<T> ldr = v3(T(0.0), T(0.0), T(0.0));
     V3<T> hdr = v3(T(0.0), T(0.0), T(0.0));
     for (std::uint32_t s = 0; s < spp; ++s) {
         T ox = T(0.5), oy = T(0.5);
-        if (spp > 1u) {
+        if (u.ext.grid) {
+            ox = T(double((float(s % ss) + 0.5f) * sub_w));
+            oy = T(double((float(s / ss) + 0.5f) * sub_w));
+        } else if (spp > 1u) {
             ox = T(double(halton_xy[2 * s]));
             oy = T(double(halton_xy[2 * s + 1]));
         }
@@ -1035,6 +1044,7 @@ inline PixelOut<T> render_pixel(const Uni<T>& u, const CamFrame<T>& f, std::uint
         ldr = ldr * inv;
         hdr = hdr * inv;
     }
+    if (u.ext.grid) ldr = aces(hdr * u.exposure);
     p.hdr[0] = val(hdr.x); p.hdr[1] = val(hdr.y); p.hdr[2] = val(hdr.z);
     p.ldr[0] = val(ldr.x); p.ldr[1] = val(ldr.y); p.ldr[2] = val(ldr.z);
     p.rgba[0] = to_u8(ldr.x);

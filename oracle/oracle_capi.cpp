@@ -18,6 +18,7 @@ struct OrcExtras {  // mirrors orc::Extras in double; all zero = reference seman
     double peak_T_scale;  // 0 is read as 1
     double jitter_x;      // added to every sample offset
     double jitter_y;
+    double grid_sampler;  // non-zero: the realtime backends' grid sampler
 };
 
 template <class T> Uni<T> load_uni(const void* raw, const OrcExtras* ex) {
@@ -32,6 +33,7 @@ template <class T> Uni<T> load_uni(const void* raw, const OrcExtras* ex) {
         o.ext.peak_T_scale = T(ex->peak_T_scale == 0.0 ? 1.0 : ex->peak_T_scale);
         o.ext.jitter_x = T(ex->jitter_x);
         o.ext.jitter_y = T(ex->jitter_y);
+        o.ext.grid = ex->grid_sampler != 0.0;
     }
     return o;
 }

@@ -167,16 +167,23 @@ SingContext* sing_create(uint32_t width, uint32_t height, int device) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->d_ctl, sizeof(sing::QueueCtl));
-    if (e == hipSuccess) e = hipMalloc(&c->d_halton, 1024 * sizeof(float2));
+    if (e == hipSuccess) e = hipMalloc(&c->d_halton, (1024 + sing::kGridTableLen) * sizeof(float2));
     if (e != hipSuccess) {
         sing_set_global_error(std::string("sing_create: HIP setup failed: ") + hipGetErrorString(e));
         sing_destroy(c);
         return nullptr;
     }
-    // Halton(2,3) offsets of samples 1..1024 (geodesic_kernel.cu:491-495).
-    std::vector<float2> tab(1024);
+    // Halton(2,3) offsets of samples 1..1024 (geodesic_kernel.cu:491-495),
+    // followed by the grid sampler's tables for ss = 1..4.
+    std::vector<float2> tab(1024 + sing::kGridTableLen);
     for (std::uint32_t s = 0; s < 1024; ++s)
         tab[s] = make_float2(sing::halton_f(s + 1u, 2u), sing::halton_f(s + 1u, 3u));
+    for (std::uint32_t ss = 1; ss <= 4u; ++ss)
+        for (std::uint32_t s = 0; s < ss * ss; ++s) {
+            float ox, oy;
+            sing::grid_offset_f(s, ss, ox, oy);
+            tab[1024 + sing::grid_table_base(ss) + s] = make_float2(ox, oy);
+        }
     e = hipMemcpy(c->d_halton, tab.data(), tab.size() * sizeof(float2), hipMemcpyHostToDevice);
     if (e != hipSuccess || c->alloc_frame(width, height) != SING_OK || c->ensure_events(8) != SING_OK) {
         sing_set_global_error(e != hipSuccess ? std::string("sing_create: halton upload failed") : c->err);
@@ -226,6 +233,7 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
     if (opts_in) o = *opts_in;
     if (o.precision > SING_PREC_MIXED) return c->fail(SING_ERR_ARG, "sing_render: unknown precision %u", o.precision);
     if (o.post > SING_POST_BLOOM) return c->fail(SING_ERR_ARG, "sing_render: unknown post pass %u", o.post);
+    if (o.sampler > SING_SAMPLER_GRID) return c->fail(SING_ERR_ARG, "sing_render: unknown sampler %u", o.sampler);
     if (u->metric_type > SING_METRIC_KERR)
         return c->fail(SING_ERR_ARG, "sing_render: unknown metric %u", u->metric_type);
     if (!(o.adapt_tol >= 0.0f && o.adapt_tol <= 0.1f))
@@ -264,6 +272,12 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
     ch.tiles_x = (c->width + 7u) / 8u;
     ch.n_tiles = ch.tiles_x * ((c->height + 7u) / 8u);
     const sing::LaunchShape shape = c->launch_shape(o);
+    const float2* offsets = c->d_halton;
+    if (sf.grid) {
+        std::uint32_t ss = 1;
+        while (ss * ss < spp) ++ss;
+        offsets = c->d_halton + 1024 + sing::grid_table_base(ss);
+    }
 
     if ((e = hipMemsetAsync(c->d_ctl, 0, sizeof(sing::QueueCtl), c->stream)) != hipSuccess)
         return c->hip_fail(e, "hipMemsetAsync(queue)");
@@ -275,7 +289,7 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
         if (k > 0 && (e = hipMemsetAsync(c->d_ctl, 0, offsetof(sing::QueueCtl, steps), c->stream)) != hipSuccess)
             return c->hip_fail(e, "hipMemsetAsync(queue)");
         hipEventRecord(ev[4 * k + 0], c->stream);
-        if ((e = sing::launch_trace_chunk(sf, sd, c->d_halton, ch, c->d_ctl, o.precision, shape, c->stream)) !=
+        if ((e = sing::launch_trace_chunk(sf, sd, offsets, ch, c->d_ctl, o.precision, shape, c->stream)) !=
             hipSuccess)
             return c->hip_fail(e, "trace kernel launch");
         hipEventRecord(ev[4 * k + 1], c->stream);

@@ -141,6 +141,8 @@ void print_help() {
               "  --peak-temp-scale S  Multiplies the disc peak temperature (default 1)\n"
               "  --turb-seed N        Seeded turbulence band phases (0 = reference bands)\n"
               "  --adapt-tol T        Error-controlled RK4 with per-step tolerance T (Kerr scenes)\n"
+              "  --sampler S          halton (default: Halton offsets, mean of tone-mapped samples) |\n"
+              "                       grid (NxN grid, N = min(spp, 4), tone map of the mean radiance)\n"
               "  --stats              Print per-frame timing and step counts.\n");
 }
 
@@ -223,6 +225,15 @@ int main(int argc, char** argv) {
         else if (a == "--peak-temp-scale" && has) temp_scale = std::strtof(argv[++i], nullptr);
         else if (a == "--turb-seed" && has) turb_seed = std::uint32_t(std::strtoul(argv[++i], nullptr, 10));
         else if (a == "--adapt-tol" && has) opts.adapt_tol = std::strtof(argv[++i], nullptr);
+        else if (a == "--sampler" && has) {
+            const std::string_view p = argv[++i];
+            if (p == "halton") opts.sampler = SING_SAMPLER_HALTON;
+            else if (p == "grid") opts.sampler = SING_SAMPLER_GRID;
+            else {
+                std::fprintf(stderr, "hip_cli: --sampler must be halton or grid\n");
+                return 2;
+            }
+        }
         else {
             std::fprintf(stderr, "hip_cli: unknown argument: %.*s\n", int(a.size()), a.data());
             print_help();

@@ -42,6 +42,8 @@ template <class R> struct SceneConst {
     // error-controlled stepping (Kerr): per-step tolerance, 0 = the reference's fixed step
     R tol;
     std::uint32_t width, height, metric, flags, max_steps, spp;
+    // grid sampler: spp = ss * ss offsets from the grid table, ACES of the mean radiance
+    std::uint32_t grid;
 };
 
 // Same arithmetic as the reference kernel's per-sample camera and metric
@@ -133,7 +135,24 @@ template <class R> inline SceneConst<R> make_scene_const(const SingUniforms& u, 
     s.max_steps = u.max_steps;
     std::uint32_t spp = u.supersample;
     s.spp = spp < 1u ? 1u : (spp > 1024u ? 1024u : spp);
+    s.grid = o.sampler == SING_SAMPLER_GRID ? 1u : 0u;
+    if (s.grid) {
+        const std::uint32_t ss = s.spp < 4u ? s.spp : 4u;
+        s.spp = ss * ss;
+    }
     return s;
+}
+
+// Grid sampler offsets (render/metal/shaders/geodesic_kernel.metal:417-424):
+// sample sy * ss + sx sits at ((sx + 0.5) / ss, (sy + 0.5) / ss), in float.
+// The tables for ss = 1..4 are stored back to back; grid_table_base gives where
+// the table for one ss starts.
+constexpr std::uint32_t kGridTableLen = 1u + 4u + 9u + 16u;
+inline std::uint32_t grid_table_base(std::uint32_t ss) { return (ss - 1u) * ss * (2u * ss - 1u) / 6u; }
+inline void grid_offset_f(std::uint32_t s, std::uint32_t ss, float& ox, float& oy) {
+    const float sub_w = 1.0f / float(ss);
+    ox = (float(s % ss) + 0.5f) * sub_w;
+    oy = (float(s / ss) + 0.5f) * sub_w;
 }
 
 // Halton radical inverse in float (geodesic_kernel.cu:442-451); the kernels

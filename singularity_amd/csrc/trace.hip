@@ -225,6 +225,11 @@ __global__ __launch_bounds__(256) void shade_kernel(SceneConst<float> sc, const 
         l0 *= inv; l1 *= inv; l2 *= inv;
         h0 *= inv; h1 *= inv; h2 *= inv;
     }
+    if (sc.grid) {  // tone map of the mean radiance (render/metal/shaders/blit.metal:64-68)
+        l0 = aces1(h0 * ex);
+        l1 = aces1(h1 * ex);
+        l2 = aces1(h2 * ex);
+    }
     if (out.rgba) out.rgba[pix] = make_uchar4(to_u8(l0), to_u8(l1), to_u8(l2), 255);
     if (out.hdr) {
         out.hdr[3u * pix + 0u] = h0;

@@ -44,13 +44,13 @@ def read_pgm(path: Path) -> np.ndarray:
     return np.frombuffer(data[m.end():], np.uint8).reshape(h, w)
 
 
-def library_frame(scene_file, az=0.0, t=0.0, spp=1, prec=sa.PREC_FP32):
+def library_frame(scene_file, az=0.0, t=0.0, spp=1, prec=sa.PREC_FP32, **kw):
     s = sa.scene_from_config(sa.load_scene_config(scene_file))
     s.supersample = spp
     u = sa.pack_uniforms(s, sa.orbital_camera(30.0, 0.15, az, 60.0), W, H, t)
     be = sa.HipBackend(W, H)
     try:
-        return be.render_outputs(u, sa.OUT_RGBA8 | sa.OUT_HDR | sa.OUT_CLASS, prec)
+        return be.render_outputs(u, sa.OUT_RGBA8 | sa.OUT_HDR | sa.OUT_CLASS, prec, **kw)
     finally:
         be.shutdown()
 
@@ -66,6 +66,18 @@ def test_png_equals_library_frame(tmp_path):
     np.testing.assert_array_equal(load_png(out), f.rgba)
     np.testing.assert_array_equal(read_pfm(tmp_path / "a.pfm"), f.hdr)
     np.testing.assert_array_equal(read_pgm(tmp_path / "a.pgm"), f.cls)
+
+
+def test_grid_sampler_png_equals_library_frame(tmp_path):
+    scene = GOLDEN / "kerr_a094.scene"
+    out = tmp_path / "g.png"
+    r = run("--scene", scene, "--output", out, "--res", f"{W}x{H}", "--samples-per-pixel", 3, "--sampler", "grid",
+            "--hdr-output", tmp_path / "g.pfm")
+    assert r.returncode == 0, r.stderr
+    f = library_frame(scene, spp=3, sampler=sa.SAMPLER_GRID)
+    np.testing.assert_array_equal(load_png(out), f.rgba)
+    np.testing.assert_array_equal(read_pfm(tmp_path / "g.pfm"), f.hdr)
+    assert not np.array_equal(f.hdr, library_frame(scene, spp=3).hdr)
 
 
 def test_orbit_sequence(tmp_path):

@@ -25,7 +25,8 @@ def test_help_lists_the_reference_flags():
     assert r.returncode == 0
     for flag in ("--scene", "--output", "--res", "--samples-per-pixel", "--camera-distance", "--camera-elevation",
                  "--camera-azimuth", "--fov-y-deg", "--frames", "--output-pattern", "--encode-mp4", "--mp4-fps",
-                 "--precision", "--hdr-output", "--class-output", "--bloom", "--adapt-tol", "--turb-seed"):
+                 "--precision", "--hdr-output", "--class-output", "--bloom", "--adapt-tol", "--turb-seed",
+                 "--sampler"):
         assert flag in r.stdout, flag
 
 
@@ -37,6 +38,7 @@ def test_help_lists_the_reference_flags():
     (("--frames", "0"), "--frames must be in [1, 100000]"),
     (("--mp4-fps", "0"), "--mp4-fps must be in [1, 240]"),
     (("--precision", "fp16"), "--precision must be fp32, fp64 or mixed"),
+    (("--sampler", "stratified"), "--sampler must be halton or grid"),
     (("--bogus",), "unknown argument: --bogus"),
     (("--frames", "3"), "--frames > 1 requires --output-pattern"),
     (("--frames", "3", "--output-pattern", "f.png"), "must contain a printf placeholder"),

@@ -277,6 +277,26 @@ def adjoint_leg(evals: int = 50) -> dict:
     return res
 
 
+def fan_leg(n_rays: int = 1 << 20, max_steps: int = 5000) -> dict:
+    """The CPU driver's benchmark mode (cli/main.cpp:1492-1621) on the GPU: the
+    equatorial ray fan over b in [-10, 10] M with h = 0.1, one lane per ray,
+    at 2^20 rays (the driver's default is 100 rays on one CPU thread).  Reports
+    RK4 steps/s of the integration kernel (HIP events)."""
+    be = sa.HipBackend(1, 1)
+    res = {"workload": f"ray fan b in [-10, 10] M, {n_rays} rays, max {max_steps} steps, h = 0.1 M"}
+    for name, metric, spin in (("schw", sa.METRIC_SCHWARZSCHILD, 0.0), ("kerr_a09", sa.METRIC_KERR, 0.9)):
+        sa.benchmark_fan(be, metric, spin, 4096, 200)  # warm-up
+        best = None
+        for _ in range(3):
+            total, _, st = sa.benchmark_fan(be, metric, spin, n_rays, max_steps)
+            best = st.kernel_ms if best is None else min(best, st.kernel_ms)
+        res[name] = {"total_steps": total, "kernel_ms": round(best, 3),
+                     "steps_per_s": round(total / (best * 1e-3), 1)}
+        log(f"fan {name}: {res[name]}")
+    be.shutdown()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -363,6 +383,7 @@ def main() -> int:
         out["adaptive_bloom_1080p"] = c2_leg()
         out["kerr_a0998_4k"] = c3_leg()
         out["batch_64_scenes"] = c4_leg()
+        out["integrator_fan"] = fan_leg()
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg()
         out["grads_per_s"] = out["adjoint"]["mixed"]["grads_per_s"]

@@ -331,6 +331,60 @@ typedef struct SingBatchStats {
 int sing_render_batch(SingContext* ctx, const SingBatchItem* items, uint32_t n, const SingBatchOpts* opts,
                       SingBatchStats* stats);
 
+/* --- the CPU driver's diagnostics (cli/main.cpp:1269-1621) -------------------
+ * kerr-geometry, photon-orbit and benchmark of singularity_cli, in float like
+ * the reference.  The geometry is closed forms on the host (no device needed);
+ * the orbit and the ray fan are set up on the host exactly as the driver does
+ * and integrated on the device by the renderer's RK4 steppers. */
+
+/* cli/main.cpp:1415-1476 (core/include/physics/kerr.hpp:36-108). */
+typedef struct SingKerrGeometry {
+    float mass, spin, spin_over_mass;
+    float outer_horizon, inner_horizon;
+    float ergosphere_polar, ergosphere_equatorial;
+    float photon_sphere_prograde, photon_sphere_retrograde;
+    float isco_prograde, isco_retrograde;
+} SingKerrGeometry;
+/* SING_ERR_ARG (message in sing_last_error(NULL)) for mass <= 0 or spin
+ * outside [0, mass]. */
+int sing_kerr_geometry(float mass, float spin, SingKerrGeometry* out);
+
+/* cli/main.cpp:1279-1410: one equatorial circular null orbit at r_init for
+ * `orbits` periods with RK4 step h_step (Schwarzschild when spin = 0, else the
+ * prograde Kerr orbit with E = 1, L_z = (r sqrt(Delta) - 2 M a)/(r - 2M)).
+ * Rows are those of the driver's CSV: the start, every
+ * max(1, int(period / (200 h))) steps and the last step, or a row with
+ * step = -1 where the ray leaves (non-finite r or r < 1.001 r+).  When cap is
+ * below the rows the orbit can produce, returns SING_ERR_ARG with that bound in
+ * *n_rows.  Argument errors (the driver's exit-2 cases) give SING_ERR_ARG. */
+typedef struct SingOrbitRow {
+    int32_t step;
+    float t, r, theta, phi;
+} SingOrbitRow;
+int sing_photon_orbit(SingContext* ctx, float mass, float spin, float r_init, float orbits, float h_step,
+                      SingOrbitRow* rows, uint32_t cap, uint32_t* n_rows, double* kernel_ms);
+
+/* cli/main.cpp:1492-1590: the benchmark's equatorial ray fan (M = 1,
+ * h = 0.1): ray i has impact parameter b_min + (b_max - b_min)(i + 1/2)/n_rays
+ * and steps until max_steps, a non-finite r or phi, or r below the cut
+ * (1.01 r_s Schwarzschild, 1.02 r+ Kerr).  One device lane per ray. */
+typedef struct SingFanOpts {
+    uint32_t metric;      /* SING_METRIC_SCHWARZSCHILD or SING_METRIC_KERR */
+    float spin;           /* a/M, Kerr only: [0, 1) */
+    uint32_t n_rays;      /* >= 1 */
+    uint32_t max_steps;
+    float b_min, b_max;   /* b_min < b_max, in M */
+    uint32_t reserved[2];
+} SingFanOpts;
+typedef struct SingFanStats {
+    uint64_t total_steps; /* RK4 steps over all rays */
+    double kernel_ms;     /* the integration kernel (HIP events) */
+    double setup_ms;      /* host set-up and upload of the initial states */
+    uint64_t reserved;
+} SingFanStats;
+/* steps_per_ray: n_rays counts, or NULL. */
+int sing_benchmark_fan(SingContext* ctx, const SingFanOpts* opts, uint32_t* steps_per_ray, SingFanStats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -118,6 +118,12 @@ def _type_oracle(L: C.CDLL) -> None:
     L.orc_kerr_adaptive_step.argtypes = [P, P, C.c_double, C.c_double, P, P]
     L.orc_kerr_rk4_substeps.argtypes = [P, P, C.c_double, C.c_int, P]
     L.orc_loss_f64.argtypes = [P, P, P, C.c_double, C.c_double, C.c_double, P, C.c_int64, P, C.c_int, P]
+    L.orc_kerr_geometry.restype = C.c_int
+    L.orc_kerr_geometry.argtypes = [C.c_int, C.c_double, C.c_double, P]
+    L.orc_photon_orbit.restype = C.c_int
+    L.orc_photon_orbit.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, P, C.c_int, P]
+    L.orc_bench_fan.restype = C.c_longlong
+    L.orc_bench_fan.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, P]
 
 
 def _ptr(a: np.ndarray | None):
@@ -218,6 +224,37 @@ def count_step(kerr: bool = True) -> dict:
 
 def kerr_radius(which: int, M: float, a: float) -> float:
     return float(strict().orc_kerr_closed_form(which, M, a))
+
+
+KERR_GEOMETRY_KEYS = ("mass", "spin", "spin_over_mass", "outer_horizon", "inner_horizon", "ergosphere_polar",
+                      "ergosphere_equatorial", "photon_sphere_prograde", "photon_sphere_retrograde", "isco_prograde",
+                      "isco_retrograde")
+
+
+def kerr_geometry(mass: float, spin: float, precision: int = 32) -> dict | None:
+    """The eleven Kerr scalars of the CPU driver's kerr-geometry mode (None: argument error)."""
+    o = np.zeros(11, np.float64)
+    if strict().orc_kerr_geometry(precision, mass, spin, _ptr(o)) != 0:
+        return None
+    return dict(zip(KERR_GEOMETRY_KEYS, o.tolist()))
+
+
+def photon_orbit(mass: float, spin: float, r_init: float, orbits: float = 1.0, h: float = 0.01,
+                 precision: int = 32) -> np.ndarray | None:
+    """Rows (step, t, r, theta, phi) of the CPU driver's photon-orbit mode (None: argument error)."""
+    n = C.c_int(0)
+    strict().orc_photon_orbit(precision, mass, spin, r_init, orbits, h, None, 0, C.byref(n))
+    rows = np.zeros((max(n.value, 1), 5), np.float64)
+    rc = strict().orc_photon_orbit(precision, mass, spin, r_init, orbits, h, _ptr(rows), n.value, C.byref(n))
+    return None if rc != 0 else rows[: n.value]
+
+
+def bench_fan(kerr: bool, spin: float, n_rays: int, max_steps: int, b_min: float = -10.0, b_max: float = 10.0,
+              precision: int = 32) -> tuple[int, np.ndarray]:
+    """Total and per-ray RK4 step counts of the CPU driver's benchmark ray fan."""
+    per = np.zeros(n_rays, np.uint32)
+    total = strict().orc_bench_fan(precision, int(kerr), spin, n_rays, max_steps, b_min, b_max, _ptr(per))
+    return int(total), per
 
 
 def bloom_pipeline(hdr_rgb: np.ndarray, threshold: float, strength: float, exposure: float,

@@ -1194,4 +1194,164 @@ inline void bloom_composite(const float* hdr, const float* bloom, int W, int H, 
         }
 }
 
+// ---------------------------------------------------------------------------
+// The CPU driver's diagnostics (cli/main.cpp:1269-1621): Kerr geometry
+// scalars, the circular photon orbit trail and the benchmark ray fan.  The
+// product runs the orbit and the fan on the device; these are its checker.
+// ---------------------------------------------------------------------------
+template <class T> inline T kerr_ergo_outer(T M, T a, T theta) {  // kerr.hpp:53-57
+    const T c = xcos(theta);
+    const T d = M * M - a * a * c * c;
+    return M + xsqrt(d > T(0.0) ? d : T(0.0));
+}
+
+// cli/main.cpp:1415-1476, in output order: mass, spin, spin/mass, r+, r-,
+// ergosphere at the pole and at the equator, photon orbits (pro, retro),
+// ISCO (pro, retro).  Returns 2 (usage error) for M <= 0 or a outside [0, M].
+template <class T> inline int kerr_geometry(T M, T a, T* out) {
+    if (M <= T(0.0) || a < T(0.0) || a > M) return 2;
+    const T half_pi = T(0.5) * T(3.14159265358979323846);
+    out[0] = M;
+    out[1] = a;
+    out[2] = a / M;
+    out[3] = kerr_r_plus(M, a);
+    out[4] = kerr_r_minus(M, a);
+    out[5] = kerr_ergo_outer(M, a, T(0.0));
+    out[6] = kerr_ergo_outer(M, a, half_pi);
+    out[7] = kerr_photon_pro(M, a);
+    out[8] = kerr_photon_ret(M, a);
+    out[9] = kerr_isco(M, a, true);
+    out[10] = kerr_isco(M, a, false);
+    return 0;
+}
+
+struct OrbitRow {
+    std::int32_t step;  // -1: the sentinel row after a ray left the orbit (non-finite or inside the cut)
+    double t, r, theta, phi;
+};
+
+// cli/main.cpp:1279-1410: one equatorial circular null orbit at r_init,
+// integrated for `orbits` periods with step h.  Rows: the initial state, then
+// every sub_every = max(1, int(period / (200 h))) steps and the last step, or a
+// -1 sentinel row where the ray leaves (r non-finite or below 1.001 r+).
+// Schwarzschild (a = 0): the 8-component geodesic, u^phi = sqrt(1 - 2M/r)/r.
+// Kerr: the Hamiltonian form with E = 1, L_z = (r sqrt(Delta) - 2 M a)/(r - 2M),
+// Q = 0; the period comes from d(phi)/d(lambda) at the start.  Returns 0, or 2
+// for the argument errors the driver rejects.
+template <class T>
+inline int photon_orbit(T M, T a, T r_init, T orbits, T h, std::vector<OrbitRow>& rows) {
+    rows.clear();
+    const T pi = T(3.14159265358979323846);
+    if (M <= T(0.0) || a < T(0.0) || a > M) return 2;
+    const T rs = T(2.0) * M;
+    const T r_plus = (a > T(0.0)) ? kerr_r_plus(M, a) : rs;
+    if (r_init <= r_plus * T(1.001) || orbits <= T(0.0) || h <= T(0.0)) return 2;
+    auto emit = [&](int step, T t, T r, T th, T ph) { rows.push_back({step, double(t), double(r), double(th), double(ph)}); };
+    if (a <= T(0.0)) {
+        const T f = T(1.0) - rs / r_init;
+        SState<T> y{T(0.0), r_init, T(0.5) * pi, T(0.0), T(1.0), T(0.0), T(0.0), xsqrt(f) / r_init};
+        const T period = T(2.0) * pi / y.uphi;
+        const int total = int(orbits * period / h);
+        const int every = std::max(1, int(period / (h * T(200.0))));
+        emit(0, y.t, y.r, y.theta, y.phi);
+        for (int step = 1; step <= total; ++step) {
+            y = schw_rk4(y, h, rs);
+            if (!xisfinite(y.r) || y.r < rs * T(1.001)) {
+                emit(-1, y.t, y.r, y.theta, y.phi);
+                break;
+            }
+            if (step % every == 0 || step == total) emit(step, y.t, y.r, y.theta, y.phi);
+        }
+        return 0;
+    }
+    const T Delta = r_init * r_init - T(2.0) * M * r_init + a * a;
+    if (Delta <= T(0.0)) return 2;
+    const T b_pro = (r_init * xsqrt(Delta) - T(2.0) * M * a) / (r_init - T(2.0) * M);
+    const KConst<T> c{T(1.0), b_pro, T(0.0), a, M};
+    KState<T> y{T(0.0), r_init, T(0.5) * pi, T(0.0), T(0.0), T(0.0)};
+    const KState<T> d0 = kerr_rhs(y, c);
+    if (!xisfinite(d0.phi) || d0.phi <= T(0.0)) return 2;
+    const T period = T(2.0) * pi / d0.phi;
+    const int total = int(orbits * period / h);
+    const int every = std::max(1, int(period / (h * T(200.0))));
+    emit(0, y.t, y.r, y.theta, y.phi);
+    for (int step = 1; step <= total; ++step) {
+        y = kerr_rk4(y, h, c);
+        if (!xisfinite(y.r) || y.r < r_plus * T(1.001)) {
+            emit(-1, y.t, y.r, y.theta, y.phi);
+            break;
+        }
+        if (step % every == 0 || step == total) emit(step, y.t, y.r, y.theta, y.phi);
+    }
+    return 0;
+}
+
+// cli/main.cpp:1492-1590: the benchmark's equatorial ray fan (M = 1, h = 0.1).
+// Ray i has impact parameter b_i = b_min + (b_max - b_min)(i + 1/2)/n and
+// starts at x = max(50, 2 max|b|) on the line y = b moving in -x.  Each ray
+// steps until max_steps, a non-finite r or phi, or r below the cut (1.01 r_s
+// Schwarzschild, 1.02 r+ Kerr).  Kerr rays take u^t from the null condition
+// (future root) and become (E, L_z, Q = 0) and p_r = Sigma u^r / Delta; a ray
+// whose quadratic has no real root is skipped (0 steps).  per_ray (may be null)
+// gets each ray's step count; the return value is the total.
+template <class T>
+inline long long bench_fan(bool kerr, T spin, int n_rays, int max_steps, T b_min, T b_max, std::uint32_t* per_ray) {
+    const T pi = T(3.14159265358979323846);
+    const T M = T(1.0), rs = T(2.0) * M, h = T(0.1);
+    const T a = kerr ? spin * M : T(0.0);
+    const T bmin = b_min * M, bmax = b_max * M;
+    const T ab_min = bmin < T(0.0) ? -bmin : bmin, ab_max = bmax < T(0.0) ? -bmax : bmax;
+    const T x0 = std::max(T(50.0) * M, T(2.0) * std::max(ab_min, ab_max));
+    long long total = 0;
+    for (int i = 0; i < n_rays; ++i) {
+        const T b = bmin + (bmax - bmin) * (T(double(i)) + T(0.5)) / T(double(n_rays));
+        const T r0 = xsqrt(x0 * x0 + b * b);
+        const T phi0 = xatan2(b, x0);
+        const T ur = -xcos(phi0);
+        const T uphi = xsin(phi0) / r0;
+        int n = 0;
+        if (!kerr) {
+            const T cut = T(1.01) * rs;
+            const T f = T(1.0) - rs / r0;
+            const T ut = xsqrt(((ur * ur) / f + r0 * r0 * uphi * uphi) / f);
+            SState<T> y{T(0.0), r0, T(0.5) * pi, phi0, ut, ur, T(0.0), uphi};
+            for (int step = 0; step < max_steps; ++step) {
+                if (!xisfinite(y.r) || !xisfinite(y.phi)) break;
+                if (y.r < cut) break;
+                y = schw_rk4(y, h, rs);
+                ++n;
+            }
+        } else {
+            const T cut = T(1.02) * kerr_r_plus(M, a);
+            const T Delta = r0 * r0 - T(2.0) * M * r0 + a * a;
+            const T g_tt = -(T(1.0) - T(2.0) * M / r0);
+            const T g_tp = -T(2.0) * M * a / r0;
+            const T g_rr = r0 * r0 / Delta;
+            const T g_pp = ((r0 * r0 + a * a) * (r0 * r0 + a * a) - a * a * Delta) / (r0 * r0);
+            const T qa = g_tt, qb = T(2.0) * g_tp * uphi, qc = g_rr * ur * ur + g_pp * uphi * uphi;
+            const T disc = qb * qb - T(4.0) * qa * qc;
+            if (disc >= T(0.0)) {
+                const T ut = (-qb - xsqrt(disc)) / (T(2.0) * qa);
+                const KConst<T> c{-(g_tt * ut + g_tp * uphi), g_tp * ut + g_pp * uphi, T(0.0), a, M};
+                KState<T> y{T(0.0), r0, T(0.5) * pi, phi0, T(0.0), T(0.0)};
+                // kerr_hamilton.h:192-200: p_r = Sigma u^r / Delta, p_theta = Sigma u^theta (= 0)
+                const T ct = xcos(y.theta);
+                const T Sig = r0 * r0 + a * a * ct * ct;
+                const T Del = r0 * r0 - T(2.0) * M * r0 + a * a;
+                y.p_r = Sig * ur / Del;
+                y.p_theta = Sig * T(0.0);
+                for (int step = 0; step < max_steps; ++step) {
+                    if (!xisfinite(y.r) || !xisfinite(y.phi)) break;
+                    if (y.r < cut) break;
+                    y = kerr_rk4(y, h, c);
+                    ++n;
+                }
+            }
+        }
+        if (per_ray) per_ray[i] = std::uint32_t(n);
+        total += n;
+    }
+    return total;
+}
+
 }  // namespace orc

@@ -440,3 +440,40 @@ void orc_kerr_rk4_substeps(const double* y, const double* c, double h, int n, do
     out[0] = s.r; out[1] = s.theta; out[2] = s.phi; out[3] = s.p_r; out[4] = s.p_theta;
 }
 }
+
+extern "C" {
+// The CPU driver's diagnostics (cli/main.cpp:1269-1621); precision 32 runs the
+// reference's float arithmetic, 64 the same steps in double.
+int orc_kerr_geometry(int precision, double M, double a, double* out11) {
+    if (precision == 32) {
+        float o[11];
+        const int rc = kerr_geometry<float>(float(M), float(a), o);
+        for (int i = 0; i < 11; ++i) out11[i] = o[i];
+        return rc;
+    }
+    return kerr_geometry<double>(M, a, out11);
+}
+// rows: cap x 5 doubles (step, t, r, theta, phi); *n_rows = rows produced
+// (may exceed cap: then only cap are written).  Returns 0 or 2 (argument error).
+int orc_photon_orbit(int precision, double M, double a, double r_init, double orbits, double h, double* rows,
+                     int cap, int* n_rows) {
+    std::vector<OrbitRow> v;
+    const int rc = precision == 32 ? photon_orbit<float>(float(M), float(a), float(r_init), float(orbits), float(h), v)
+                                   : photon_orbit<double>(M, a, r_init, orbits, h, v);
+    *n_rows = int(v.size());
+    for (int i = 0; i < int(v.size()) && i < cap; ++i) {
+        rows[5 * i + 0] = v[i].step;
+        rows[5 * i + 1] = v[i].t;
+        rows[5 * i + 2] = v[i].r;
+        rows[5 * i + 3] = v[i].theta;
+        rows[5 * i + 4] = v[i].phi;
+    }
+    return rc;
+}
+long long orc_bench_fan(int precision, int kerr, double spin, int n_rays, int max_steps, double b_min, double b_max,
+                        std::uint32_t* per_ray) {
+    if (precision == 32)
+        return bench_fan<float>(kerr != 0, float(spin), n_rays, max_steps, float(b_min), float(b_max), per_ray);
+    return bench_fan<double>(kerr != 0, spin, n_rays, max_steps, b_min, b_max, per_ray);
+}
+}

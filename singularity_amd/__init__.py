@@ -491,3 +491,70 @@ def render_batch(be: "HipBackend", scenes: list, opts: BatchOpts | None = None, 
     for f in frames:
         f.stats = st
     return frames, st
+
+
+# --- the CPU driver's diagnostics (include/sing_abi.h: sing_kerr_geometry, ...) --
+class KerrGeometry(C.Structure):
+    _fields_ = [(k, C.c_float) for k in (
+        "mass", "spin", "spin_over_mass", "outer_horizon", "inner_horizon", "ergosphere_polar",
+        "ergosphere_equatorial", "photon_sphere_prograde", "photon_sphere_retrograde", "isco_prograde",
+        "isco_retrograde")]
+
+    def as_dict(self) -> dict:
+        return {k: float(getattr(self, k)) for k, _ in self._fields_}
+
+
+class OrbitRow(C.Structure):
+    _fields_ = [("step", C.c_int32), ("t", C.c_float), ("r", C.c_float), ("theta", C.c_float), ("phi", C.c_float)]
+
+
+class FanOpts(C.Structure):
+    _fields_ = [("metric", C.c_uint32), ("spin", C.c_float), ("n_rays", C.c_uint32), ("max_steps", C.c_uint32),
+                ("b_min", C.c_float), ("b_max", C.c_float), ("reserved", C.c_uint32 * 2)]
+
+
+class FanStats(C.Structure):
+    _fields_ = [("total_steps", C.c_uint64), ("kernel_ms", C.c_double), ("setup_ms", C.c_double),
+                ("reserved", C.c_uint64)]
+
+
+_SIGS["sing_kerr_geometry"] = (C.c_int, [C.c_float, C.c_float, C.POINTER(KerrGeometry)])
+_SIGS["sing_photon_orbit"] = (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                        C.POINTER(OrbitRow), C.c_uint32, C.POINTER(C.c_uint32),
+                                        C.POINTER(C.c_double)])
+_SIGS["sing_benchmark_fan"] = (C.c_int, [C.c_void_p, C.POINTER(FanOpts), C.c_void_p, C.POINTER(FanStats)])
+
+
+def kerr_geometry(mass: float, spin: float) -> dict:
+    """cli/main.cpp:1415-1476 kerr-geometry scalars (float, host closed forms)."""
+    g = KerrGeometry()
+    rc = lib().sing_kerr_geometry(mass, spin, C.byref(g))
+    if rc != SING_OK:
+        raise SingError(rc, lib().sing_last_error(None).decode())
+    return g.as_dict()
+
+
+def photon_orbit(be: "HipBackend", mass: float, spin: float, r_init: float, orbits: float = 1.0,
+                 h_step: float = 0.01) -> tuple[np.ndarray, np.ndarray, float]:
+    """cli/main.cpp:1279-1410 photon-orbit on the device: (steps int32, rows (n, 4) float32 t r theta phi,
+    kernel ms)."""
+    bound = C.c_uint32(0)
+    rc = lib().sing_photon_orbit(be._ctx, mass, spin, r_init, orbits, h_step, None, 0, C.byref(bound), None)
+    if bound.value == 0:
+        raise SingError(rc, lib().sing_last_error(be._ctx).decode())
+    rows = (OrbitRow * bound.value)()
+    n, ms = C.c_uint32(0), C.c_double(0.0)
+    be._check(lib().sing_photon_orbit(be._ctx, mass, spin, r_init, orbits, h_step, rows, bound.value, C.byref(n),
+                                      C.byref(ms)))
+    a = np.frombuffer(rows, dtype=np.dtype([("step", "<i4"), ("v", "<f4", (4,))]), count=n.value)
+    return a["step"].copy(), a["v"].copy(), ms.value
+
+
+def benchmark_fan(be: "HipBackend", metric: int, spin: float = 0.0, n_rays: int = 100, max_steps: int = 5000,
+                  b_min: float = -10.0, b_max: float = 10.0) -> tuple[int, np.ndarray, FanStats]:
+    """cli/main.cpp:1492-1590 benchmark ray fan on the device: (total steps, per-ray steps, stats)."""
+    o = FanOpts(metric=metric, spin=spin, n_rays=n_rays, max_steps=max_steps, b_min=b_min, b_max=b_max)
+    per = np.zeros(n_rays, np.uint32)
+    st = FanStats()
+    be._check(lib().sing_benchmark_fan(be._ctx, C.byref(o), per.ctypes.data, C.byref(st)))
+    return int(st.total_steps), per, st

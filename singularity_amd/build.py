@@ -6,6 +6,7 @@ source tree:
 
     singularity_amd/libsing_hip.so   the C-ABI library (include/sing_abi.h)
     singularity_amd/hip_cli          flag-compatible replacement of cuda_cli
+    singularity_amd/hip_diag         the CPU driver's kerr-geometry / photon-orbit / benchmark modes
     singularity_amd/hbm_copy         HBM counter calibration for profiling (256 MiB copy)
 
 Usage: python -m singularity_amd.build  (or import and call build()).
@@ -26,6 +27,7 @@ CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libsing_hip.so"
 CLI = PKG / "hip_cli"
+DIAG = PKG / "hip_diag"
 COPY = PKG / "hbm_copy"
 
 ARCH = "gfx950"
@@ -41,7 +43,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-Wall"]
 
-HIP_SOURCES = ["trace.hip", "post.hip", "context.hip", "adjoint.hip", "batch.hip"]
+HIP_SOURCES = ["trace.hip", "post.hip", "context.hip", "adjoint.hip", "batch.hip", "diag.hip"]
 CXX_SOURCES = ["host_scene.cpp"]
 
 
@@ -98,6 +100,17 @@ def build(verbose: bool = False) -> Path:
             ]
         )
         os.replace(tmpc, CLI)
+    diag_src = CSRC / "hip_diag.cpp"
+    if diag_src.exists():
+        tmpd = DIAG.with_suffix(".tmp")
+        _run(
+            [
+                "g++", "-O2", "-std=c++20", "-Wall", "-o", str(tmpd), str(diag_src),
+                f"-L{PKG}", "-lsing_hip", "-Wl,-rpath,$ORIGIN",
+                "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib",
+            ]
+        )
+        os.replace(tmpd, DIAG)
     copy_src = CSRC / "hbm_copy.hip"
     if copy_src.exists():
         tmpk = COPY.with_suffix(".tmp")

@@ -98,10 +98,11 @@ template <class R> __device__ __forceinline__ R dclamp(R x, R lo, R hi) { return
 // ---- Kerr Hamiltonian RHS (kerr_hamilton.h:97-155) ----------------------------
 // State (r, theta, phi, p_r, p_theta); st/ct = sin/cos(theta).
 template <class R> struct KerrK {
-    R EE, L, L2, cB1, cB0, cphi;  // E^2, L_z, L_z^2, 4 M a E L_z, a^2 L_z^2, 2 M a E
+    R E, EE, L, L2, cB1, cB0, cphi;  // E, E^2, L_z, L_z^2, 4 M a E L_z, a^2 L_z^2, 2 M a E
 };
 template <class R> __device__ __forceinline__ KerrK<R> kerr_k(const SceneConst<R>& sc, R E, R L) {
     KerrK<R> k;
+    k.E = E;
     k.EE = E * E;
     k.L = L;
     k.L2 = L * L;
@@ -111,9 +112,11 @@ template <class R> __device__ __forceinline__ KerrK<R> kerr_k(const SceneConst<R
     return k;
 }
 
-template <class R>
+// T_OUT: also d t / d lambda = (A E - 2 M a r L_z) / (Sigma Delta) into *dt (the
+// renderer does not track t; the photon-orbit diagnostic does).
+template <class R, bool T_OUT = false>
 __device__ __forceinline__ void kerr_rhs_sc(R r, R pr, R pth, R st, R ct, const SceneConst<R>& sc, const KerrK<R>& k,
-                                            R& dr, R& dth, R& dph, R& dpr, R& dpth) {
+                                            R& dr, R& dth, R& dph, R& dpr, R& dpth, R* dt = nullptr) {
     const R a2 = sc.a2;
     const R s2 = st * st;
     const R c2 = ct * ct;
@@ -130,6 +133,7 @@ __device__ __forceinline__ void kerr_rhs_sc(R r, R pr, R pth, R st, R ct, const 
     dr = Del * pr * iS;
     dth = pth * iS;
     dph = (k.cphi * r * s2 + (Del - a2 * s2) * k.L) * (iSD * iss);
+    if constexpr (T_OUT) *dt = (A * k.E - sc.twoM * sc.a * r * k.L) * iSD;
     // 2H = B/(Sigma Delta) + L^2/(Sigma sin^2) + (Delta p_r^2 + p_th^2)/Sigma,
     // B = -A E^2 + 4 M r a E L - a^2 L^2  (kerr_hamilton.h:121-152)
     const R B = k.cB1 * r - A * k.EE - k.cB0;
@@ -158,25 +162,27 @@ template <class R> __device__ __forceinline__ R kerr_pole_scale(R st) {
 }
 
 // Classical RK4 step of size h (kerr_hamilton.h:171-177); (st, ct) = sin/cos of
-// the starting theta, reused by the first stage.
-template <class R>
+// the starting theta, reused by the first stage.  T_OUT: t advances too (*t).
+template <class R, bool T_OUT = false>
 __device__ __forceinline__ void kerr_rk4_raw(R& r, R& th, R& ph, R& pr, R& pth, R h, R st, R ct,
-                                             const SceneConst<R>& sc, const KerrK<R>& k) {
+                                             const SceneConst<R>& sc, const KerrK<R>& k, R* t = nullptr) {
     const R hh = R(0.5) * h;
-    R k_r, k_th, k_ph, k_pr, k_pth;
-    kerr_rhs_sc(r, pr, pth, st, ct, sc, k, k_r, k_th, k_ph, k_pr, k_pth);
-    R a_r = k_r, a_th = k_th, a_ph = k_ph, a_pr = k_pr, a_pth = k_pth;
+    R k_r, k_th, k_ph, k_pr, k_pth, k_t = R(0);
+    kerr_rhs_sc<R, T_OUT>(r, pr, pth, st, ct, sc, k, k_r, k_th, k_ph, k_pr, k_pth, &k_t);
+    R a_r = k_r, a_th = k_th, a_ph = k_ph, a_pr = k_pr, a_pth = k_pth, a_t = k_t;
     R y_r = r + hh * k_r, y_th = th + hh * k_th, y_pr = pr + hh * k_pr, y_pth = pth + hh * k_pth;
     dsincos(y_th, &st, &ct);
-    kerr_rhs_sc(y_r, y_pr, y_pth, st, ct, sc, k, k_r, k_th, k_ph, k_pr, k_pth);
+    kerr_rhs_sc<R, T_OUT>(y_r, y_pr, y_pth, st, ct, sc, k, k_r, k_th, k_ph, k_pr, k_pth, &k_t);
     a_r += R(2) * k_r; a_th += R(2) * k_th; a_ph += R(2) * k_ph; a_pr += R(2) * k_pr; a_pth += R(2) * k_pth;
+    if constexpr (T_OUT) a_t += R(2) * k_t;
     y_r = r + hh * k_r; y_th = th + hh * k_th; y_pr = pr + hh * k_pr; y_pth = pth + hh * k_pth;
     dsincos(y_th, &st, &ct);
-    kerr_rhs_sc(y_r, y_pr, y_pth, st, ct, sc, k, k_r, k_th, k_ph, k_pr, k_pth);
+    kerr_rhs_sc<R, T_OUT>(y_r, y_pr, y_pth, st, ct, sc, k, k_r, k_th, k_ph, k_pr, k_pth, &k_t);
     a_r += R(2) * k_r; a_th += R(2) * k_th; a_ph += R(2) * k_ph; a_pr += R(2) * k_pr; a_pth += R(2) * k_pth;
+    if constexpr (T_OUT) a_t += R(2) * k_t;
     y_r = r + h * k_r; y_th = th + h * k_th; y_pr = pr + h * k_pr; y_pth = pth + h * k_pth;
     dsincos(y_th, &st, &ct);
-    kerr_rhs_sc(y_r, y_pr, y_pth, st, ct, sc, k, k_r, k_th, k_ph, k_pr, k_pth);
+    kerr_rhs_sc<R, T_OUT>(y_r, y_pr, y_pth, st, ct, sc, k, k_r, k_th, k_ph, k_pr, k_pth, &k_t);
     a_r += k_r; a_th += k_th; a_ph += k_ph; a_pr += k_pr; a_pth += k_pth;
     const R h6 = h * R(1.0 / 6.0);
     r += h6 * a_r;
@@ -184,15 +190,16 @@ __device__ __forceinline__ void kerr_rk4_raw(R& r, R& th, R& ph, R& pr, R& pth, 
     ph += h6 * a_ph;
     pr += h6 * a_pr;
     pth += h6 * a_pth;
+    if constexpr (T_OUT) *t += h6 * (a_t + k_t);
 }
 
 // The reference's damped step (kerr_hamilton.h:165-178): RK4 with h = hs * pole scale.
-template <class R>
+template <class R, bool T_OUT = false>
 __device__ __forceinline__ void kerr_rk4(R& r, R& th, R& ph, R& pr, R& pth, R hs, const SceneConst<R>& sc,
-                                         const KerrK<R>& k) {
+                                         const KerrK<R>& k, R* t = nullptr) {
     R st, ct;
     dsincos(th, &st, &ct);
-    kerr_rk4_raw(r, th, ph, pr, pth, hs * kerr_pole_scale(st), st, ct, sc, k);
+    kerr_rk4_raw<R, T_OUT>(r, th, ph, pr, pth, hs * kerr_pole_scale(st), st, ct, sc, k, t);
 }
 
 // Error-controlled step by step doubling: one step of h against two of h/2
@@ -316,8 +323,10 @@ __device__ __forceinline__ void schw_rhs(R r, R th, R ut, R ur, R uth, R uph, R 
     d_uph = R(-2) * ur * uph * ir - R(2) * (ct * is) * uth * uph;
 }
 
-template <class R>
-__device__ __forceinline__ void schw_rk4(R& r, R& th, R& ph, R& ut, R& ur, R& uth, R& uph, R h, R rs) {
+// T_OUT: t advances too (*t; d t / d lambda = u^t).
+template <class R, bool T_OUT = false>
+__device__ __forceinline__ void schw_rk4(R& r, R& th, R& ph, R& ut, R& ur, R& uth, R& uph, R h, R rs,
+                                         R* t = nullptr) {
     const R hh = R(0.5) * h;
     R d_ut, d_ur, d_uth, d_uph;
     // stage 1: position derivatives are the velocities themselves
@@ -325,23 +334,28 @@ __device__ __forceinline__ void schw_rk4(R& r, R& th, R& ph, R& ut, R& ur, R& ut
     R a_r = ur, a_th = uth, a_ph = uph, a_ut = d_ut, a_ur = d_ur, a_uth = d_uth, a_uph = d_uph;
     R y_r = r + hh * ur, y_th = th + hh * uth;
     R y_ut = ut + hh * d_ut, y_ur = ur + hh * d_ur, y_uth = uth + hh * d_uth, y_uph = uph + hh * d_uph;
+    R a_t = ut;  // d t / d lambda = u^t at each stage (T_OUT only)
+    if constexpr (T_OUT) a_t += R(2) * y_ut;
     // stage 2
     schw_rhs(y_r, y_th, y_ut, y_ur, y_uth, y_uph, rs, d_ut, d_ur, d_uth, d_uph);
     a_r += R(2) * y_ur; a_th += R(2) * y_uth; a_ph += R(2) * y_uph;
     a_ut += R(2) * d_ut; a_ur += R(2) * d_ur; a_uth += R(2) * d_uth; a_uph += R(2) * d_uph;
     R n_r = r + hh * y_ur, n_th = th + hh * y_uth;
     R n_ut = ut + hh * d_ut, n_ur = ur + hh * d_ur, n_uth = uth + hh * d_uth, n_uph = uph + hh * d_uph;
+    if constexpr (T_OUT) a_t += R(2) * n_ut;
     // stage 3
     schw_rhs(n_r, n_th, n_ut, n_ur, n_uth, n_uph, rs, d_ut, d_ur, d_uth, d_uph);
     a_r += R(2) * n_ur; a_th += R(2) * n_uth; a_ph += R(2) * n_uph;
     a_ut += R(2) * d_ut; a_ur += R(2) * d_ur; a_uth += R(2) * d_uth; a_uph += R(2) * d_uph;
     y_r = r + h * n_ur; y_th = th + h * n_uth;
     y_ut = ut + h * d_ut; y_ur = ur + h * d_ur; y_uth = uth + h * d_uth; y_uph = uph + h * d_uph;
+    if constexpr (T_OUT) a_t += y_ut;
     // stage 4
     schw_rhs(y_r, y_th, y_ut, y_ur, y_uth, y_uph, rs, d_ut, d_ur, d_uth, d_uph);
     a_r += y_ur; a_th += y_uth; a_ph += y_uph;
     a_ut += d_ut; a_ur += d_ur; a_uth += d_uth; a_uph += d_uph;
     const R h6 = h * R(1.0 / 6.0);
+    if constexpr (T_OUT) *t += h6 * a_t;
     r += h6 * a_r; th += h6 * a_th; ph += h6 * a_ph;
     ut += h6 * a_ut; ur += h6 * a_ur; uth += h6 * a_uth; uph += h6 * a_uph;
 }

@@ -72,6 +72,33 @@ __device__ __forceinline__ void dsincos(double x, double* s, double* c) {
     *s = (q & 2) ? -ss : ss;
     *c = ((q + 1) & 2) ? -cc : cc;
 }
+// sin and cos of th0 + d from s0 = sin th0 and c0 = cos th0 by the angle-
+// addition formulas, with sin d and cos d from the same Taylor polynomials as
+// dsincos (accurate for |d| <= pi/4).  An RK4 stage moves theta by a small d,
+// so this replaces the stage's reduction and quadrant selection by 15
+// FMA-class instructions; a larger |d| takes dsincos.  In double the stages
+// keep the full dsincos (the FP64 pass is a small share of a frame).
+__device__ __forceinline__ void sincos_shift(float th0, float d, float s0, float c0, float* s, float* c) {
+    if (__builtin_expect(fabsf(d) <= 0.785398163f, 1)) {
+        const float d2 = d * d;
+        float ps = fmaf(d2, 2.7557319e-06f, -1.9841270e-04f);
+        ps = fmaf(d2, ps, 8.3333338e-03f);
+        ps = fmaf(d2, ps, -1.6666667e-01f);
+        const float sd = fmaf(d * d2, ps, d);
+        float pc = fmaf(d2, -2.7557320e-07f, 2.4801588e-05f);
+        pc = fmaf(d2, pc, -1.3888889e-03f);
+        pc = fmaf(d2, pc, 4.1666668e-02f);
+        pc = fmaf(d2, pc, -0.5f);
+        const float cd = fmaf(d2, pc, 1.0f);
+        *s = fmaf(s0, cd, c0 * sd);
+        *c = fmaf(c0, cd, -(s0 * sd));
+    } else {
+        dsincos(th0 + d, s, c);
+    }
+}
+__device__ __forceinline__ void sincos_shift(double th0, double d, double, double, double* s, double* c) {
+    dsincos(th0 + d, s, c);
+}
 __device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float dlog(float x) { return logf(x); }
@@ -98,7 +125,7 @@ template <class R> __device__ __forceinline__ R dclamp(R x, R lo, R hi) { return
 // ---- Kerr Hamiltonian RHS (kerr_hamilton.h:97-155) ----------------------------
 // State (r, theta, phi, p_r, p_theta); st/ct = sin/cos(theta).
 template <class R> struct KerrK {
-    R E, EE, L, L2, cB1, cB0, cphi;  // E, E^2, L_z, L_z^2, 4 M a E L_z, a^2 L_z^2, 2 M a E
+    R E, EE, L, L2, cB1, cB0, cphi, mhcB1;  // E, E^2, L_z, L_z^2, 4 M a E L_z, a^2 L_z^2, 2 M a E, -2 M a E L_z
 };
 template <class R> __device__ __forceinline__ KerrK<R> kerr_k(const SceneConst<R>& sc, R E, R L) {
     KerrK<R> k;
@@ -109,6 +136,7 @@ template <class R> __device__ __forceinline__ KerrK<R> kerr_k(const SceneConst<R
     k.cB1 = R(2) * sc.twoM * sc.a * E * L;
     k.cB0 = sc.a2 * k.L2;
     k.cphi = sc.twoM * sc.a * E;
+    k.mhcB1 = R(-0.5) * k.cB1;
     return k;
 }
 
@@ -117,42 +145,41 @@ template <class R> __device__ __forceinline__ KerrK<R> kerr_k(const SceneConst<R
 template <class R, bool T_OUT = false>
 __device__ __forceinline__ void kerr_rhs_sc(R r, R pr, R pth, R st, R ct, const SceneConst<R>& sc, const KerrK<R>& k,
                                             R& dr, R& dth, R& dph, R& dpr, R& dpth, R* dt = nullptr) {
+    // 2H = B/(Sigma Delta) + L^2/(Sigma sin^2) + (Delta p_r^2 + p_th^2)/Sigma,
+    // B = -A E^2 + 4 M r a E L - a^2 L^2  (kerr_hamilton.h:121-152).  The
+    // partials are the reference's, regrouped for fewer instructions: the
+    // -1/2 of dp/dlambda = -(1/2) d(2H)/dq folded into each term, Sigma =
+    // r^2 + a^2 - a^2 sin^2, Delta / (Sigma Delta) = 1/Sigma, and every
+    // d/dtheta term sharing sin(theta) cos(theta) (d Sigma/d theta =
+    // -2 a^2 sin cos, dA/d theta = -2 a^2 Delta sin cos).
     const R a2 = sc.a2;
     const R s2 = st * st;
-    const R c2 = ct * ct;
-    const R safe = (s2 < R(1e-10)) ? R(1e-10) : s2;
+    const R safe = dmax(s2, R(1e-10));
     const R r2 = r * r;
-    const R Sig = r2 + a2 * c2;
-    const R Del = r2 - sc.twoM * r + a2;
+    const R as2 = a2 * s2;
     const R r2a2 = r2 + a2;
-    const R A = r2a2 * r2a2 - a2 * Del * s2;
+    const R Sig = r2a2 - as2;
+    const R Del = r2 - sc.twoM * r + a2;
+    const R A = r2a2 * r2a2 - Del * as2;
+    const R B = (k.cB1 * r - k.cB0) - A * k.EE;
     const R iS = drcp(Sig);
     const R iD = drcp(Del);
     const R iss = drcp(safe);
     const R iSD = iS * iD;
+    const R iSiss = iS * iss;
     dr = Del * pr * iS;
     dth = pth * iS;
-    dph = (k.cphi * r * s2 + (Del - a2 * s2) * k.L) * (iSD * iss);
+    dph = (k.cphi * r * s2 + (Del - as2) * k.L) * (iD * iSiss);
     if constexpr (T_OUT) *dt = (A * k.E - sc.twoM * sc.a * r * k.L) * iSD;
-    // 2H = B/(Sigma Delta) + L^2/(Sigma sin^2) + (Delta p_r^2 + p_th^2)/Sigma,
-    // B = -A E^2 + 4 M r a E L - a^2 L^2  (kerr_hamilton.h:121-152)
-    const R B = k.cB1 * r - A * k.EE - k.cB0;
-    const R twor = R(2) * r;
-    const R Del_r = twor - sc.twoM;
-    const R A_r = R(2) * twor * r2a2 - a2 * Del_r * s2;
-    const R B_r = k.cB1 - A_r * k.EE;
-    const R sin2t = R(2) * st * ct;
-    const R Sig_t = -a2 * sin2t;
-    const R B_t = a2 * Del * sin2t * k.EE;
-    const R iS2 = iS * iS;
+    const R rmM = r - sc.M;                                       // (1/2) d Delta/dr
+    const R hBr = ((r + r) * r2a2 - rmM * as2) * k.EE + k.mhcB1;  // -(1/2) dB/dr
+    const R BiSD = B * iSD;
+    const R Xh = r * Del + Sig * rmM;                             // (1/2) d(Sigma Delta)/dr
     const R pr2 = pr * pr;
     const R Dp2 = Del * pr2 + pth * pth;
-    const R dHdr = B_r * iSD - B * (twor * Del + Sig * Del_r) * (iSD * iSD) - k.L2 * twor * (iS2 * iss)
-                   + Del_r * pr2 * iS - Dp2 * twor * iS2;
-    const R dHdth = B_t * iSD - B * Sig_t * (iS2 * iD) - k.L2 * (Sig_t * safe + Sig * sin2t) * (iS2 * iss * iss)
-                    - Dp2 * Sig_t * iS2;
-    dpr = R(-0.5) * dHdr;
-    dpth = R(-0.5) * dHdth;
+    dpr = hBr * iSD + BiSD * Xh * iSD + k.L2 * r * (iS * iSiss) - rmM * pr2 * iS + Dp2 * r * (iS * iS);
+    const R Y = (k.EE + BiSD) + Dp2 * iS;
+    dpth = (st * ct) * (k.L2 * (Sig - a2 * safe) * (iSiss * iSiss) - (a2 * iS) * Y);
 }
 
 // Pole damping factor clamp(5|sin theta|, 0.05, 1) (kerr_hamilton.h:166-169).
@@ -163,25 +190,35 @@ template <class R> __device__ __forceinline__ R kerr_pole_scale(R st) {
 
 // Classical RK4 step of size h (kerr_hamilton.h:171-177); (st, ct) = sin/cos of
 // the starting theta, reused by the first stage.  T_OUT: t advances too (*t).
-template <class R, bool T_OUT = false>
+// SHIFT: stages 2-4 rotate (sin, cos) of the starting theta by the stage's
+// theta increment (sincos_shift) instead of evaluating sincos afresh; the
+// error-controlled step, which runs three RK4 steps per attempt, keeps the
+// full evaluation (the rotation's fallback branches cost it occupancy).
+template <class R, bool T_OUT = false, bool SHIFT = true>
 __device__ __forceinline__ void kerr_rk4_raw(R& r, R& th, R& ph, R& pr, R& pth, R h, R st, R ct,
                                              const SceneConst<R>& sc, const KerrK<R>& k, R* t = nullptr) {
     const R hh = R(0.5) * h;
+    const R st0 = st, ct0 = ct;
     R k_r, k_th, k_ph, k_pr, k_pth, k_t = R(0);
     kerr_rhs_sc<R, T_OUT>(r, pr, pth, st, ct, sc, k, k_r, k_th, k_ph, k_pr, k_pth, &k_t);
     R a_r = k_r, a_th = k_th, a_ph = k_ph, a_pr = k_pr, a_pth = k_pth, a_t = k_t;
-    R y_r = r + hh * k_r, y_th = th + hh * k_th, y_pr = pr + hh * k_pr, y_pth = pth + hh * k_pth;
-    dsincos(y_th, &st, &ct);
+    // the stages need theta only through sin/cos: rotate (st0, ct0) by the
+    // stage's theta increment
+    R y_r = r + hh * k_r, y_pr = pr + hh * k_pr, y_pth = pth + hh * k_pth;
+    if constexpr (SHIFT) sincos_shift(th, hh * k_th, st0, ct0, &st, &ct);
+    else dsincos(th + hh * k_th, &st, &ct);
     kerr_rhs_sc<R, T_OUT>(y_r, y_pr, y_pth, st, ct, sc, k, k_r, k_th, k_ph, k_pr, k_pth, &k_t);
     a_r += R(2) * k_r; a_th += R(2) * k_th; a_ph += R(2) * k_ph; a_pr += R(2) * k_pr; a_pth += R(2) * k_pth;
     if constexpr (T_OUT) a_t += R(2) * k_t;
-    y_r = r + hh * k_r; y_th = th + hh * k_th; y_pr = pr + hh * k_pr; y_pth = pth + hh * k_pth;
-    dsincos(y_th, &st, &ct);
+    y_r = r + hh * k_r; y_pr = pr + hh * k_pr; y_pth = pth + hh * k_pth;
+    if constexpr (SHIFT) sincos_shift(th, hh * k_th, st0, ct0, &st, &ct);
+    else dsincos(th + hh * k_th, &st, &ct);
     kerr_rhs_sc<R, T_OUT>(y_r, y_pr, y_pth, st, ct, sc, k, k_r, k_th, k_ph, k_pr, k_pth, &k_t);
     a_r += R(2) * k_r; a_th += R(2) * k_th; a_ph += R(2) * k_ph; a_pr += R(2) * k_pr; a_pth += R(2) * k_pth;
     if constexpr (T_OUT) a_t += R(2) * k_t;
-    y_r = r + h * k_r; y_th = th + h * k_th; y_pr = pr + h * k_pr; y_pth = pth + h * k_pth;
-    dsincos(y_th, &st, &ct);
+    y_r = r + h * k_r; y_pr = pr + h * k_pr; y_pth = pth + h * k_pth;
+    if constexpr (SHIFT) sincos_shift(th, h * k_th, st0, ct0, &st, &ct);
+    else dsincos(th + h * k_th, &st, &ct);
     kerr_rhs_sc<R, T_OUT>(y_r, y_pr, y_pth, st, ct, sc, k, k_r, k_th, k_ph, k_pr, k_pth, &k_t);
     a_r += k_r; a_th += k_th; a_ph += k_ph; a_pr += k_pr; a_pth += k_pth;
     const R h6 = h * R(1.0 / 6.0);
@@ -220,11 +257,11 @@ __device__ __forceinline__ bool kerr_adaptive_step(R& r, R& th, R& ph, R& pr, R&
     dsincos(th, &st, &ct);
     const R h = h_cur * kerr_pole_scale(st);
     R br = r, bth = th, bph = ph, bpr = pr, bpt = pth;
-    kerr_rk4_raw(br, bth, bph, bpr, bpt, h, st, ct, sc, k);
+    kerr_rk4_raw<R, false, false>(br, bth, bph, bpr, bpt, h, st, ct, sc, k);
     R hr = r, hth = th, hph = ph, hpr = pr, hpt = pth;
-    kerr_rk4_raw(hr, hth, hph, hpr, hpt, R(0.5) * h, st, ct, sc, k);
+    kerr_rk4_raw<R, false, false>(hr, hth, hph, hpr, hpt, R(0.5) * h, st, ct, sc, k);
     dsincos(hth, &st, &ct);
-    kerr_rk4_raw(hr, hth, hph, hpr, hpt, R(0.5) * h, st, ct, sc, k);
+    kerr_rk4_raw<R, false, false>(hr, hth, hph, hpr, hpt, R(0.5) * h, st, ct, sc, k);
     const R er = (hr - br) / (R(1) + (hr < R(0) ? -hr : hr));
     const R et = hth - bth, ep = hph - bph;
     const R aer = er < R(0) ? -er : er, aet = et < R(0) ? -et : et, aep = ep < R(0) ? -ep : ep;

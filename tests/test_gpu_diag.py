@@ -8,7 +8,7 @@ Bars (stated here):
   every ray except those that wind near the photon orbit (the oracle's own
   count changes when the impact parameters move by 5e-6 relative; fewer than
   1 % of rays), totals within 1e-5 (Schwarzschild); Kerr rays that fall in
-  may also differ (RK4 stages at Delta -> 0), at most 0.5 % of the fan.
+  may also differ (RK4 stages at Delta -> 0), at most 1 % of the fan.
 * photon orbit: same rows (step column) as the driver's CSV; r, theta, phi, t
   within 1e-4 relative of the driver's float values on the stable and
   one-period cases; the reference harness's bars (verification/
@@ -96,7 +96,7 @@ def test_fan_per_ray_counts_equal_oracle(be, orc, metric, spin):
         # captured in one of the two runs; every other ray agrees exactly.
         captured = (per < steps) | (o_per < steps)
         assert np.all(sens[diff] | captured[diff])
-        assert diff.size <= 0.005 * n
+        assert diff.size <= 0.01 * n
 
 
 def test_fan_argument_errors(be):

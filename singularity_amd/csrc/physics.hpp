@@ -119,7 +119,18 @@ __device__ __forceinline__ float dmin(float a, float b) { return fminf(a, b); }
 __device__ __forceinline__ double dmin(double a, double b) { return fmin(a, b); }
 // Reciprocal on the RHS hot path: v_rcp_f32 (1 ulp) in FP32, exact division in FP64.
 __device__ __forceinline__ float drcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ double drcp(double x) { return 1.0 / x; }
+// Double reciprocal: the hardware estimate refined by two Newton steps
+// (e = 1 - x r, r += r e; the error squares each step), 5 instructions
+// instead of the ~10 of an IEEE division.  Used for 1/Sigma, 1/Delta,
+// 1/sin^2 and the Schwarzschild 1/r, 1/f, 1/sin, whose arguments stay finite
+// and far from zero and overflow on the path.
+__device__ __forceinline__ double drcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
 template <class R> __device__ __forceinline__ R dclamp(R x, R lo, R hi) { return dmin(dmax(x, lo), hi); }
 
 // ---- Kerr Hamiltonian RHS (kerr_hamilton.h:97-155) ----------------------------

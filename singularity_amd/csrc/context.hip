@@ -168,6 +168,7 @@ SingContext* sing_create(uint32_t width, uint32_t height, int device) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->d_ctl, sizeof(sing::QueueCtl));
     if (e == hipSuccess) e = hipMalloc(&c->d_halton, (1024 + sing::kGridTableLen) * sizeof(float2));
+    if (e == hipSuccess) e = hipMalloc(&c->d_sd, sizeof(sing::SceneConst<double>));
     if (e != hipSuccess) {
         sing_set_global_error(std::string("sing_create: HIP setup failed: ") + hipGetErrorString(e));
         sing_destroy(c);
@@ -200,6 +201,7 @@ void sing_destroy(SingContext* c) {
     c->free_frame();
     hipFree(c->d_ctl);
     hipFree(c->d_halton);
+    hipFree(c->d_sd);
     hipFree(c->d_rec);
     hipFree(c->d_band);
     c->free_diff();
@@ -281,6 +283,10 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
 
     if ((e = hipMemsetAsync(c->d_ctl, 0, sizeof(sing::QueueCtl), c->stream)) != hipSuccess)
         return c->hip_fail(e, "hipMemsetAsync(queue)");
+    c->h_sd = sd;
+    if ((e = hipMemcpyAsync(c->d_sd, &c->h_sd, sizeof(sd), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+        return c->hip_fail(e, "hipMemcpyAsync(scene constants)");
+    ch.sd = c->d_sd;
     hipEvent_t* ev = c->events.data();
     for (std::uint32_t k = 0; k < n_chunks; ++k) {
         ch.s0 = k * S;

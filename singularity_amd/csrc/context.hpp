@@ -38,6 +38,8 @@ struct SingContext {
     std::size_t rec_capacity = 0;
     sing::QueueCtl* d_ctl = nullptr;
     float2* d_halton = nullptr;
+    sing::SceneConst<double>* d_sd = nullptr;  // device copy of the frame's FP64 scene constants
+    sing::SceneConst<double> h_sd{};           // its host source (lives until the copy has run)
     std::uint32_t outputs_valid = 0;  // SING_OUT_* of the last frame
     std::string err;
     // differentiable render

@@ -86,7 +86,8 @@ __global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneC
                         bool started;
                         bool deferred = false;
                         if (F64) {
-                            started = init_ray<double, S, KERR>(y, sd, gx, gy, double(ox), double(oy), (bool*)nullptr);
+                            started = init_ray<double, S, KERR>(y, *ch.sd, gx, gy, double(ox), double(oy),
+                                                                (bool*)nullptr);
                         } else if (PASS == 1) {
                             bool hard = false;
                             started = init_ray<float, S, KERR>(y, sf, gx, gy, ox, oy, &hard);

@@ -62,6 +62,10 @@ struct Chunk {
     std::uint32_t s0, S;        // first sample and sample count of the chunk
     std::uint32_t P;            // pixels
     std::uint32_t tiles_x, n_tiles;
+    // the FP64 passes' scene constants in device memory: a ray's set-up reads
+    // them where it runs instead of holding ~90 scalar registers of kernel
+    // arguments live across the persistent loop
+    const SceneConst<double>* sd;
 };
 
 struct LaunchShape {

@@ -200,17 +200,9 @@ __global__ __launch_bounds__(256) void trace_batch_kernel(const BatchScene* __re
         }
 
         // (3) integrate a batch of steps
-        std::uint32_t term = kTermLive;
-        for (std::uint32_t k = 0; k < step_batch; ++k) {
-            if (live) {
-                const std::uint32_t t = F64 ? advance<double, S, KERR, false, ADAPT>(y, bs.sd, &lane_rej)
-                                            : advance<float, S, KERR, PASS == 1, ADAPT>(y, bs.sf, &lane_rej);
-                if (t != kTermLive) {
-                    live = false;
-                    term = t;
-                }
-            }
-        }
+        std::uint32_t term;
+        if constexpr (F64) term = run_steps<double, S, KERR, false, ADAPT>(y, bs.sd, live, step_batch, &lane_rej);
+        else term = run_steps<float, S, KERR, PASS == 1, ADAPT>(y, bs.sf, live, step_batch, &lane_rej);
 
         // (4) emit finished rays; in the mixed pass, ambiguous outcomes go to FP64
         if (term != kTermLive) {

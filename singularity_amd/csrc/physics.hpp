@@ -6,8 +6,8 @@
 // render/cuda/kernels/geodesic_kernel.cu:77-403); the arithmetic is arranged
 // for the CDNA4 vector ALU:
 //   * the integrated time coordinate t is dropped (no output depends on it);
-//   * each RHS evaluates 3 reciprocals (1/Sigma, 1/Delta, 1/sin^2) instead of
-//     13 divides, and per-ray products of E and L_z are formed once per step;
+//   * each RHS evaluates one reciprocal (1/(Sigma Delta sin^2)) instead of
+//     13 divides, and per-ray products of E and L_z are formed once per ray;
 //   * sin/cos of theta for the pole damping are reused by the first RK4 stage.
 #pragma once
 
@@ -73,31 +73,42 @@ __device__ __forceinline__ void dsincos(double x, double* s, double* c) {
     *c = ((q + 1) & 2) ? -cc : cc;
 }
 // sin and cos of th0 + d from s0 = sin th0 and c0 = cos th0 by the angle-
-// addition formulas, with sin d and cos d from the same Taylor polynomials as
-// dsincos (accurate for |d| <= pi/4).  An RK4 stage moves theta by a small d,
-// so this replaces the stage's reduction and quadrant selection by 15
-// FMA-class instructions; a larger |d| takes dsincos.  In double the stages
-// keep the full dsincos (the FP64 pass is a small share of a frame).
+// addition formulas.  An RK4 stage moves theta by a small d (|d| > 0.125 in
+// about 3 of 10^4 stages of the a = 0.94 headline scene), so for |d| <= 0.125
+// sin d and cos d come from short Taylor polynomials and the stage needs no
+// reduction or quadrant selection; a larger |d| takes dsincos.  Truncation on
+// |d| <= 0.125: FP32 sin to d^5 (next term d^7/7! < 1e-10), cos to d^4 (next
+// d^6/6! < 5.3e-9, 0.09 ulp); FP64 sin to d^11 (next d^13/13! < 3e-22), cos
+// to d^10 (next d^12/12! < 3e-20).
 __device__ __forceinline__ void sincos_shift(float th0, float d, float s0, float c0, float* s, float* c) {
-    if (__builtin_expect(fabsf(d) <= 0.785398163f, 1)) {
+    if (__builtin_expect(fabsf(d) <= 0.125f, 1)) {
         const float d2 = d * d;
-        float ps = fmaf(d2, 2.7557319e-06f, -1.9841270e-04f);
-        ps = fmaf(d2, ps, 8.3333338e-03f);
-        ps = fmaf(d2, ps, -1.6666667e-01f);
-        const float sd = fmaf(d * d2, ps, d);
-        float pc = fmaf(d2, -2.7557320e-07f, 2.4801588e-05f);
-        pc = fmaf(d2, pc, -1.3888889e-03f);
-        pc = fmaf(d2, pc, 4.1666668e-02f);
-        pc = fmaf(d2, pc, -0.5f);
-        const float cd = fmaf(d2, pc, 1.0f);
+        const float sd = fmaf(d * d2, fmaf(d2, 8.3333333e-03f, -1.6666667e-01f), d);
+        const float cd = fmaf(d2, fmaf(d2, 4.1666668e-02f, -0.5f), 1.0f);
         *s = fmaf(s0, cd, c0 * sd);
         *c = fmaf(c0, cd, -(s0 * sd));
     } else {
         dsincos(th0 + d, s, c);
     }
 }
-__device__ __forceinline__ void sincos_shift(double th0, double d, double, double, double* s, double* c) {
-    dsincos(th0 + d, s, c);
+__device__ __forceinline__ void sincos_shift(double th0, double d, double s0, double c0, double* s, double* c) {
+    if (__builtin_expect(fabs(d) <= 0.125, 1)) {
+        const double d2 = d * d;
+        double ps = fma(d2, 2.505210838544172e-08, -2.755731922398589e-06);
+        ps = fma(d2, ps, 1.984126984126984e-04);
+        ps = fma(d2, ps, -8.333333333333333e-03);
+        ps = fma(d2, ps, 1.6666666666666666e-01);
+        const double sd = fma(-(d * d2), ps, d);
+        double pc = fma(d2, -2.755731922398589e-07, 2.48015873015873e-05);
+        pc = fma(d2, pc, -1.388888888888889e-03);
+        pc = fma(d2, pc, 4.1666666666666664e-02);
+        pc = fma(d2, pc, -0.5);
+        const double cd = fma(d2, pc, 1.0);
+        *s = fma(s0, cd, c0 * sd);
+        *c = fma(c0, cd, -(s0 * sd));
+    } else {
+        dsincos(th0 + d, s, c);
+    }
 }
 __device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
@@ -121,9 +132,9 @@ __device__ __forceinline__ double dmin(double a, double b) { return fmin(a, b); 
 __device__ __forceinline__ float drcp(float x) { return __builtin_amdgcn_rcpf(x); }
 // Double reciprocal: the hardware estimate refined by two Newton steps
 // (e = 1 - x r, r += r e; the error squares each step), 5 instructions
-// instead of the ~10 of an IEEE division.  Used for 1/Sigma, 1/Delta,
-// 1/sin^2 and the Schwarzschild 1/r, 1/f, 1/sin, whose arguments stay finite
-// and far from zero and overflow on the path.
+// instead of the ~10 of an IEEE division.  Used for the Kerr RHS's
+// 1/(Sigma Delta sin^2) and the Schwarzschild 1/r, 1/f, 1/sin, whose
+// arguments stay finite and far from zero and overflow on the path.
 __device__ __forceinline__ double drcp(double x) {
     double r = __builtin_amdgcn_rcp(x);
     double e = fma(-x, r, 1.0);
@@ -132,14 +143,99 @@ __device__ __forceinline__ double drcp(double x) {
     return fma(r, e, r);
 }
 template <class R> __device__ __forceinline__ R dclamp(R x, R lo, R hi) { return dmin(dmax(x, lo), hi); }
+// max(x, lo) for x = sin^2 theta >= 0 and lo > 0.  Non-negative floats order
+// like their bit patterns, so in FP32 this is one integer max (fmaxf would
+// first canonicalise its operand).
+__device__ __forceinline__ float pos_max(float x, float lo) {
+    return __uint_as_float(max(__float_as_uint(x), __float_as_uint(lo)));
+}
+__device__ __forceinline__ double pos_max(double x, double lo) { return fmax(x, lo); }
+// min(x, y) of two non-negative values (one integer min in FP32, as pos_max).
+__device__ __forceinline__ float pos_min(float x, float y) {
+    return __uint_as_float(min(__float_as_uint(x), __float_as_uint(y)));
+}
+__device__ __forceinline__ double pos_min(double x, double y) { return fmin(x, y); }
+__device__ __forceinline__ float dabs(float x) { return fabsf(x); }
+__device__ __forceinline__ double dabs(double x) { return fabs(x); }
 
 // ---- Kerr Hamiltonian RHS (kerr_hamilton.h:97-155) ----------------------------
 // State (r, theta, phi, p_r, p_theta); st/ct = sin/cos(theta).
 template <class R> struct KerrK {
-    R E, EE, L, L2, cB1, cB0, cphi, mhcB1;  // E, E^2, L_z, L_z^2, 4 M a E L_z, a^2 L_z^2, 2 M a E, -2 M a E L_z
+    R E, L, L2, aE, a2E2, aL, twoE, c2aEL;  // E, L_z, L_z^2, a E, a^2 E^2, a L_z, 2 E, -2 a E L_z
 };
 template <class R> __device__ __forceinline__ KerrK<R> kerr_k(const SceneConst<R>& sc, R E, R L) {
     KerrK<R> k;
+    k.E = E;
+    k.L = L;
+    k.L2 = L * L;
+    k.aE = sc.a * E;
+    k.a2E2 = k.aE * k.aE;
+    k.aL = sc.a * L;
+    k.twoE = E + E;
+    k.c2aEL = R(-2) * k.aE * L;
+    return k;
+}
+
+// The reference's Hamiltonian is 2H = B/(Sigma Delta) + L^2/(Sigma s2) +
+// (Delta p_r^2 + p_th^2)/Sigma with B = -A E^2 + 4 M r a E L - a^2 L^2,
+// A = (r^2+a^2)^2 - Delta a^2 sin^2, s2 = max(sin^2, 1e-10)
+// (kerr_hamilton.h:67-94, 121-152).  Since r^2 + a^2 - Delta = 2 M r, the same
+// function is 2H = F / Sigma with Carter's separated numerator
+//   F = Delta p_r^2 + p_th^2 - P^2/Delta + L^2/s2 - 2 a E L + a^2 E^2 sin^2,
+//   P = (r^2 + a^2) E - a L,
+// and Hamilton's equations (dq/dl = dH/dp, dp/dl = -dH/dq, H = F/(2 Sigma)):
+//   dr/dl    = Delta p_r / Sigma,          dth/dl = p_th / Sigma,
+//   dphi/dl  = (a P/Delta + L/s2 - a E) / Sigma,
+//   dp_r/dl  = (2 r E P/Delta - (r - M)(p_r^2 + P^2/Delta^2) + r F/Sigma) / Sigma,
+//   dp_th/dl = sin cos ((L^2/s2^2 - a^2 E^2) - a^2 F/Sigma) / Sigma
+// — the reference's partials of the same 2H, clamp included (its dphi/dl
+// numerator has sin^2 where this has s2: the two differ only below
+// |sin| = 1e-5, by 1e-10 relative).  44 operations and one reciprocal per
+// evaluation instead of the direct form's ~57 and three.
+// T_OUT: also dt/dl = (A E - 2 M a r L_z)/(Sigma Delta) = ((r^2+a^2) P/Delta +
+// a L - a^2 E sin^2)/Sigma into *dt (the renderer does not track t; the
+// photon-orbit diagnostic does).
+template <class R, bool T_OUT = false>
+__device__ __forceinline__ void kerr_rhs_sc(R r, R pr, R pth, R st, R ct, const SceneConst<R>& sc, const KerrK<R>& k,
+                                            R& dr, R& dth, R& dph, R& dpr, R& dpth, R* dt = nullptr) {
+    const R a2 = sc.a2;
+    const R s2 = st * st;
+    const R safe = pos_max(s2, R(1e-10));
+    const R r2a2 = r * r + a2;
+    const R Sig = r2a2 - a2 * s2;
+    const R Del = r * (r - sc.twoM) + a2;
+    const R P = r2a2 * k.E - k.aL;
+    // one reciprocal: q = 1/(Sigma Delta s2), then 1/Sigma, 1/Delta and 1/s2
+    // by multiplying back
+    const R SD = Sig * Del;
+    const R q = drcp(SD * safe);
+    const R iS = q * (Del * safe);
+    const R iD = q * (Sig * safe);
+    const R u = q * SD;
+    const R PiD = P * iD;
+    const R pr2 = pr * pr;
+    const R F = ((Del * pr2 + pth * pth) - P * PiD) + (k.L2 * u + k.c2aEL) + k.a2E2 * s2;
+    dr = Del * pr * iS;
+    dth = pth * iS;
+    dph = (sc.a * PiD + (k.L * u - k.aE)) * iS;
+    if constexpr (T_OUT) *dt = (r2a2 * PiD + (k.aL - sc.a * k.aE * s2)) * iS;
+    const R FiS = F * iS;
+    const R rmM = r - sc.M;
+    dpr = (((k.twoE * r) * PiD - rmM * (PiD * PiD + pr2)) + r * FiS) * iS;
+    dpth = ((st * ct) * iS) * ((k.L2 * (u * u) - k.a2E2) - a2 * FiS);
+}
+
+// The reference's own grouping of the same RHS (kerr_hamilton.h:121-152:
+// 2H = B/(Sigma Delta) + L^2/(Sigma s2) + (Delta p_r^2 + p_th^2)/Sigma and its
+// quotient-rule partials), with 1/Sigma, 1/Delta, 1/s2 as three reciprocals.
+// Used by the CPU driver's diagnostics (orbit, ray fan), whose fixed-step
+// rays near capture are chaotic: step counts there depend on rounding, and
+// this grouping reproduces the driver's totals.
+template <class R> struct KerrKDirect {
+    R E, EE, L, L2, cB1, cB0, cphi, mhcB1;  // E, E^2, L_z, L_z^2, 4 M a E L_z, a^2 L_z^2, 2 M a E, -2 M a E L_z
+};
+template <class R> __device__ __forceinline__ KerrKDirect<R> kerr_k_direct(const SceneConst<R>& sc, R E, R L) {
+    KerrKDirect<R> k;
     k.E = E;
     k.EE = E * E;
     k.L = L;
@@ -150,22 +246,17 @@ template <class R> __device__ __forceinline__ KerrK<R> kerr_k(const SceneConst<R
     k.mhcB1 = R(-0.5) * k.cB1;
     return k;
 }
-
-// T_OUT: also d t / d lambda = (A E - 2 M a r L_z) / (Sigma Delta) into *dt (the
-// renderer does not track t; the photon-orbit diagnostic does).
 template <class R, bool T_OUT = false>
-__device__ __forceinline__ void kerr_rhs_sc(R r, R pr, R pth, R st, R ct, const SceneConst<R>& sc, const KerrK<R>& k,
-                                            R& dr, R& dth, R& dph, R& dpr, R& dpth, R* dt = nullptr) {
-    // 2H = B/(Sigma Delta) + L^2/(Sigma sin^2) + (Delta p_r^2 + p_th^2)/Sigma,
-    // B = -A E^2 + 4 M r a E L - a^2 L^2  (kerr_hamilton.h:121-152).  The
-    // partials are the reference's, regrouped for fewer instructions: the
-    // -1/2 of dp/dlambda = -(1/2) d(2H)/dq folded into each term, Sigma =
-    // r^2 + a^2 - a^2 sin^2, Delta / (Sigma Delta) = 1/Sigma, and every
-    // d/dtheta term sharing sin(theta) cos(theta) (d Sigma/d theta =
-    // -2 a^2 sin cos, dA/d theta = -2 a^2 Delta sin cos).
+__device__ __forceinline__ void kerr_rhs_sc(R r, R pr, R pth, R st, R ct, const SceneConst<R>& sc,
+                                            const KerrKDirect<R>& k, R& dr, R& dth, R& dph, R& dpr, R& dpth,
+                                            R* dt = nullptr) {
+    // the -1/2 of dp/dl = -(1/2) d(2H)/dq folded into each partial, Sigma =
+    // r^2 + a^2 - a^2 sin^2, Delta/(Sigma Delta) = 1/Sigma, the d/dtheta terms
+    // sharing sin cos (d Sigma/d theta = -2 a^2 sin cos, dA/d theta = -2 a^2
+    // Delta sin cos)
     const R a2 = sc.a2;
     const R s2 = st * st;
-    const R safe = dmax(s2, R(1e-10));
+    const R safe = pos_max(s2, R(1e-10));
     const R r2 = r * r;
     const R as2 = a2 * s2;
     const R r2a2 = r2 + a2;
@@ -198,6 +289,10 @@ template <class R> __device__ __forceinline__ R kerr_pole_scale(R st) {
     const R ps = (st < R(0) ? -st : st) * R(5);
     return (ps > R(1)) ? R(1) : ((ps < R(0.05)) ? R(0.05) : ps);
 }
+// FP32: |sin| * 5 with the abs folded into the multiply, clamped by one med3.
+template <> __device__ __forceinline__ float kerr_pole_scale<float>(float st) {
+    return __builtin_amdgcn_fmed3f(fabsf(st) * 5.0f, 0.05f, 1.0f);
+}
 
 // Classical RK4 step of size h (kerr_hamilton.h:171-177); (st, ct) = sin/cos of
 // the starting theta, reused by the first stage.  T_OUT: t advances too (*t).
@@ -205,9 +300,9 @@ template <class R> __device__ __forceinline__ R kerr_pole_scale(R st) {
 // theta increment (sincos_shift) instead of evaluating sincos afresh; the
 // error-controlled step, which runs three RK4 steps per attempt, keeps the
 // full evaluation (the rotation's fallback branches cost it occupancy).
-template <class R, bool T_OUT = false, bool SHIFT = true>
+template <class R, bool T_OUT = false, bool SHIFT = true, class K = KerrK<R>>
 __device__ __forceinline__ void kerr_rk4_raw(R& r, R& th, R& ph, R& pr, R& pth, R h, R st, R ct,
-                                             const SceneConst<R>& sc, const KerrK<R>& k, R* t = nullptr) {
+                                             const SceneConst<R>& sc, const K& k, R* t = nullptr) {
     const R hh = R(0.5) * h;
     const R st0 = st, ct0 = ct;
     R k_r, k_th, k_ph, k_pr, k_pth, k_t = R(0);
@@ -242,12 +337,13 @@ __device__ __forceinline__ void kerr_rk4_raw(R& r, R& th, R& ph, R& pr, R& pth, 
 }
 
 // The reference's damped step (kerr_hamilton.h:165-178): RK4 with h = hs * pole scale.
-template <class R, bool T_OUT = false>
+// K = KerrK (the renderer's RHS) or KerrKDirect (the reference's grouping).
+template <class R, bool T_OUT = false, class K = KerrK<R>>
 __device__ __forceinline__ void kerr_rk4(R& r, R& th, R& ph, R& pr, R& pth, R hs, const SceneConst<R>& sc,
-                                         const KerrK<R>& k, R* t = nullptr) {
+                                         const K& k, R* t = nullptr) {
     R st, ct;
     dsincos(th, &st, &ct);
-    kerr_rk4_raw<R, T_OUT>(r, th, ph, pr, pth, hs * kerr_pole_scale(st), st, ct, sc, k, t);
+    kerr_rk4_raw<R, T_OUT, true, K>(r, th, ph, pr, pth, hs * kerr_pole_scale(st), st, ct, sc, k, t);
 }
 
 // Error-controlled step by step doubling: one step of h against two of h/2

@@ -118,17 +118,9 @@ __global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneC
         if (__ballot(!retired) == 0ull) break;
 
         // (2) integrate a batch of steps
-        std::uint32_t term = kTermLive;
-        for (std::uint32_t k = 0; k < step_batch; ++k) {
-            if (live) {
-                const std::uint32_t t = F64 ? advance<double, S, KERR, false, ADAPT>(y, sd, &lane_rej)
-                                            : advance<float, S, KERR, PASS == 1, ADAPT>(y, sf, &lane_rej);
-                if (t != kTermLive) {
-                    live = false;
-                    term = t;
-                }
-            }
-        }
+        std::uint32_t term;
+        if constexpr (F64) term = run_steps<double, S, KERR, false, ADAPT>(y, sd, live, step_batch, &lane_rej);
+        else term = run_steps<float, S, KERR, PASS == 1, ADAPT>(y, sf, live, step_batch, &lane_rej);
 
         // (3) emit finished rays; in the mixed pass, hand ambiguous outcomes to FP64
         if (term != kTermLive) {

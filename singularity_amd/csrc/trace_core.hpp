@@ -52,60 +52,73 @@ __device__ __forceinline__ std::uint64_t shfl_u64(std::uint64_t v, int src) {
     return (std::uint64_t(hi) << 32) | lo;
 }
 
+// Per-ray products of E and L_z for the Kerr RHS (E, L_z are constants of
+// the motion, so these are formed once per batch of steps, not per step).
+template <class R, class S, bool KERR>
+__device__ __forceinline__ KerrK<R> ray_k(const RayState<S>& y, const SceneConst<R>& sc) {
+    if constexpr (KERR) return kerr_k(sc, R(y.v2), R(y.v3));
+    else return KerrK<R>{};
+}
+
 // Termination checks in the reference's order (geodesic_kernel.cu:351-401)
-// followed by one RK4 step.  Arithmetic in R, storage in S.  With ADAPT (Kerr)
-// the step is one attempt of the error-controlled step; a rejected attempt
-// leaves the state unchanged (the checks it repeats are idempotent) and
-// counts in *rejected.
+// followed by one RK4 step.  Arithmetic in R, storage in S; k = ray_k(y, sc)
+// (Kerr).  With ADAPT (Kerr) the step is one attempt of the error-controlled
+// step; a rejected attempt leaves the state unchanged (the checks it repeats
+// are idempotent) and counts in *rejected.  The four exit conditions are
+// tested together so that a marching ray takes one branch; which one fired
+// (budget, horizon, escape, then a crossing inside the annulus) is sorted out
+// only on the rare exit path.
 template <class R, class S, bool KERR, bool TRACK = false, bool ADAPT = false>
-__device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneConst<R>& sc,
+__device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneConst<R>& sc, const KerrK<R>& k,
                                                  std::uint32_t* rejected = nullptr) {
-    if (y.steps >= sc.max_steps) return kTermBudget;
     R r = R(y.r), th = R(y.th);
-    if (!__builtin_isfinite(r) || r < sc.cut) return kTermHorizon;
-    if (r > sc.escape_r) {
-        if (ADAPT) {
-            R the = th, phe = R(y.ph);
-            if (y.steps > 0u) {
-                const KerrK<R> k = kerr_k(sc, R(y.v2), R(y.v3));
-                const KerrDense<R> d = kerr_dense(R(y.prev_r), R(y.prev_th), R(y.prev_uph), R(y.prev_v0),
-                                                  R(y.prev_v1), r, th, R(y.ph), R(y.v0), R(y.v1), R(y.hprev), sc, k);
-                kerr_dense_escape(d, sc.escape_r, the, phe);
-            }
-            y.aux0 = S(the);
-            y.aux1 = S(phe);
-        }
-        return kTermEscape;
-    }
     const R mid = R(1.5707963267948966192313216916397514);
-    if (sc.flags & SING_FLAG_DISC_ON) {
-        const R pth = R(y.prev_th);
-        const R dp = pth - mid, dc = th - mid;
-        if (dp * dc < R(0)) {
-            const R frac = (mid - pth) / (th - pth);
-            const R pr0 = R(y.prev_r);
-            R rc = pr0 + frac * (r - pr0);
-            R phc = R(0);
+    const R dp = R(y.prev_th) - mid, dc = th - mid;
+    const bool budget = y.steps >= sc.max_steps;
+    const bool horizon = !__builtin_isfinite(r) || r < sc.cut;
+    const bool escape = r > sc.escape_r;
+    const bool cross = (sc.flags & SING_FLAG_DISC_ON) && dp * dc < R(0);
+    if (__builtin_expect(budget || horizon || escape || cross, false)) {
+        if (budget) return kTermBudget;
+        if (horizon) return kTermHorizon;
+        if (escape) {
             if (ADAPT) {
-                const KerrK<R> k = kerr_k(sc, R(y.v2), R(y.v3));
-                const KerrDense<R> d = kerr_dense(pr0, pth, R(y.prev_uph), R(y.prev_v0), R(y.prev_v1), r, th, R(y.ph),
-                                                  R(y.v0), R(y.v1), R(y.hprev), sc, k);
-                kerr_dense_crossing(d, frac, rc, phc);
-            }
-            if (TRACK) {
-                const R tol = R(kEdgeRel) * rc;
-                const R d_in = rc - sc.r_in, d_out = rc - sc.r_out;
-                if ((d_in < tol && d_in > -tol) || (d_out < tol && d_out > -tol)) y.near_eq = S(0);
-            }
-            if (rc >= sc.r_in && rc <= sc.r_out) {
-                y.aux0 = S(rc);
-                if (ADAPT) y.aux1 = S(phc);
-                if (!KERR) {
-                    const R pu = R(y.prev_uph);
-                    y.aux1 = S(pu + frac * (R(y.v3) - pu));
+                R the = th, phe = R(y.ph);
+                if (y.steps > 0u) {
+                    const KerrDense<R> d = kerr_dense(R(y.prev_r), R(y.prev_th), R(y.prev_uph), R(y.prev_v0),
+                                                      R(y.prev_v1), r, th, R(y.ph), R(y.v0), R(y.v1), R(y.hprev), sc,
+                                                      k);
+                    kerr_dense_escape(d, sc.escape_r, the, phe);
                 }
-                return kTermDisc;
+                y.aux0 = S(the);
+                y.aux1 = S(phe);
             }
+            return kTermEscape;
+        }
+        // a crossing of the equatorial plane (disc_intersection.h:23-55)
+        const R pth = R(y.prev_th);
+        const R frac = (mid - pth) / (th - pth);
+        const R pr0 = R(y.prev_r);
+        R rc = pr0 + frac * (r - pr0);
+        R phc = R(0);
+        if (ADAPT) {
+            const KerrDense<R> d = kerr_dense(pr0, pth, R(y.prev_uph), R(y.prev_v0), R(y.prev_v1), r, th, R(y.ph),
+                                              R(y.v0), R(y.v1), R(y.hprev), sc, k);
+            kerr_dense_crossing(d, frac, rc, phc);
+        }
+        if (TRACK) {
+            const R tol = R(kEdgeRel) * rc;
+            const R d_in = rc - sc.r_in, d_out = rc - sc.r_out;
+            if ((d_in < tol && d_in > -tol) || (d_out < tol && d_out > -tol)) y.near_eq = S(0);
+        }
+        if (rc >= sc.r_in && rc <= sc.r_out) {
+            y.aux0 = S(rc);
+            if (ADAPT) y.aux1 = S(phc);
+            if (!KERR) {
+                const R pu = R(y.prev_uph);
+                y.aux1 = S(pu + frac * (R(y.v3) - pu));
+            }
+            return kTermDisc;
         }
     }
     y.prev_th = y.th;
@@ -113,7 +126,6 @@ __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneCons
     R ph = R(y.ph);
     if (KERR) {
         R pr = R(y.v0), pt = R(y.v1);
-        const KerrK<R> k = kerr_k(sc, R(y.v2), R(y.v3));
         if (ADAPT) {
             R hc = R(y.hcur), hused;
             y.prev_uph = y.ph;
@@ -144,12 +156,27 @@ __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneCons
     y.th = S(th);
     y.ph = S(ph);
     y.steps += 1;
-    if (TRACK) {
-        const R d = th - mid;
-        const R ad = d < R(0) ? -d : d;
-        if (ad < R(y.near_eq)) y.near_eq = S(ad);
-    }
+    if (TRACK) y.near_eq = S(pos_min(R(y.near_eq), dabs(th - mid)));
     return kTermLive;
+}
+
+// Up to n steps of a live ray (advance); returns its termination code, or
+// kTermLive if it is still marching, and clears live when it ends.
+template <class R, class S, bool KERR, bool TRACK, bool ADAPT>
+__device__ __forceinline__ std::uint32_t run_steps(RayState<S>& y, const SceneConst<R>& sc, bool& live,
+                                                   std::uint32_t n, std::uint32_t* rejected) {
+    std::uint32_t term = kTermLive;
+    const KerrK<R> k = ray_k<R, S, KERR>(y, sc);
+    for (std::uint32_t i = 0; i < n; ++i) {
+        if (live) {
+            const std::uint32_t t = advance<R, S, KERR, TRACK, ADAPT>(y, sc, k, rejected);
+            if (t != kTermLive) {
+                live = false;
+                term = t;
+            }
+        }
+    }
+    return term;
 }
 
 // Mixed precision, after the FP32 march: does the outcome need FP64?  (A

@@ -416,7 +416,10 @@ def adam_fit(be: "HipBackend", base: Uniforms, start, target: np.ndarray, cam: D
     steps each gradient entry is clipped to clip x its running RMS: pixels whose
     rays pass near the photon orbit have steep but finite derivatives, and one
     such spike would otherwise inflate Adam's second moment for hundreds of
-    steps.  Returns (params, history of (loss, params, grad)).
+    steps.  Near the optimum those pixels keep the normalised step from
+    vanishing, so the iterates wander within the noise; the result is the
+    evaluated iterate of lowest loss.  Returns (params, history of (loss,
+    params, grad)).
     """
     x = np.array(start, np.float64)
     lr = np.array([0.03, 0.015, 0.075, 0.3, 0.015]) if lr is None else np.asarray(lr, np.float64)
@@ -444,7 +447,8 @@ def adam_fit(be: "HipBackend", base: Uniforms, start, target: np.ndarray, cam: D
         step = sched * lr * (m / (1 - beta1**k)) / (np.sqrt(v / (1 - beta2**k)) + 1e-300)
         x = x - np.where(free, step, 0.0)
         x[0] = min(max(x[0], 0.0), 0.99)
-    return x, history
+    best = min(range(len(history)), key=lambda i: history[i][0])
+    return history[best][1].copy(), history
 
 
 # --- batched render (include/sing_abi.h: sing_render_batch) -------------------

@@ -97,6 +97,27 @@ SING_HD float hmin(float a, float b) { return fminf(a, b); }
 SING_HD double hmin(double a, double b) { return fmin(a, b); }
 SING_HD float hval(float x) { return x; }
 SING_HD double hval(double x) { return x; }
+// Reciprocal of the RHS denominators: on the device the hardware estimate
+// (v_rcp_f32, 1 ulp) in FP32 and the FP64 estimate refined by two Newton
+// steps; the host build (unit tests) divides.
+SING_HD float hrcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+SING_HD double hrcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+#else
+    return 1.0 / x;
+#endif
+}
 
 template <int N, class T> SING_HD Dual<N, T> hsqrt(const Dual<N, T>& x) {
     const T s = hsqrt(x.v);
@@ -173,43 +194,38 @@ template <class T> struct KG {  // adjoints of the constants
     T E, L, a;
 };
 
+// The RHS in Carter's separated form of the same Hamiltonian, 2H = F/Sigma,
+// F = Delta p_r^2 + p_th^2 - P^2/Delta + L^2/s2 - 2 a E L + a^2 E^2 sin^2,
+// P = (r^2 + a^2) E - a L (s2 = max(sin^2, 1e-10); r^2 + a^2 - Delta = 2 M r
+// makes this the reference's B/(Sigma Delta) + L^2/(Sigma s2) + (Delta p_r^2 +
+// p_th^2)/Sigma), with one reciprocal.  kerr_rhs_vjp differentiates the
+// reference's grouping of the same function.
 template <class T> SING_HD KY<T> kerr_rhs(const KY<T>& y, const KC<T>& c) {
     T st, ct;
     hsincos(y.th, &st, &ct);
-    const T twoM = T(2) * c.M;
-    const T a2 = c.a * c.a, EE = c.E * c.E, L2 = c.L * c.L;
-    const T cB1 = T(2) * twoM * c.a * c.E * c.L, cB0 = a2 * L2, cphi = twoM * c.a * c.E;
-    const T s2 = st * st, c2 = ct * ct;
+    const T a = c.a, a2 = a * a, E = c.E, L = c.L;
+    const T aE = a * E;
+    const T s2 = st * st;
     const T safe = s2 < T(1e-10) ? T(1e-10) : s2;
-    const T r = y.r, r2 = r * r;
-    const T Sig = r2 + a2 * c2;
-    const T Del = r2 - twoM * r + a2;
-    const T r2a2 = r2 + a2;
-    const T A = r2a2 * r2a2 - a2 * Del * s2;
-    const T iS = T(1) / Sig, iD = T(1) / Del, iss = T(1) / safe;
-    const T iSD = iS * iD;
+    const T r = y.r;
+    const T r2a2 = r * r + a2;
+    const T Sig = r2a2 - a2 * s2;
+    const T Del = r * (r - T(2) * c.M) + a2;
+    const T P = r2a2 * E - a * L;
+    const T SD = Sig * Del;
+    const T q = hrcp(SD * safe);
+    const T iS = q * (Del * safe), iD = q * (Sig * safe), u = q * SD;
+    const T PiD = P * iD;
+    const T pr2 = y.pr * y.pr;
+    const T F = ((Del * pr2 + y.pt * y.pt) - P * PiD) + (L * L * u - T(2) * aE * L) + aE * aE * s2;
+    const T FiS = F * iS;
+    const T rmM = r - c.M;
     KY<T> d;
     d.r = Del * y.pr * iS;
     d.th = y.pt * iS;
-    const T Nph = cphi * r * s2 + (Del - a2 * s2) * c.L;
-    d.ph = Nph * iSD * iss;
-    const T B = cB1 * r - A * EE - cB0;
-    const T twor = T(2) * r;
-    const T Del_r = twor - twoM;
-    const T A_r = T(2) * twor * r2a2 - a2 * Del_r * s2;
-    const T B_r = cB1 - A_r * EE;
-    const T sin2t = T(2) * st * ct;
-    const T Sig_t = -a2 * sin2t;
-    const T B_t = a2 * Del * sin2t * EE;
-    const T iS2 = iS * iS;
-    const T pr2 = y.pr * y.pr;
-    const T Dp2 = Del * pr2 + y.pt * y.pt;
-    const T X = twor * Del + Sig * Del_r;
-    const T Y = Sig_t * safe + Sig * sin2t;
-    const T dHdr = B_r * iSD - B * X * iSD * iSD - L2 * twor * iS2 * iss + Del_r * pr2 * iS - Dp2 * twor * iS2;
-    const T dHdth = B_t * iSD - B * Sig_t * iS2 * iD - L2 * Y * iS2 * iss * iss - Dp2 * Sig_t * iS2;
-    d.pr = T(-0.5) * dHdr;
-    d.pt = T(-0.5) * dHdth;
+    d.ph = (a * PiD + (L * u - aE)) * iS;
+    d.pr = (((E + E) * r) * PiD - rmM * (PiD * PiD + pr2) + r * FiS) * iS;
+    d.pt = ((st * ct) * iS) * ((L * L * (u * u) - aE * aE) - a2 * FiS);
     return d;
 }
 
@@ -229,7 +245,7 @@ template <class T> SING_HD void kerr_rhs_vjp(const KY<T>& y, const KC<T>& c, con
     const T Del = r2 - twoM * r + a2;
     const T r2a2 = r2 + a2;
     const T A = r2a2 * r2a2 - a2 * Del * s2;
-    const T iS = T(1) / Sig, iD = T(1) / Del, iss = T(1) / safe;
+    const T iS = hrcp(Sig), iD = hrcp(Del), iss = hrcp(safe);
     const T iSD = iS * iD;
     const T Nph = cphi * r * s2 + (Del - a2 * s2) * c.L;
     const T B = cB1 * r - A * EE - cB0;

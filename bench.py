@@ -356,7 +356,6 @@ def main() -> int:
             "traffic": round(FL.TRACE_HBM_BYTES_PER_RAY * rays_per_frame / max(1, int(stats[-1].chunks))),
             "kernel": "trace (FP32 pass + FP64 band pass)",
             "flops_per_step": FL.KERR_STEP_FLOPS, "steps_per_frame": int(steps_total),
-            "frac_of_nonpacked_fp32": round(achieved / FL.PEAK_FP32_NONPACKED_TFLOPS, 4),
         },
         "kernel_ms_per_frame": round(kern_ms, 2), "trace_ms_per_frame": round(trace_ms, 2),
         "shade_ms_per_frame": round(float(np.mean([s.shade_ms for s in stats])), 2),

@@ -36,6 +36,10 @@ struct SingContext {
     sing::Rec* d_rec = nullptr;
     std::uint32_t* d_band = nullptr;
     std::size_t rec_capacity = 0;
+    // per work item of an FP32 pass: the set-up kernel's ray init and record index
+    float4* d_init = nullptr;
+    std::uint32_t* d_item_rec = nullptr;
+    std::size_t item_capacity = 0;
     sing::QueueCtl* d_ctl = nullptr;
     float2* d_halton = nullptr;
     sing::SceneConst<double>* d_sd = nullptr;  // device copy of the frame's FP64 scene constants
@@ -72,6 +76,7 @@ struct SingContext {
     void free_frame();
     int alloc_frame(std::uint32_t w, std::uint32_t h);
     int ensure_records(std::size_t rays);
+    int ensure_items(std::size_t items);
     int ensure_events(std::size_t n);
     std::uint32_t chunk_samples(std::uint32_t spp) const;
     sing::LaunchShape launch_shape(const SingRenderOpts& o) const;

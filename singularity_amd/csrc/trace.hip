@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <type_traits>
 
@@ -52,64 +53,78 @@ __global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneC
     RayState<S> y{};
 
     for (;;) {
-        // (1) claim rays for idle lanes with one atomic per wave
-        const std::uint64_t want = __ballot(need && !retired);
-        if (want) {
-            const int leader = __ffsll((unsigned long long)want) - 1;
-            std::uint64_t base = 0;
-            if (lane == leader) base = atomicAdd((unsigned long long*)counter, (unsigned long long)__popcll(want));
-            base = shfl_u64(base, leader);
-            if (need && !retired) {
-                const std::uint64_t item = base + __popcll(want & ((1ull << lane) - 1ull));
-                if (item >= n_items) {
-                    retired = true;
-                } else {
-                    std::uint32_t gx, gy, s;
-                    bool ok;
-                    if (PASS == 3) {
-                        rec_idx = ch.band[item];
-                        const std::uint32_t sl = rec_idx / ch.P, pix = rec_idx - sl * ch.P;
-                        gx = pix % W;
-                        gy = pix / W;
-                        s = ch.s0 + sl;
-                        ok = true;
+        if constexpr (!F64) {
+            // (1) claim set-up rays for idle lanes with one atomic per wave and
+            // round; an item the set-up kernel resolved (kItemSkip) sends its
+            // lane round again
+            for (int round = 0; round < 4; ++round) {
+                const std::uint64_t want = __ballot(need && !retired);
+                if (!want) break;
+                const int leader = __ffsll((unsigned long long)want) - 1;
+                std::uint64_t base = 0;
+                if (lane == leader) base = atomicAdd((unsigned long long*)counter, (unsigned long long)__popcll(want));
+                base = shfl_u64(base, leader);
+                if (need && !retired) {
+                    const std::uint64_t item = base + __popcll(want & ((1ull << lane) - 1ull));
+                    if (item >= n_items) {
+                        retired = true;
                     } else {
-                        ok = item_to_ray(item, ch, W, H, rec_idx, gx, gy, s);
-                    }
-                    if (ok) {
-                        float ox = 0.5f, oy = 0.5f;
-                        if (spp > 1u) {
-                            const float2 hv = halton[s];
-                            ox = hv.x;
-                            oy = hv.y;
-                        }
-                        bool started;
-                        bool deferred = false;
-                        if (F64) {
-                            started = init_ray<double, S, KERR>(y, *ch.sd, gx, gy, double(ox), double(oy),
-                                                                (bool*)nullptr);
-                        } else if (PASS == 1) {
-                            bool hard = false;
-                            started = init_ray<float, S, KERR>(y, sf, gx, gy, ox, oy, &hard);
-                            deferred = started && hard;
-                        } else {
-                            started = init_ray<float, S, KERR>(y, sf, gx, gy, ox, oy, (bool*)nullptr);
-                        }
-                        need = false;
-                        if (deferred) {
-                            const std::uint32_t slot = atomicAdd(&ctl->band_count, 1u);
-                            ch.band[slot] = rec_idx;
-                            ch.rec[rec_idx] = deferred_marker();
-                            need = true;
-                        } else if (!started) {
-                            Rec rc;
-                            rc.a = rc.b = rc.c = 0.0f;
-                            rc.meta = kTermInitFail;
-                            ch.rec[rec_idx] = rc;
-                            lane_rays += 1;
-                            need = true;
-                        } else {
+                        const std::uint32_t ri = ch.item_rec[item];
+                        if (ri != kItemSkip) {
+                            rec_idx = ri;
+                            start_ray<S, KERR>(y, sf, ch.init[item]);
+                            need = false;
                             live = true;
+                        }
+                    }
+                }
+            }
+        } else {
+            // (1) FP64 passes: claim rays for idle lanes with one atomic per
+            // wave and set them up here, in double
+            const std::uint64_t want = __ballot(need && !retired);
+            if (want) {
+                const int leader = __ffsll((unsigned long long)want) - 1;
+                std::uint64_t base = 0;
+                if (lane == leader) base = atomicAdd((unsigned long long*)counter, (unsigned long long)__popcll(want));
+                base = shfl_u64(base, leader);
+                if (need && !retired) {
+                    const std::uint64_t item = base + __popcll(want & ((1ull << lane) - 1ull));
+                    if (item >= n_items) {
+                        retired = true;
+                    } else {
+                        std::uint32_t gx, gy, s;
+                        bool ok;
+                        if (PASS == 3) {
+                            rec_idx = ch.band[item];
+                            const std::uint32_t sl = rec_idx / ch.P, pix = rec_idx - sl * ch.P;
+                            gx = pix % W;
+                            gy = pix / W;
+                            s = ch.s0 + sl;
+                            ok = true;
+                        } else {
+                            ok = item_to_ray(item, ch, W, H, rec_idx, gx, gy, s);
+                        }
+                        if (ok) {
+                            float ox = 0.5f, oy = 0.5f;
+                            if (spp > 1u) {
+                                const float2 hv = halton[s];
+                                ox = hv.x;
+                                oy = hv.y;
+                            }
+                            const bool started = init_ray<double, S, KERR>(y, *ch.sd, gx, gy, double(ox),
+                                                                           double(oy), (bool*)nullptr);
+                            need = false;
+                            if (!started) {
+                                Rec rc;
+                                rc.a = rc.b = rc.c = 0.0f;
+                                rc.meta = kTermInitFail;
+                                ch.rec[rec_idx] = rc;
+                                lane_rays += 1;
+                                need = true;
+                            } else {
+                                live = true;
+                            }
                         }
                     }
                 }
@@ -151,6 +166,71 @@ __global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneC
         atomicAdd((unsigned long long*)&ctl->rays, (unsigned long long)lane_rays);
         if (F64) atomicAdd((unsigned long long*)&ctl->rays64, (unsigned long long)lane_rays);
     }
+}
+
+// Set-up of an FP32 pass (PASS 0: every ray; 1: mixed, deferring hard rays):
+// one thread per work item computes the camera ray, the initial momenta and
+// constants (init_ray) and, in the mixed pass, whether FP32 is accurate
+// enough for it (hard_ray).  A ray to trace gets its init entry and record
+// index; a ray that fails to start gets its record here and counts as
+// finished; a hard ray is appended to the FP64 list (one atomic per wave).
+// Moving this out of the persistent trace kernel keeps its lane refill to two
+// loads: with the set-up inline, every refill ran the set-up code for the
+// whole wave.
+template <int METRIC, int PASS>
+__global__ __launch_bounds__(256) void setup_kernel(SceneConst<float> sf, const float2* __restrict__ halton, Chunk ch,
+                                                    QueueCtl* __restrict__ ctl) {
+    constexpr bool KERR = (METRIC >= 1);
+    const int lane = __lane_id();
+    const std::uint64_t n_items = std::uint64_t(ch.S) * ch.n_tiles * 64u;
+    const std::uint64_t stride = std::uint64_t(gridDim.x) * 256u;
+    std::uint64_t fails = 0;
+    // every lane of a wave runs the same number of iterations (the bound is
+    // rounded up to whole waves), so the ballot below sees all 64 lanes
+    const std::uint64_t start = std::uint64_t(blockIdx.x) * 256u + threadIdx.x;
+    const std::uint64_t first_lane = start - std::uint64_t(lane);
+    for (std::uint64_t base = first_lane; base < n_items; base += stride) {
+        const std::uint64_t item = base + std::uint64_t(lane);
+        std::uint32_t out = kItemSkip, rec_idx = 0, gx = 0, gy = 0, s = 0;
+        bool hard = false;
+        if (item < n_items && item_to_ray(item, ch, sf.width, sf.height, rec_idx, gx, gy, s)) {
+            float ox = 0.5f, oy = 0.5f;
+            if (sf.spp > 1u) {
+                const float2 hv = halton[s];
+                ox = hv.x;
+                oy = hv.y;
+            }
+            RayState<float> y{};
+            const bool started = init_ray<float, float, KERR>(y, sf, gx, gy, ox, oy, PASS == 1 ? &hard : nullptr);
+            if (!started) {
+                Rec rc;
+                rc.a = rc.b = rc.c = 0.0f;
+                rc.meta = kTermInitFail;
+                ch.rec[rec_idx] = rc;
+                fails += 1;
+                hard = false;
+            } else if (!hard) {
+                ch.init[item] = make_float4(y.v0, y.v1, y.v2, y.v3);
+                out = rec_idx;
+            }
+        }
+        if (PASS == 1) {
+            const std::uint64_t hw = __ballot(hard);
+            if (hw) {
+                const int leader = __ffsll((unsigned long long)hw) - 1;
+                std::uint32_t slot = 0;
+                if (lane == leader) slot = atomicAdd(&ctl->band_count, std::uint32_t(__popcll(hw)));
+                slot = __shfl(slot, leader);
+                if (hard) {
+                    ch.band[slot + __popcll(hw & ((1ull << lane) - 1ull))] = rec_idx;
+                    ch.rec[rec_idx] = deferred_marker();
+                }
+            }
+        }
+        if (item < n_items) ch.item_rec[item] = out;
+    }
+    for (int off = 32; off > 0; off >>= 1) fails += shfl_u64(fails, lane ^ off);
+    if (lane == 0 && fails) atomicAdd((unsigned long long*)&ctl->rays, (unsigned long long)fails);
 }
 
 // Mixed precision: escape records whose direction sits on a starfield cell
@@ -245,7 +325,7 @@ hipError_t launch_trace_chunk(const SceneConst<float>& sf, const SceneConst<doub
                               const Chunk& ch, QueueCtl* ctl, std::uint32_t mode, const LaunchShape& shape,
                               hipStream_t stream) {
     const dim3 block(256);
-    const std::uint32_t sb = shape.step_batch ? shape.step_batch : 8u;
+    const std::uint32_t sb = shape.step_batch ? shape.step_batch : 32u;
     const bool kerr = sf.metric == SING_METRIC_KERR;
     const bool adapt = kerr && sf.tol > 0.0f;
 #define SING_TRACE(P, G)                                                                                  \
@@ -257,17 +337,29 @@ hipError_t launch_trace_chunk(const SceneConst<float>& sf, const SceneConst<doub
         else                                                                                              \
             hipLaunchKernelGGL((trace_kernel<0, P>), dim3(G), block, 0, stream, sf, sd, halton, ch, ctl, sb); \
     } while (0)
+    const std::uint64_t n_items = std::uint64_t(ch.S) * ch.n_tiles * 64u;
+    const dim3 setup_grid(std::uint32_t(std::min<std::uint64_t>((n_items + 255u) / 256u, shape.blocks * 8ull)));
+#define SING_SETUP(P)                                                                                          \
+    do {                                                                                                       \
+        if (kerr)                                                                                              \
+            hipLaunchKernelGGL((setup_kernel<1, P>), setup_grid, block, 0, stream, sf, halton, ch, ctl);        \
+        else                                                                                                   \
+            hipLaunchKernelGGL((setup_kernel<0, P>), setup_grid, block, 0, stream, sf, halton, ch, ctl);        \
+    } while (0)
     if (mode == SING_PREC_FP32) {
+        SING_SETUP(0);
         SING_TRACE(0, shape.blocks);
     } else if (mode == SING_PREC_FP64) {
         SING_TRACE(2, shape.blocks64);
     } else {
+        SING_SETUP(1);
         SING_TRACE(1, shape.blocks);
         if (sf.flags & SING_FLAG_STARFIELD_ON)
             hipLaunchKernelGGL(audit_kernel, dim3(shape.blocks), block, 0, stream, sf, ch, ctl);
         SING_TRACE(3, shape.blocks64);
     }
 #undef SING_TRACE
+#undef SING_SETUP
     return hipGetLastError();
 }
 

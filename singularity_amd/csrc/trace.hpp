@@ -56,9 +56,19 @@ struct QueueCtl {
     std::uint64_t rejected;    // error-controlled stepping: rejected step attempts
 };
 
+// Work item of an FP32 pass that needs no trace (tile padding outside the
+// image, or a ray whose record the set-up kernel already wrote).
+constexpr std::uint32_t kItemSkip = 0xFFFFFFFFu;
+
 struct Chunk {
     Rec* rec;                   // S * P records, index s_local * P + pixel
     std::uint32_t* band;        // deferred record indices (capacity S * P)
+    // FP32 passes, per work item (S * n_tiles * 64): the ray's initial
+    // (v0, v1, v2, v3) of RayState — Kerr (p_r, p_theta, E, L_z),
+    // Schwarzschild (u^t, u^r, u^theta, u^phi) — and its record index, or
+    // kItemSkip; written by the set-up kernel before the trace
+    float4* init;
+    std::uint32_t* item_rec;
     std::uint32_t s0, S;        // first sample and sample count of the chunk
     std::uint32_t P;            // pixels
     std::uint32_t tiles_x, n_tiles;

@@ -24,6 +24,7 @@ template <class S> struct RayState {
     S aux0, aux1;      // disc crossing radius; u^phi (Schwarzschild) or phi (error-controlled Kerr) at it
     S near_eq;         // mixed precision: min |theta - pi/2| over the steps (0 = ambiguous crossing)
     S hcur;            // error-controlled stepping: current step size
+    S sth, cth;        // fixed-step Kerr: sin and cos of th carried from the previous step
     std::uint32_t steps;
     std::uint32_t tries;  // error-controlled stepping: rejections of the current step
 };
@@ -139,7 +140,25 @@ __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneCons
             }
             y.hprev = S(hused);
         } else {
-            kerr_rk4(r, th, ph, pr, pt, sc.h, sc, k);
+            // sin/cos of theta: evaluated in full on every 8th step of the ray
+            // (steps = 0 mod 8, so the result does not depend on the launch
+            // shape), otherwise carried from the previous step, rotated by the
+            // exact difference of the stored angles (sincos_shift); the
+            // rotation error stays below a few ulp between full evaluations.
+            R st, ct;
+            if ((y.steps & 7u) == 0u) {
+                dsincos(th, &st, &ct);
+            } else {
+                st = R(y.sth);
+                ct = R(y.cth);
+            }
+            const R th0 = th;
+            kerr_rk4_raw<R>(r, th, ph, pr, pt, sc.h * kerr_pole_scale(st), st, ct, sc, k);
+            if (((y.steps + 1u) & 7u) != 0u) {
+                sincos_shift(th0, th - th0, st, ct, &st, &ct);
+                y.sth = S(st);
+                y.cth = S(ct);
+            }
         }
         y.v0 = S(pr);
         y.v1 = S(pt);
@@ -243,6 +262,26 @@ __device__ __forceinline__ bool init_ray(RayState<S>& y, const SceneConst<R>& sc
         }
     }
     return true;
+}
+
+// A ray set up by the set-up kernel: the camera position plus the initial
+// (v0, v1, v2, v3) that init_ray computed for it.
+template <class S, bool KERR>
+__device__ __forceinline__ void start_ray(RayState<S>& y, const SceneConst<float>& sc, const float4 v) {
+    y.r = S(sc.cam_r);
+    y.th = S(sc.cam_theta);
+    y.ph = S(sc.cam_phi);
+    y.prev_r = y.r;
+    y.prev_th = y.th;
+    y.steps = 0;
+    y.near_eq = S(1e30);
+    y.hcur = S(sc.h);
+    y.tries = 0u;
+    y.v0 = S(v.x);
+    y.v1 = S(v.y);
+    y.v2 = S(v.z);
+    y.v3 = S(v.w);
+    if (!KERR) y.prev_uph = y.v3;
 }
 
 template <class S, bool KERR, bool ADAPT = false>

@@ -26,7 +26,7 @@ PEAK_HBM_GBS = 8000.0
 POST_BYTES_PER_PIXEL = 32
 
 # Measured HBM traffic of one mixed-precision FP32 trace dispatch at 1920x1080 x
-# 16 spp (33 177 600 rays): rocprofv3 FETCH_SIZE 99.4 KiB x 2.0 + WRITE_SIZE
-# 604647.1 KiB x 1.0 (factors from the 256 MiB copy of singularity_amd/hbm_copy),
+# 16 spp (33 177 600 rays): rocprofv3 FETCH_SIZE 103.4 KiB x 2.0 + WRITE_SIZE
+# 609736.0 KiB x 1.0 (factors from the 256 MiB copy of singularity_amd/hbm_copy),
 # per traced ray: 16-byte termination record, deferral markers and list entries.
-TRACE_HBM_BYTES_PER_RAY = (604647.1 * 1024 + 99.4 * 1024 * 2.0) / 33177600.0
+TRACE_HBM_BYTES_PER_RAY = (609736.0 * 1024 + 103.4 * 1024 * 2.0) / 33177600.0

@@ -220,6 +220,8 @@ __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict
     b.steps[pix] = step;
 }
 
+// MODE 0: every pixel whose forward ran in R's precision; 1: the pixels of
+// b.list; 2: the pixels of b.list, then those of b.list2.
 template <class R, int MODE>
 __global__ __launch_bounds__(256) void diff_bwd_kernel(const DiffSet* __restrict__ ds, DiffBuffers b) {
     const std::uint32_t idx = blockIdx.x * 256u + threadIdx.x;
@@ -227,6 +229,14 @@ __global__ __launch_bounds__(256) void diff_bwd_kernel(const DiffSet* __restrict
     if (MODE == 1) {
         if (idx >= *b.count) return;
         pix = b.list[idx];
+    } else if (MODE == 2) {  // the classified list, then the late list, in one launch
+        const std::uint32_t n1 = *b.count;
+        if (idx < n1) {
+            pix = b.list[idx];
+        } else {
+            if (idx - n1 >= *b.count2) return;
+            pix = b.list2[idx - n1];
+        }
     } else {
         if (idx >= b.P) return;
         pix = idx;
@@ -421,9 +431,11 @@ hipError_t launch_diff(const DiffSet* ds_dev, const DiffBuffers& b, std::uint32_
         (void)hipStreamWaitEvent(stream, ev[2], 0);
         (void)hipEventRecord(ev[0], stream);
         if (backward) {
+            // every FP64 ray (classified and late) in one launch: each launch
+            // lasts as long as its longest ray, so one launch instead of two
+            // in a row halves the FP64 backward's latency floor
             (void)hipStreamWaitEvent(stream2, ev[0], 0);
-            hipLaunchKernelGGL((diff_bwd_kernel<double, 1>), grid, block, 0, stream2, ds_dev, b);
-            hipLaunchKernelGGL((diff_bwd_kernel<double, 1>), grid, block, 0, stream2, ds_dev, late);
+            hipLaunchKernelGGL((diff_bwd_kernel<double, 2>), grid, block, 0, stream2, ds_dev, b);
             (void)hipEventRecord(ev[3], stream2);
             hipLaunchKernelGGL((diff_bwd_kernel<float, 0>), grid, block, 0, stream, ds_dev, b);
             (void)hipStreamWaitEvent(stream, ev[3], 0);

@@ -1,11 +1,13 @@
 // singularity_amd/csrc/valu_rate.hip — VALU issue-rate calibration for the
 // trace kernels' roofline.  Each kernel runs 8 independent dependency chains
-// per lane of one instruction class (FP32 FMA, FP32 multiply, FP32 reciprocal,
-// FP64 FMA) or alternating between two (multiply and FMA), over a grid of 16
-// four-wave blocks per CU, with no memory traffic in the loop, and reports
-// lane-instructions per second and per CU per shader clock.  The shader clock is read in block 0 as the ratio of clock64() ticks
-// to the 100 MHz constant wall clock over the kernel.  A result of 64 lanes
-// per CU-clock means one wave64 instruction per CU per clock.
+// per lane of one instruction class (FP32 FMA and multiply with scalar,
+// vector-register or literal operands, FP32 reciprocal, FP64 FMA) or alternating between
+// two (multiply and FMA), over a grid of 16 four-wave blocks per CU, with no
+// memory traffic in the loop, and reports lane-instructions per second and per
+// CU per shader clock.  The shader clock is read in block 0 as the ratio of
+// clock64() ticks to the 100 MHz constant wall clock over the kernel.  A
+// result of 64 lanes per CU-clock means one wave64 instruction per CU per
+// clock.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -18,6 +20,15 @@ constexpr int kIters = 4096;
 
 __device__ unsigned long long g_ticks[2];
 
+// A register copy the optimiser cannot see through, so that per-lane operands
+// stay in vector registers and are not folded into constants.
+__device__ __forceinline__ float opaque(float x) {
+    float r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+__device__ __forceinline__ double opaque(double x) { return x; }
+
 template <class T, int OP>
 __global__ __launch_bounds__(256) void rate_kernel(T* __restrict__ out, T a, T b) {
     unsigned long long c0 = 0, w0 = 0;
@@ -25,14 +36,22 @@ __global__ __launch_bounds__(256) void rate_kernel(T* __restrict__ out, T a, T b
         c0 = clock64();
         w0 = wall_clock64();
     }
-    T x[kChains];
-    for (int c = 0; c < kChains; ++c) x[c] = a + T(threadIdx.x + c);
+    T x[kChains], yv[kChains], zv[kChains];
+    for (int c = 0; c < kChains; ++c) {
+        x[c] = a + T(threadIdx.x + c);
+        yv[c] = opaque(a - T(1e-4) * T((threadIdx.x + c) & 7));  // per-lane operands held in VGPRs
+        zv[c] = opaque(b * T(1 + ((threadIdx.x + c) & 3)));
+    }
     for (int i = 0; i < kIters; ++i) {
 #pragma unroll
         for (int c = 0; c < kChains; ++c) {
             if constexpr (OP == 0) x[c] = x[c] * a + b;  // FMA
             if constexpr (OP == 1) x[c] = x[c] * a;      // multiply
             if constexpr (OP == 2) x[c] = __builtin_amdgcn_rcpf(x[c]);
+            if constexpr (OP == 4) x[c] = x[c] * zv[c] + x[c];  // FMA, VGPR operands (v_fmac form)
+            if constexpr (OP == 5) x[c] = x[c] * yv[c];          // multiply, VGPR operands
+            if constexpr (OP == 6) x[c] = x[c] * yv[c] + T(0.1234567);  // FMA, VGPR operands and a literal
+            if constexpr (OP == 7) x[c] = x[c] * yv[c] + zv[c];  // FMA, three VGPR operands (VOP3)
             if constexpr (OP == 3) {  // independent multiply and FMA chains, alternating
                 if (c & 1) x[c] = x[c] * a;
                 else x[c] = x[c] * b + a;
@@ -96,6 +115,10 @@ int main() {
                     run<float, 1>(buf, blocks, cus, wall_khz, "fp32 mul") &&
                     run<float, 2>(buf, blocks, cus, wall_khz, "fp32 rcp") &&
                     run<float, 3>(buf, blocks, cus, wall_khz, "mul|fma mix") &&
+                    run<float, 4>(buf, blocks, cus, wall_khz, "fp32 fmac v") &&
+                    run<float, 5>(buf, blocks, cus, wall_khz, "fp32 mul v") &&
+                    run<float, 6>(buf, blocks, cus, wall_khz, "fp32 fma vk") &&
+                    run<float, 7>(buf, blocks, cus, wall_khz, "fp32 fma vvv") &&
                     run<double, 0>(buf, blocks, cus, wall_khz, "fp64 fma");
     (void)hipFree(buf);
     if (!ok) {

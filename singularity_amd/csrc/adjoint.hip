@@ -376,26 +376,44 @@ __global__ __launch_bounds__(256) void diff_bwd_kernel(const DiffSet* __restrict
     g[4] = double(g_ts);
 }
 
-// Fixed-order reduction of per-pixel loss (1) and gradients (5) into out[6].
-__global__ __launch_bounds__(1024) void diff_reduce_kernel(const double* __restrict__ loss,
-                                                           const double* __restrict__ grad, std::uint32_t P,
-                                                           double* __restrict__ out) {
-    __shared__ double sh[6][1024];
+// Loss and gradient sums in a fixed order (bit-reproducible, independent of
+// scheduling): stage 1, kReduceBlocks blocks each sum a contiguous slice of
+// pixels (a block-stride loop, then a tree in LDS); stage 2, one block sums
+// the per-block partials in block order.
+__global__ __launch_bounds__(256) void diff_reduce_partial_kernel(const double* __restrict__ loss,
+                                                                  const double* __restrict__ grad, std::uint32_t P,
+                                                                  double* __restrict__ partial) {
+    __shared__ double sh[6][256];
     const std::uint32_t t = threadIdx.x;
+    const std::uint32_t per = (P + kReduceBlocks - 1u) / kReduceBlocks;
+    const std::uint32_t lo = blockIdx.x * per, hi = min(P, lo + per);
     double s[6] = {0, 0, 0, 0, 0, 0};
-    for (std::uint32_t i = t; i < P; i += 1024u) {
+    for (std::uint32_t i = lo + t; i < hi; i += 256u) {
         s[0] += loss[i];
         for (int k = 0; k < 5; ++k) s[1 + k] += grad[5u * i + k];
     }
     for (int k = 0; k < 6; ++k) sh[k][t] = s[k];
     __syncthreads();
-    for (std::uint32_t w = 512; w > 0; w >>= 1) {
+    for (std::uint32_t w = 128; w > 0; w >>= 1) {
         if (t < w)
             for (int k = 0; k < 6; ++k) sh[k][t] += sh[k][t + w];
         __syncthreads();
     }
-    if (t == 0)
-        for (int k = 0; k < 6; ++k) out[k] = sh[k][0];
+    if (t < 6u) partial[6u * blockIdx.x + t] = sh[t][0];
+}
+
+__global__ __launch_bounds__(256) void diff_reduce_final_kernel(const double* __restrict__ partial,
+                                                                double* __restrict__ out) {
+    __shared__ double sh[6][kReduceBlocks];
+    const std::uint32_t t = threadIdx.x;
+    for (int k = 0; k < 6; ++k) sh[k][t] = partial[6u * t + k];
+    __syncthreads();
+    for (std::uint32_t w = kReduceBlocks / 2u; w > 0; w >>= 1) {
+        if (t < w)
+            for (int k = 0; k < 6; ++k) sh[k][t] += sh[k][t + w];
+        __syncthreads();
+    }
+    if (t < 6u) out[t] = sh[t][0];
 }
 
 }  // namespace
@@ -441,7 +459,9 @@ hipError_t launch_diff(const DiffSet* ds_dev, const DiffBuffers& b, std::uint32_
             (void)hipStreamWaitEvent(stream, ev[3], 0);
         }
     }
-    hipLaunchKernelGGL(diff_reduce_kernel, dim3(1), dim3(1024), 0, stream, b.loss, b.grad, b.P, out6);
+    hipLaunchKernelGGL(diff_reduce_partial_kernel, dim3(kReduceBlocks), dim3(256), 0, stream, b.loss, b.grad, b.P,
+                       b.partial);
+    hipLaunchKernelGGL(diff_reduce_final_kernel, dim3(1), dim3(kReduceBlocks), 0, stream, b.partial, out6);
     return hipGetLastError();
 }
 
@@ -465,6 +485,7 @@ void SingContext::free_diff() {
     (void)hipFree(diff.list2);
     (void)hipFree(diff.count2);
     (void)hipFree(diff.steps);
+    (void)hipFree(diff.partial);
     (void)hipFree(d_target);
     (void)hipFree(d_weight);
     (void)hipFree(d_out6);
@@ -498,6 +519,7 @@ int SingContext::ensure_diff(std::size_t P, std::size_t ncp, std::size_t C) {
     A((void**)&diff.list2, P * sizeof(std::uint32_t));
     A((void**)&diff.count2, sizeof(std::uint32_t));
     A((void**)&diff.steps, P * sizeof(std::uint32_t));
+    A((void**)&diff.partial, sing::kReduceBlocks * 6 * sizeof(double));
     A((void**)&d_target, P * 3 * sizeof(float));
     A((void**)&d_weight, P * sizeof(float));
     A((void**)&d_out6, 6 * sizeof(double));

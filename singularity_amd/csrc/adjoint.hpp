@@ -21,6 +21,9 @@ struct DiffSet {
 };
 
 // Per-pixel device buffers of one differentiable frame.
+// Blocks of the first stage of the loss/gradient reduction.
+constexpr std::uint32_t kReduceBlocks = 256;
+
 struct DiffBuffers {
     double* cp;             // [ncp][5][P] states every C steps (double storage)
     double* cross;          // [8][7][P] step, alpha, emission rgb, r_cross, phi
@@ -39,6 +42,7 @@ struct DiffBuffers {
     std::uint32_t* list2;   // pixels handed to FP64 after the FP32 march (grazing, near-hole emission)
     std::uint32_t* count2;  // length of list2
     std::uint32_t* steps;   // [P] RK4 steps of the forward
+    double* partial;        // [kReduceBlocks][6] per-block sums of loss and gradient
     std::uint32_t P, ncp, C;
 };
 

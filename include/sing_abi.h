@@ -306,8 +306,8 @@ typedef struct SingBatchOpts {
                                of the same layout (sizes, sample counts, metrics, options) */
     uint32_t autotune;      /* 1: time launch-shape candidates on a 1/16 sample of this batch and keep the
                                fastest for this (metric, precision, stepping) in the context */
-    uint32_t step_batch;    /* RK4 steps between lane refills (0 = tuned or 8) */
-    uint32_t waves_per_cu;  /* persistent waves per CU (0 = tuned or 16) */
+    uint32_t step_batch;    /* RK4 steps between lane refills (0 = tuned or 32) */
+    uint32_t waves_per_cu;  /* persistent waves per CU (0 = tuned or 24) */
     float adapt_tol;        /* > 0: error-controlled stepping (applies to the Kerr scenes) */
     uint32_t turb_seed;     /* seeded disc-band phases, as SingRenderOpts */
     uint32_t record_budget_mib; /* device memory for one group's termination records (0 = 32 GiB); scenes are

@@ -318,7 +318,7 @@ hipError_t launch_batch_trace(const BatchLaunch& L, const BatchGroup& g, std::ui
                               const LaunchShape& shape, std::uint32_t sample_div, bool starfield_any,
                               hipStream_t stream) {
     const dim3 block(256);
-    const std::uint32_t sb = shape.step_batch ? shape.step_batch : 8u;
+    const std::uint32_t sb = shape.step_batch ? shape.step_batch : 32u;
     const std::uint32_t n_kerr = g.count - g.n_schw, k0 = g.first + g.n_schw;
 #define SING_BTRACE(P, G)                                                                                       \
     do {                                                                                                        \
@@ -580,16 +580,16 @@ extern "C" int sing_render_batch(SingContext* c, const SingBatchItem* items, uin
             return r == hipSuccess ? SING_OK : c->hip_fail(r, "batch autotune");
         };
         float best = 0.0f, ms = 0.0f;
-        std::uint32_t best_sb = sb ? sb : 8u, best_w = wpc ? wpc : 16u;
+        std::uint32_t best_sb = sb ? sb : 32u, best_w = wpc ? wpc : 24u;
         if ((rc = time_shape(best_sb, best_w, &best)) != SING_OK) return rc;  // also warms the code up
         if ((rc = time_shape(best_sb, best_w, &best)) != SING_OK) return rc;
         if (!o.waves_per_cu)
-            for (std::uint32_t w_c : {8u, 12u, 24u}) {
+            for (std::uint32_t w_c : {12u, 16u, 32u}) {
                 if ((rc = time_shape(best_sb, w_c, &ms)) != SING_OK) return rc;
                 if (ms < best) best = ms, best_w = w_c;
             }
         if (!o.step_batch)
-            for (std::uint32_t s_b : {4u, 16u, 32u}) {
+            for (std::uint32_t s_b : {8u, 16u, 64u}) {
                 if ((rc = time_shape(s_b, best_w, &ms)) != SING_OK) return rc;
                 if (ms < best) best = ms, best_sb = s_b;
             }
@@ -600,8 +600,8 @@ extern "C" int sing_render_batch(SingContext* c, const SingBatchItem* items, uin
             if (kv.first == tkey) kv.second = (std::uint64_t(sb) << 32) | wpc, stored = true;
         if (!stored) c->b_tuned.push_back({tkey, (std::uint64_t(sb) << 32) | wpc});
     }
-    if (!sb) sb = 8u;
-    if (!wpc) wpc = 16u;
+    if (!sb) sb = 32u;
+    if (!wpc) wpc = 24u;
     const sing::LaunchShape shape = shape_of(c, sb, wpc);
 
     // graph key: everything the recorded launches depend on

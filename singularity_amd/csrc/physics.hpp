@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "scene_const.hpp"
 
@@ -297,9 +298,7 @@ template <> __device__ __forceinline__ float kerr_pole_scale<float>(float st) {
 // Classical RK4 step of size h (kerr_hamilton.h:171-177); (st, ct) = sin/cos of
 // the starting theta, reused by the first stage.  T_OUT: t advances too (*t).
 // SHIFT: stages 2-4 rotate (sin, cos) of the starting theta by the stage's
-// theta increment (sincos_shift) instead of evaluating sincos afresh; the
-// error-controlled step, which runs three RK4 steps per attempt, keeps the
-// full evaluation (the rotation's fallback branches cost it occupancy).
+// theta increment (sincos_shift) instead of evaluating sincos afresh.
 template <class R, bool T_OUT = false, bool SHIFT = true, class K = KerrK<R>>
 __device__ __forceinline__ void kerr_rk4_raw(R& r, R& th, R& ph, R& pr, R& pth, R h, R st, R ct,
                                              const SceneConst<R>& sc, const K& k, R* t = nullptr) {
@@ -363,12 +362,16 @@ __device__ __forceinline__ bool kerr_adaptive_step(R& r, R& th, R& ph, R& pr, R&
     R st, ct;
     dsincos(th, &st, &ct);
     const R h = h_cur * kerr_pole_scale(st);
+    // FP32: stage sin/cos by rotation; FP64 keeps the full evaluation (the
+    // rotation's fallback branches would push the FP64 kernel past 256 VGPRs)
+    constexpr bool SH = std::is_same<R, float>::value;
     R br = r, bth = th, bph = ph, bpr = pr, bpt = pth;
-    kerr_rk4_raw<R, false, false>(br, bth, bph, bpr, bpt, h, st, ct, sc, k);
+    kerr_rk4_raw<R, false, SH>(br, bth, bph, bpr, bpt, h, st, ct, sc, k);
     R hr = r, hth = th, hph = ph, hpr = pr, hpt = pth;
-    kerr_rk4_raw<R, false, false>(hr, hth, hph, hpr, hpt, R(0.5) * h, st, ct, sc, k);
-    dsincos(hth, &st, &ct);
-    kerr_rk4_raw<R, false, false>(hr, hth, hph, hpr, hpt, R(0.5) * h, st, ct, sc, k);
+    kerr_rk4_raw<R, false, SH>(hr, hth, hph, hpr, hpt, R(0.5) * h, st, ct, sc, k);
+    if constexpr (SH) sincos_shift(th, hth - th, st, ct, &st, &ct);  // the start's, rotated by the exact difference
+    else dsincos(hth, &st, &ct);
+    kerr_rk4_raw<R, false, SH>(hr, hth, hph, hpr, hpt, R(0.5) * h, st, ct, sc, k);
     const R er = (hr - br) / (R(1) + (hr < R(0) ? -hr : hr));
     const R et = hth - bth, ep = hph - bph;
     const R aer = er < R(0) ? -er : er, aet = et < R(0) ? -et : et, aep = ep < R(0) ? -ep : ep;

@@ -145,7 +145,7 @@ def test_autotune_picks_a_candidate_and_keeps_outputs(be):
     scenes = scenes_mixed()
     ref, _ = sa.render_batch(be, scenes, sa.BatchOpts(precision=sa.PREC_FP32), hdr=True)
     frames, st = sa.render_batch(be, scenes, sa.BatchOpts(precision=sa.PREC_FP32, autotune=1), hdr=True)
-    assert st.step_batch in (4, 8, 16, 32) and st.waves_per_cu in (8, 12, 16, 24)
+    assert st.step_batch in (8, 16, 32, 64) and st.waves_per_cu in (12, 16, 24, 32)
     for f, r in zip(frames, ref):
         np.testing.assert_array_equal(f.rgba, r.rgba)
         np.testing.assert_array_equal(f.hdr, r.hdr)

@@ -109,6 +109,20 @@ def test_frames_are_bit_reproducible(backends):
     assert a.stats.steps == b.stats.steps and a.stats.fp64_rays == b.stats.fp64_rays
 
 
+@pytest.mark.parametrize("prec", [sa.PREC_FP32, sa.PREC_MIXED])
+def test_frames_do_not_depend_on_launch_shape(backends, prec):
+    """A ray's arithmetic depends only on its own state: the full sin/cos
+    evaluation of the fixed step comes every 8th step of the ray, not at lane
+    refills, so step batch and grid size leave the frame bit-identical."""
+    u = config3(192, 108, 3)
+    ref = render(backends, u, prec, step_batch=32, waves_per_cu=24)
+    for sb, wpc in ((8, 16), (5, 12), (64, 32)):
+        f = render(backends, u, prec, step_batch=sb, waves_per_cu=wpc)
+        for k in ("rgba", "hdr", "cls", "steps"):
+            np.testing.assert_array_equal(getattr(f, k), getattr(ref, k), err_msg=f"{k} sb={sb} wpc={wpc}")
+        assert f.stats.steps == ref.stats.steps
+
+
 def test_many_samples_match_oracle_small_image(backends, orc):
     """spp up to the 1024 cap: Halton jitter, per-sample ACES mean, chunked accumulation."""
     u = scene_uniforms(spin=0.94, r_in=1.7, r_out=30.0, width=8, height=6, spp=1024)

@@ -409,10 +409,13 @@ def diff_loss_grad(be: "HipBackend", base: Uniforms, params: DiffParams, cam: Di
 
 def adam_fit(be: "HipBackend", base: Uniforms, start, target: np.ndarray, cam: DiffCamera, steps: int = 200,
              lr=None, free=None, opts: DiffOpts | None = None, beta1: float = 0.9, beta2: float = 0.999,
-             clip: float = 2.0, lr_floor: float = 0.01, callback=None):
+             clip: float = 2.0, lr_floor: float = 0.003, callback=None):
     """Adam over the 5 scene parameters (entries with free == False stay fixed).
 
-    The learning rate follows a cosine decay from lr to lr_floor * lr.  After 5
+    The learning rate follows a cosine decay from lr to lr_floor * lr (the
+    floor of 0.003 lets the last steps settle within the noise of the
+    photon-orbit pixels described below; 0.01 left the 512x512 Kerr fit
+    wandering at ~100x the loss on some starts).  After 5
     steps each gradient entry is clipped to clip x its running RMS: pixels whose
     rays pass near the photon orbit have steep but finite derivatives, and one
     such spike would otherwise inflate Adam's second moment for hundreds of

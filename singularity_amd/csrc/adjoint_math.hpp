@@ -182,6 +182,64 @@ template <class T> SING_HD void hsincos(T x, T* s, T* c) {
     *c = ((q + 1) & 2) ? -cc : cc;
 }
 
+// FP32: the render path's float reduction (two-term pi/2) and polynomials
+// (float truncation), instead of the double-length ones above.
+SING_HD void hsincos(float x, float* s, float* c) {
+    const float k = rintf(x * 0.636619772367581343f);
+    float r = fmaf(-k, 1.5707963705062866f, x);
+    r = fmaf(-k, -4.371138828673793e-08f, r);
+    const float r2 = r * r;
+    float ps = fmaf(r2, 2.7557319e-06f, -1.9841270e-04f);
+    ps = fmaf(r2, ps, 8.3333338e-03f);
+    ps = fmaf(r2, ps, -1.6666667e-01f);
+    const float sr = fmaf(r * r2, ps, r);
+    float pc = fmaf(r2, -2.7557320e-07f, 2.4801588e-05f);
+    pc = fmaf(r2, pc, -1.3888889e-03f);
+    pc = fmaf(r2, pc, 4.1666668e-02f);
+    pc = fmaf(r2, pc, -0.5f);
+    const float cr = fmaf(r2, pc, 1.0f);
+    const int q = int(k);
+    const float ss = (q & 1) ? cr : sr;
+    const float cc = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -ss : ss;
+    *c = ((q + 1) & 2) ? -cc : cc;
+}
+
+// sin/cos of th0 + d from (s0, c0) = sin/cos th0 by angle addition, with
+// Taylor polynomials of sin d and cos d for |d| <= 0.125 (FP32 to d^5 / d^4,
+// FP64 to d^11 / d^10: truncation below 0.1 ulp); a larger |d| evaluates
+// sin/cos of th0 + d in full.  Stages 2-4 of a step use it.
+SING_HD void hsincos_shift(float th0, float d, float s0, float c0, float* s, float* c) {
+    if (fabsf(d) <= 0.125f) {
+        const float d2 = d * d;
+        const float sd = fmaf(d * d2, fmaf(d2, 8.3333333e-03f, -1.6666667e-01f), d);
+        const float cd = fmaf(d2, fmaf(d2, 4.1666668e-02f, -0.5f), 1.0f);
+        *s = fmaf(s0, cd, c0 * sd);
+        *c = fmaf(c0, cd, -(s0 * sd));
+    } else {
+        hsincos(th0 + d, s, c);
+    }
+}
+SING_HD void hsincos_shift(double th0, double d, double s0, double c0, double* s, double* c) {
+    if (fabs(d) <= 0.125) {
+        const double d2 = d * d;
+        double ps = fma(d2, 2.505210838544172e-08, -2.755731922398589e-06);
+        ps = fma(d2, ps, 1.984126984126984e-04);
+        ps = fma(d2, ps, -8.333333333333333e-03);
+        ps = fma(d2, ps, 1.6666666666666666e-01);
+        const double sd = fma(-(d * d2), ps, d);
+        double pc = fma(d2, -2.755731922398589e-07, 2.48015873015873e-05);
+        pc = fma(d2, pc, -1.388888888888889e-03);
+        pc = fma(d2, pc, 4.1666666666666664e-02);
+        pc = fma(d2, pc, -0.5);
+        const double cd = fma(d2, pc, 1.0);
+        *s = fma(s0, cd, c0 * sd);
+        *c = fma(c0, cd, -(s0 * sd));
+    } else {
+        hsincos(th0 + d, s, c);
+    }
+}
+
 // ------------------------------------------------------------- Kerr RHS + VJP
 // State y = (r, theta, phi, p_r, p_theta); constants (E, L_z, a) and M.
 template <class T> struct KY {
@@ -200,9 +258,7 @@ template <class T> struct KG {  // adjoints of the constants
 // makes this the reference's B/(Sigma Delta) + L^2/(Sigma s2) + (Delta p_r^2 +
 // p_th^2)/Sigma), with one reciprocal.  kerr_rhs_vjp differentiates the
 // reference's grouping of the same function.
-template <class T> SING_HD KY<T> kerr_rhs(const KY<T>& y, const KC<T>& c) {
-    T st, ct;
-    hsincos(y.th, &st, &ct);
+template <class T> SING_HD KY<T> kerr_rhs_sc(const KY<T>& y, const KC<T>& c, T st, T ct) {
     const T a = c.a, a2 = a * a, E = c.E, L = c.L;
     const T aE = a * E;
     const T s2 = st * st;
@@ -227,6 +283,11 @@ template <class T> SING_HD KY<T> kerr_rhs(const KY<T>& y, const KC<T>& c) {
     d.pr = (((E + E) * r) * PiD - rmM * (PiD * PiD + pr2) + r * FiS) * iS;
     d.pt = ((st * ct) * iS) * ((L * L * (u * u) - aE * aE) - a2 * FiS);
     return d;
+}
+template <class T> SING_HD KY<T> kerr_rhs(const KY<T>& y, const KC<T>& c) {
+    T st, ct;
+    hsincos(y.th, &st, &ct);
+    return kerr_rhs_sc(y, c, st, ct);
 }
 
 // v^T (d rhs / d(y, E, L, a)): accumulates into gy (no phi dependence) and gc.
@@ -423,9 +484,7 @@ template <class T> SING_HD T ky_dot(const KY<T>& a, const KY<T>& b) {
 }
 
 // Pole damping h = hs * clamp(5 |sin theta|, 0.05, 1) (kerr_hamilton.h:166-170) and d h / d theta.
-template <class T> SING_HD T pole_step(T th, T hs, T* dh_dth) {
-    T st, ct;
-    hsincos(th, &st, &ct);
+template <class T> SING_HD T pole_step_sc(T st, T ct, T hs, T* dh_dth) {
     const T as = st < T(0) ? -st : st;
     const T ps = as * T(5);
     if (ps > T(1)) {
@@ -439,15 +498,34 @@ template <class T> SING_HD T pole_step(T th, T hs, T* dh_dth) {
     *dh_dth = hs * T(5) * (st < T(0) ? -ct : ct);
     return hs * ps;
 }
+template <class T> SING_HD T pole_step(T th, T hs, T* dh_dth) {
+    T st, ct;
+    hsincos(th, &st, &ct);
+    return pole_step_sc(st, ct, hs, dh_dth);
+}
 
+// sin/cos of a stage's theta th0 + d: FP32 rotates (s0, c0) (hsincos_shift);
+// FP64 evaluates in full (the rotation's registers push the FP64 backward
+// kernel, which recomputes steps inside the VJP sweep, into more spills).
+SING_HD void stage_sincos(float th0, float d, float s0, float c0, float* s, float* c) {
+    hsincos_shift(th0, d, s0, c0, s, c);
+}
+SING_HD void stage_sincos(double th0, double d, double, double, double* s, double* c) { hsincos(th0 + d, s, c); }
+
+// The damped RK4 step.  sin/cos of theta are evaluated once, shared by the
+// pole damping and the first stage.
 template <class T> SING_HD KY<T> kerr_step(const KY<T>& y, const KC<T>& c, T hs) {
-    T dh;
-    const T h = pole_step(y.th, hs, &dh);
+    T dh, st, ct, sn, cs;
+    hsincos(y.th, &st, &ct);
+    const T h = pole_step_sc(st, ct, hs, &dh);
     const T hh = T(0.5) * h;
-    const KY<T> k1 = kerr_rhs(y, c);
-    const KY<T> k2 = kerr_rhs(ky_axpy(y, k1, hh), c);
-    const KY<T> k3 = kerr_rhs(ky_axpy(y, k2, hh), c);
-    const KY<T> k4 = kerr_rhs(ky_axpy(y, k3, h), c);
+    const KY<T> k1 = kerr_rhs_sc(y, c, st, ct);
+    stage_sincos(y.th, hh * k1.th, st, ct, &sn, &cs);
+    const KY<T> k2 = kerr_rhs_sc(ky_axpy(y, k1, hh), c, sn, cs);
+    stage_sincos(y.th, hh * k2.th, st, ct, &sn, &cs);
+    const KY<T> k3 = kerr_rhs_sc(ky_axpy(y, k2, hh), c, sn, cs);
+    stage_sincos(y.th, h * k3.th, st, ct, &sn, &cs);
+    const KY<T> k4 = kerr_rhs_sc(ky_axpy(y, k3, h), c, sn, cs);
     KY<T> s;
     s.r = k1.r + T(2) * k2.r + T(2) * k3.r + k4.r;
     s.th = k1.th + T(2) * k2.th + T(2) * k3.th + k4.th;

@@ -26,7 +26,9 @@ PEAK_HBM_GBS = 8000.0
 POST_BYTES_PER_PIXEL = 32
 
 # Measured HBM traffic of one mixed-precision FP32 trace dispatch at 1920x1080 x
-# 16 spp (33 177 600 rays): rocprofv3 FETCH_SIZE 103.4 KiB x 2.0 + WRITE_SIZE
-# 609736.0 KiB x 1.0 (factors from the 256 MiB copy of singularity_amd/hbm_copy),
-# per traced ray: 16-byte termination record, deferral markers and list entries.
-TRACE_HBM_BYTES_PER_RAY = (609736.0 * 1024 + 103.4 * 1024 * 2.0) / 33177600.0
+# 16 spp (33 177 600 rays): rocprofv3 FETCH_SIZE 376424 KiB x 2.0 + WRITE_SIZE
+# 522196 KiB x 1.0 (factors from the 256 MiB copy of singularity_amd/hbm_copy),
+# per traced ray: the set-up kernel's 16-byte init entry and 4-byte record index
+# read, the 16-byte termination record, deferral markers and list entries
+# written.
+TRACE_HBM_BYTES_PER_RAY = (522196.0 * 1024 + 376424.0 * 1024 * 2.0) / 33177600.0

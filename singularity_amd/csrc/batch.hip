@@ -81,8 +81,14 @@ __global__ __launch_bounds__(256) void batch_reset_kernel(SceneQueue* __restrict
 // scenes' deferred lists.  Scenes [first, first + count) of the table.
 // The scene table is its own read-only, non-aliased parameter so that, with a
 // wave-uniform scene index, its fields are scalar loads (as kernel arguments are).
+// The fixed-step Kerr FP32 passes are held to 80 VGPRs (6 waves per SIMD, as
+// the single-scene trace): the inline ray set-up would otherwise take 104 (5
+// waves); what that spills is reloaded in the claim code, once per step batch,
+// not in the step loop.  The fixed-step Kerr FP64 passes are held to 168
+// (3 waves per SIMD).
 template <int METRIC, int PASS>
-__global__ __launch_bounds__(256) void trace_batch_kernel(const BatchScene* __restrict__ scenes, BatchLaunch L,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(METRIC != 1 ? 1 : PASS < 2 ? 6 : 3)))
+void trace_batch_kernel(const BatchScene* __restrict__ scenes, BatchLaunch L,
                                                           std::uint32_t first, std::uint32_t count,
                                                           std::uint32_t step_batch, std::uint32_t sample_div) {
     constexpr bool KERR = (METRIC >= 1);

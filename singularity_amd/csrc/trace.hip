@@ -32,8 +32,11 @@ using namespace core;
 // METRIC 0: Schwarzschild; 1: Kerr; 2: Kerr with error-controlled steps.
 // PASS 0: FP32 trace of every ray; 1: FP32 trace deferring photon-band rays;
 // 2: FP64 trace of every ray; 3: FP64 trace of the deferred list.
+// The fixed-step Kerr FP64 passes are held to 168 VGPRs, 3 waves per SIMD (the
+// compiler's own choice, 170, leaves room for 2), without spills.
 template <int METRIC, int PASS>
-__global__ __launch_bounds__(256) void trace_kernel(SceneConst<float> sf, SceneConst<double> sd,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((METRIC == 1 && PASS >= 2) ? 3 : 1)))
+void trace_kernel(SceneConst<float> sf, SceneConst<double> sd,
                                                     const float2* __restrict__ halton, Chunk ch,
                                                     QueueCtl* __restrict__ ctl, std::uint32_t step_batch) {
     constexpr bool KERR = (METRIC >= 1);

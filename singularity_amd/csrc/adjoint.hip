@@ -380,55 +380,66 @@ __global__ __launch_bounds__(256) void diff_bwd_kernel(const DiffSet* __restrict
 // scheduling): stage 1, kReduceBlocks blocks each sum a contiguous slice of
 // pixels (a block-stride loop, then a tree in LDS); stage 2, one block sums
 // the per-block partials in block order.
+// The step total is a sum of integers below 2^53, exact in double.
 __global__ __launch_bounds__(256) void diff_reduce_partial_kernel(const double* __restrict__ loss,
-                                                                  const double* __restrict__ grad, std::uint32_t P,
-                                                                  double* __restrict__ partial) {
-    __shared__ double sh[6][256];
+                                                                  const double* __restrict__ grad,
+                                                                  const std::uint32_t* __restrict__ steps,
+                                                                  std::uint32_t P, double* __restrict__ partial) {
+    constexpr int V = int(kReduceVals);
+    __shared__ double sh[V][256];
     const std::uint32_t t = threadIdx.x;
     const std::uint32_t per = (P + kReduceBlocks - 1u) / kReduceBlocks;
     const std::uint32_t lo = blockIdx.x * per, hi = min(P, lo + per);
-    double s[6] = {0, 0, 0, 0, 0, 0};
+    double s[V] = {0, 0, 0, 0, 0, 0, 0};
     for (std::uint32_t i = lo + t; i < hi; i += 256u) {
         s[0] += loss[i];
         for (int k = 0; k < 5; ++k) s[1 + k] += grad[5u * i + k];
+        s[6] += double(steps[i]);
     }
-    for (int k = 0; k < 6; ++k) sh[k][t] = s[k];
+    for (int k = 0; k < V; ++k) sh[k][t] = s[k];
     __syncthreads();
     for (std::uint32_t w = 128; w > 0; w >>= 1) {
         if (t < w)
-            for (int k = 0; k < 6; ++k) sh[k][t] += sh[k][t + w];
+            for (int k = 0; k < V; ++k) sh[k][t] += sh[k][t + w];
         __syncthreads();
     }
-    if (t < 6u) partial[6u * blockIdx.x + t] = sh[t][0];
+    if (t < kReduceVals) partial[kReduceVals * blockIdx.x + t] = sh[t][0];
 }
 
+// out[0..kReduceVals): the sums; out[kReduceVals]: the FP64 list's length
+// (0 without a list), so one copy brings back every per-evaluation result.
 __global__ __launch_bounds__(256) void diff_reduce_final_kernel(const double* __restrict__ partial,
+                                                                const std::uint32_t* __restrict__ count,
                                                                 double* __restrict__ out) {
-    __shared__ double sh[6][kReduceBlocks];
+    constexpr int V = int(kReduceVals);
+    __shared__ double sh[V][kReduceBlocks];
     const std::uint32_t t = threadIdx.x;
-    for (int k = 0; k < 6; ++k) sh[k][t] = partial[6u * t + k];
+    for (int k = 0; k < V; ++k) sh[k][t] = partial[kReduceVals * t + k];
     __syncthreads();
     for (std::uint32_t w = kReduceBlocks / 2u; w > 0; w >>= 1) {
         if (t < w)
-            for (int k = 0; k < 6; ++k) sh[k][t] += sh[k][t + w];
+            for (int k = 0; k < V; ++k) sh[k][t] += sh[k][t + w];
         __syncthreads();
     }
-    if (t < 6u) out[t] = sh[t][0];
+    if (t < kReduceVals) out[t] = sh[t][0];
+    if (t == kReduceVals) out[t] = count ? double(*count) : 0.0;
 }
 
 }  // namespace
 
 hipError_t launch_diff(const DiffSet* ds_dev, const DiffBuffers& b, std::uint32_t precision, bool backward,
-                       double* out6, hipStream_t stream, hipStream_t stream2, const hipEvent_t* ev) {
-    // ev[0]: end of the forward (recorded on `stream`); ev[1..3]: cross-stream joins
+                       double* out, hipStream_t stream, hipStream_t stream2, hipEvent_t fwd_end,
+                       const hipEvent_t* join) {
+    // fwd_end: timing event at the end of the forward (on `stream`); join[0..3]:
+    // cross-stream joins, created without timing (they only order work)
     const dim3 block(256), grid((b.P + 255u) / 256u);
     if (precision == SING_PREC_FP32) {
         hipLaunchKernelGGL((diff_fwd_kernel<float, 0>), grid, block, 0, stream, ds_dev, b);
-        (void)hipEventRecord(ev[0], stream);
+        (void)hipEventRecord(fwd_end, stream);
         if (backward) hipLaunchKernelGGL((diff_bwd_kernel<float, 0>), grid, block, 0, stream, ds_dev, b);
     } else if (precision == SING_PREC_FP64) {
         hipLaunchKernelGGL((diff_fwd_kernel<double, 2>), grid, block, 0, stream, ds_dev, b);
-        (void)hipEventRecord(ev[0], stream);
+        (void)hipEventRecord(fwd_end, stream);
         if (backward) hipLaunchKernelGGL((diff_bwd_kernel<double, 0>), grid, block, 0, stream, ds_dev, b);
     } else {
         // Mixed: the hard rays (classified first) run in FP64 on stream2 while the
@@ -440,28 +451,30 @@ hipError_t launch_diff(const DiffSet* ds_dev, const DiffBuffers& b, std::uint32_
         late.list = b.list2;
         late.count = b.count2;
         hipLaunchKernelGGL(diff_classify_kernel, grid, block, 0, stream, ds_dev, b);
-        (void)hipEventRecord(ev[1], stream);
-        (void)hipStreamWaitEvent(stream2, ev[1], 0);
+        (void)hipEventRecord(join[0], stream);
+        (void)hipStreamWaitEvent(stream2, join[0], 0);
         hipLaunchKernelGGL((diff_fwd_kernel<double, 3>), grid, block, 0, stream2, ds_dev, b);
-        (void)hipEventRecord(ev[2], stream2);
+        (void)hipEventRecord(join[1], stream2);
         hipLaunchKernelGGL((diff_fwd_kernel<float, 1>), grid, block, 0, stream, ds_dev, b);
         hipLaunchKernelGGL((diff_fwd_kernel<double, 3>), grid, block, 0, stream, ds_dev, late);
-        (void)hipStreamWaitEvent(stream, ev[2], 0);
-        (void)hipEventRecord(ev[0], stream);
+        (void)hipStreamWaitEvent(stream, join[1], 0);
+        (void)hipEventRecord(fwd_end, stream);
         if (backward) {
             // every FP64 ray (classified and late) in one launch: each launch
             // lasts as long as its longest ray, so one launch instead of two
             // in a row halves the FP64 backward's latency floor
-            (void)hipStreamWaitEvent(stream2, ev[0], 0);
+            (void)hipEventRecord(join[2], stream);
+            (void)hipStreamWaitEvent(stream2, join[2], 0);
             hipLaunchKernelGGL((diff_bwd_kernel<double, 2>), grid, block, 0, stream2, ds_dev, b);
-            (void)hipEventRecord(ev[3], stream2);
+            (void)hipEventRecord(join[3], stream2);
             hipLaunchKernelGGL((diff_bwd_kernel<float, 0>), grid, block, 0, stream, ds_dev, b);
-            (void)hipStreamWaitEvent(stream, ev[3], 0);
+            (void)hipStreamWaitEvent(stream, join[3], 0);
         }
     }
-    hipLaunchKernelGGL(diff_reduce_partial_kernel, dim3(kReduceBlocks), dim3(256), 0, stream, b.loss, b.grad, b.P,
-                       b.partial);
-    hipLaunchKernelGGL(diff_reduce_final_kernel, dim3(1), dim3(kReduceBlocks), 0, stream, b.partial, out6);
+    hipLaunchKernelGGL(diff_reduce_partial_kernel, dim3(kReduceBlocks), dim3(256), 0, stream, b.loss, b.grad,
+                       b.steps, b.P, b.partial);
+    hipLaunchKernelGGL(diff_reduce_final_kernel, dim3(1), dim3(kReduceBlocks), 0, stream, b.partial,
+                       precision == SING_PREC_MIXED ? b.count : nullptr, out);
     return hipGetLastError();
 }
 
@@ -488,11 +501,13 @@ void SingContext::free_diff() {
     (void)hipFree(diff.partial);
     (void)hipFree(d_target);
     (void)hipFree(d_weight);
-    (void)hipFree(d_out6);
+    (void)hipFree(d_out);
+    if (h_out) (void)hipHostFree(h_out);
     d_dset = nullptr;
     diff = sing::DiffBuffers{};
     d_target = d_weight = nullptr;
-    d_out6 = nullptr;
+    d_out = nullptr;
+    h_out = nullptr;
     diff_P = diff_ncp = diff_C = 0;
 }
 
@@ -519,10 +534,12 @@ int SingContext::ensure_diff(std::size_t P, std::size_t ncp, std::size_t C) {
     A((void**)&diff.list2, P * sizeof(std::uint32_t));
     A((void**)&diff.count2, sizeof(std::uint32_t));
     A((void**)&diff.steps, P * sizeof(std::uint32_t));
-    A((void**)&diff.partial, sing::kReduceBlocks * 6 * sizeof(double));
+    A((void**)&diff.partial, sing::kReduceBlocks * sing::kReduceVals * sizeof(double));
     A((void**)&d_target, P * 3 * sizeof(float));
     A((void**)&d_weight, P * sizeof(float));
-    A((void**)&d_out6, 6 * sizeof(double));
+    A((void**)&d_out, sing::kDiffOut * sizeof(double));
+    // pinned, so the per-evaluation result copy is an ordinary stream operation
+    if (e == hipSuccess) e = hipHostMalloc((void**)&h_out, sing::kDiffOut * sizeof(double), hipHostMallocDefault);
     if (e != hipSuccess) {
         free_diff();
         return fail(SING_ERR_NOMEM, "hipMalloc of differentiable-render buffers failed: %s", hipGetErrorString(e));
@@ -564,6 +581,7 @@ int diff_run(SingContext* c, const SingUniforms* base, const SingDiffParams* p, 
     int rc = c->ensure_diff(P, ncp, C);
     if (rc != SING_OK) return rc;
     if ((rc = c->ensure_events(6)) != SING_OK) return rc;
+    if ((rc = c->ensure_join_events(4)) != SING_OK) return rc;
 
     // r_in: given, or the prograde ISCO of the spin with its derivative
     double r_in = p->r_in, drin_da = 0.0;
@@ -638,31 +656,25 @@ int diff_run(SingContext* c, const SingUniforms* base, const SingDiffParams* p, 
     b.target = c->d_target;
     b.weight = weights ? c->d_weight : nullptr;
     (void)hipEventRecord(c->events[0], c->stream);
-    if ((e = sing::launch_diff(c->d_dset, b, o.precision, backward, c->d_out6, c->stream, c->stream2,
-                               &c->events[1])) !=
+    if ((e = sing::launch_diff(c->d_dset, b, o.precision, backward, c->d_out, c->stream, c->stream2,
+                               c->events[1], c->join_events.data())) !=
         hipSuccess)
         return c->hip_fail(e, "differentiable render launch");
     (void)hipEventRecord(c->events[5], c->stream);
-    double out6[6] = {0, 0, 0, 0, 0, 0};
-    std::uint32_t n64 = 0;
-    if ((e = hipMemcpyAsync(out6, c->d_out6, sizeof(out6), hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(&n64, c->diff.count, sizeof(n64), hipMemcpyDeviceToHost, c->stream)) != hipSuccess)
+    // loss, gradient, step total and FP64 list length in one copy to pinned
+    // memory (the step total is reduced on the device, not copied per pixel)
+    if ((e = hipMemcpyAsync(c->h_out, c->d_out, sing::kDiffOut * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream)) != hipSuccess)
         return c->hip_fail(e, "hipMemcpyAsync(result)");
     if (hdr_out &&
         (e = hipMemcpyAsync(hdr_out, c->diff.hdr, P * 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream)) !=
             hipSuccess)
         return c->hip_fail(e, "hipMemcpyAsync(hdr)");
-    std::vector<std::uint32_t> steps;
-    if (stats) {
-        steps.resize(P);
-        if ((e = hipMemcpyAsync(steps.data(), c->diff.steps, P * sizeof(std::uint32_t), hipMemcpyDeviceToHost,
-                                c->stream)) != hipSuccess)
-            return c->hip_fail(e, "hipMemcpyAsync(steps)");
-    }
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return c->hip_fail(e, "hipStreamSynchronize");
-    if (loss) *loss = out6[0];
+    const double* out = c->h_out;
+    if (loss) *loss = out[0];
     if (grad) {
-        for (int k = 0; k < 5; ++k) grad[k] = out6[1 + k];
+        for (int k = 0; k < 5; ++k) grad[k] = out[1 + k];
         grad[0] += grad[2] * drin_da;
     }
     if (stats) {
@@ -673,10 +685,8 @@ int diff_run(SingContext* c, const SingUniforms* base, const SingDiffParams* p, 
         stats->forward_ms = a;
         stats->backward_ms = bms;
         stats->rays = P;
-        std::uint64_t s = 0;
-        for (std::uint32_t v : steps) s += v;
-        stats->steps = s;
-        stats->fp64_rays = (o.precision == SING_PREC_FP64) ? P : (o.precision == SING_PREC_MIXED ? n64 : 0);
+        stats->steps = std::uint64_t(out[6]);
+        stats->fp64_rays = (o.precision == SING_PREC_FP64) ? P : std::uint64_t(out[sing::kReduceVals]);
     }
     return SING_OK;
 }

@@ -23,6 +23,10 @@ struct DiffSet {
 // Per-pixel device buffers of one differentiable frame.
 // Blocks of the first stage of the loss/gradient reduction.
 constexpr std::uint32_t kReduceBlocks = 256;
+// reduced per evaluation: loss, the 5 gradient entries, the RK4 step total
+constexpr std::uint32_t kReduceVals = 7;
+// the reduction's output: those sums, then the FP64 list's length
+constexpr std::uint32_t kDiffOut = kReduceVals + 1;
 
 struct DiffBuffers {
     double* cp;             // [ncp][5][P] states every C steps (double storage)
@@ -42,11 +46,12 @@ struct DiffBuffers {
     std::uint32_t* list2;   // pixels handed to FP64 after the FP32 march (grazing, near-hole emission)
     std::uint32_t* count2;  // length of list2
     std::uint32_t* steps;   // [P] RK4 steps of the forward
-    double* partial;        // [kReduceBlocks][6] per-block sums of loss and gradient
+    double* partial;        // [kReduceBlocks][kReduceVals] per-block sums of loss, gradient, steps
     std::uint32_t P, ncp, C;
 };
 
 hipError_t launch_diff(const DiffSet* ds_dev, const DiffBuffers& b, std::uint32_t precision, bool backward,
-                       double* out6, hipStream_t stream, hipStream_t stream2, const hipEvent_t* ev);
+                       double* out, hipStream_t stream, hipStream_t stream2, hipEvent_t fwd_end,
+                       const hipEvent_t* join);
 
 }  // namespace sing

@@ -135,6 +135,16 @@ int SingContext::ensure_events(std::size_t n) {
     return SING_OK;
 }
 
+int SingContext::ensure_join_events(std::size_t n) {
+    while (join_events.size() < n) {
+        hipEvent_t e;
+        const hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+        if (r != hipSuccess) return hip_fail(r, "hipEventCreateWithFlags");
+        join_events.push_back(e);
+    }
+    return SING_OK;
+}
+
 std::uint32_t SingContext::chunk_samples(std::uint32_t spp) const {
     const std::size_t P = std::size_t(width) * height;
     std::size_t S = kRecordBudget / (P * sizeof(sing::Rec));
@@ -230,6 +240,7 @@ void sing_destroy(SingContext* c) {
     c->free_diff();
     c->free_batch();
     for (auto& e : c->events) hipEventDestroy(e);
+    for (auto& e : c->join_events) hipEventDestroy(e);
     if (c->stream2) {
         hipStreamSynchronize(c->stream2);
         hipStreamDestroy(c->stream2);

@@ -22,7 +22,8 @@ struct SingContext {
     std::uint32_t width = 0, height = 0;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // second stream (default priority) for concurrent passes
-    std::vector<hipEvent_t> events;
+    std::vector<hipEvent_t> events;       // timing events
+    std::vector<hipEvent_t> join_events;  // cross-stream joins (created without timing)
     // frame outputs
     uchar4* d_rgba = nullptr;
     float* d_hdr = nullptr;
@@ -51,7 +52,8 @@ struct SingContext {
     sing::DiffBuffers diff{};
     float* d_target = nullptr;
     float* d_weight = nullptr;
-    double* d_out6 = nullptr;
+    double* d_out = nullptr;  // the adjoint's reduced results (sing::kDiffOut doubles)
+    double* h_out = nullptr;  // pinned host copy of d_out
     std::size_t diff_P = 0, diff_ncp = 0, diff_C = 0;
     // batched render
     sing::BatchScene* d_bscenes = nullptr;
@@ -78,6 +80,7 @@ struct SingContext {
     int ensure_records(std::size_t rays);
     int ensure_items(std::size_t items);
     int ensure_events(std::size_t n);
+    int ensure_join_events(std::size_t n);
     std::uint32_t chunk_samples(std::uint32_t spp) const;
     sing::LaunchShape launch_shape(const SingRenderOpts& o) const;
     int ensure_diff(std::size_t P, std::size_t ncp, std::size_t C);

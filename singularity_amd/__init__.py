@@ -398,7 +398,11 @@ def diff_loss_grad(be: "HipBackend", base: Uniforms, params: DiffParams, cam: Di
                    weights: np.ndarray | None = None, opts: DiffOpts | None = None):
     """Loss = sum w |I - target|^2 and d loss / d(a/M, incl, r_in/M, r_out/M, T-scale)."""
     t = np.ascontiguousarray(target, dtype=np.float32)
+    if t.shape != (be.height, be.width, 3):
+        raise ValueError(f"target shape {t.shape} != {(be.height, be.width, 3)}")
     w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    if w is not None and w.shape != (be.height, be.width):
+        raise ValueError(f"weights shape {w.shape} != {(be.height, be.width)}")
     loss = C.c_double(0.0)
     grad = np.zeros(5, np.float64)
     hdr = np.empty((be.height, be.width, 3), np.float32)

@@ -7,8 +7,6 @@ source tree:
     singularity_amd/libsing_hip.so   the C-ABI library (include/sing_abi.h)
     singularity_amd/hip_cli          flag-compatible replacement of cuda_cli
     singularity_amd/hip_diag         the CPU driver's kerr-geometry / photon-orbit / benchmark modes
-    singularity_amd/hbm_copy         HBM counter calibration for profiling (256 MiB copy)
-    singularity_amd/valu_rate        VALU issue-rate calibration for profiling (FMA/MUL/RCP chains)
 
 Usage: python -m singularity_amd.build  (or import and call build()).
 """
@@ -29,8 +27,6 @@ OBJ = PKG / "_build"
 LIB = PKG / "libsing_hip.so"
 CLI = PKG / "hip_cli"
 DIAG = PKG / "hip_diag"
-COPY = PKG / "hbm_copy"
-RATE = PKG / "valu_rate"
 
 ARCH = "gfx950"
 HIP_FLAGS = [
@@ -113,16 +109,6 @@ def build(verbose: bool = False) -> Path:
             ]
         )
         os.replace(tmpd, DIAG)
-    copy_src = CSRC / "hbm_copy.hip"
-    if copy_src.exists():
-        tmpk = COPY.with_suffix(".tmp")
-        _run([hipcc, *HIP_FLAGS, "-o", str(tmpk), str(copy_src)])
-        os.replace(tmpk, COPY)
-    rate_src = CSRC / "valu_rate.hip"
-    if rate_src.exists():
-        tmpr = RATE.with_suffix(".tmp")
-        _run([hipcc, *HIP_FLAGS, "-o", str(tmpr), str(rate_src)])
-        os.replace(tmpr, RATE)
     return LIB
 
 

@@ -621,6 +621,9 @@ extern "C" int sing_render_batch(SingContext* c, const SingBatchItem* items, uin
     for (std::uint32_t k = 0; k < n; ++k)
         key.push_back((std::uint64_t(tab[k].P) << 32) ^ (std::uint64_t(tab[k].spp) << 20) ^
                       (std::uint64_t(tab[k].sf.metric) << 62) ^ items[order[k]].uniforms->width);
+    // the record-memory grouping (set by record_budget_mib) decides the recorded launches
+    key.push_back(groups.size());
+    for (const auto& g : groups) key.push_back((std::uint64_t(g.first) << 32) | g.count);
     std::uint32_t replayed = 0;
     (void)hipEventRecord(c->events[0], c->stream);
     if (o.use_graph) {

@@ -117,7 +117,7 @@ void print_help() {
               "\n"
               "Options (as singularity_cuda_cli):\n"
               "  --scene PATH         Scene config (key = value file).\n"
-              "  --output PATH        PNG output path (default: hip_render.png).\n"
+              "  --output PATH        PNG output path (default: cuda_render.png).\n"
               "  --res WxH            Output resolution (default: 1280x720)\n"
               "  --samples-per-pixel N  Halton(2,3) subpixel samples (1..1024, default 1).\n"
               "  --camera-distance M  Orbital radius in M (default 30)\n"
@@ -149,7 +149,7 @@ void print_help() {
 }  // namespace
 
 int main(int argc, char** argv) {
-    std::string scene_path, output_path = "hip_render.png", output_pattern, encode_mp4_path, hdr_path, cls_path;
+    std::string scene_path, output_path = "cuda_render.png", output_pattern, encode_mp4_path, hdr_path, cls_path;
     std::uint32_t width = 1280, height = 720, spp = 1, frames = 1, mp4_fps = 30;
     float camera_distance = 30.0f, camera_elevation = 0.15f, camera_azimuth = 0.0f, fov_y_deg = 60.0f;
     float h_step = 0.0f, escape_r = 0.0f, temp_scale = 0.0f;

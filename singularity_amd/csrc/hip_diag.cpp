@@ -34,28 +34,29 @@ void print_help() {
         "  hip_diag --mode benchmark     [options]\n"
         "\n"
         "Modes:\n"
-        "  kerr-geometry    Dump the analytic Kerr scalars (horizons, ISCO,\n"
-        "                   photon spheres, ergosphere extents) as JSON.\n"
-        "  photon-orbit     Integrate a single null circular orbit (default\n"
-        "                   r = 1.5 r_s, or the prograde Kerr photon orbit)\n"
-        "                   on the GPU and dump the (t, r, theta, phi) trail as CSV.\n"
-        "  benchmark        Time the Schwarzschild RK4 or Kerr Hamiltonian\n"
-        "                   integrator over a deterministic ray fan on the GPU\n"
-        "                   (one lane per ray) and emit one JSON line.\n"
+        "  kerr-geometry    Closed-form Kerr radii as JSON: both horizons,\n"
+        "                   ergosurface at pole and equator, prograde and\n"
+        "                   retrograde photon orbits and ISCOs.\n"
+        "  photon-orbit     March one circular null orbit (r_init, by default\n"
+        "                   3 M, or the prograde Kerr photon orbit) on the GPU\n"
+        "                   and write its t, r, theta, phi samples as CSV.\n"
+        "  benchmark        Steps/s of the Schwarzschild or Kerr RK4 stepper\n"
+        "                   over a fixed fan of equatorial rays, one GPU lane\n"
+        "                   per ray; prints one JSON line.\n"
         "\n"
         "Options:\n"
-        "  --output PATH        JSON / CSV output path (mode-dependent default)\n"
-        "  --b-range MIN MAX    (benchmark) impact parameter span in M (default -10 +10)\n"
-        "  --n-rays N           (benchmark) number of rays (default 100)\n"
-        "  --max-steps N        (benchmark) step budget per ray (default 5000)\n"
-        "  --spin VALUE         a/M in [0, 1] (default 0)\n"
-        "  --mass VALUE         M in geometrized units (default 1)\n"
-        "  --r-init VALUE       (photon-orbit) initial radius in M (default 3)\n"
-        "  --orbits VALUE       (photon-orbit) orbital periods to integrate (default 1)\n"
-        "  --h-step VALUE       (photon-orbit) RK4 step in M (default 0.01)\n"
-        "  --metric NAME        (benchmark) 'schw' (default) or 'kerr'\n"
-        "  --scene PATH         Load scene config (key = value text): mass spin\n"
-        "                       max_steps h_step are used here\n"
+        "  --output PATH        where the JSON / CSV goes (each mode has a default)\n"
+        "  --b-range MIN MAX    (benchmark) span of impact parameters, M (default -10 10)\n"
+        "  --n-rays N           (benchmark) rays in the fan (default 100)\n"
+        "  --max-steps N        (benchmark) step limit of each ray (default 5000)\n"
+        "  --spin VALUE         spin a/M, 0 to 1 (default 0)\n"
+        "  --mass VALUE         mass M, geometrized units (default 1)\n"
+        "  --r-init VALUE       (photon-orbit) starting radius, M (default 3)\n"
+        "  --orbits VALUE       (photon-orbit) periods to march (default 1)\n"
+        "  --h-step VALUE       (photon-orbit) step size, M (default 0.01)\n"
+        "  --metric NAME        (benchmark) schw (default) or kerr\n"
+        "  --scene PATH         key = value scene file; this tool reads its\n"
+        "                       mass, spin, max_steps and h_step\n"
         "  --device N           HIP device (default 0)\n");
 }
 

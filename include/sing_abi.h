@@ -167,6 +167,33 @@ int sing_capture_class(SingContext* ctx, uint8_t* dst, size_t bytes);
 int sing_capture_steps(SingContext* ctx, uint32_t* dst, size_t bytes);
 int sing_capture_hdr16(SingContext* ctx, uint16_t* dst, size_t bytes);
 
+/* --- render without device allocation ---------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the sing_render pipeline with the same kernels, options and
+ * results, writing straight into caller-owned device buffers, with every
+ * working buffer taken from one static 256 MiB device array per GPU (held until
+ * the process ends).  No context; no hipMalloc/hipFree.  Frames whose working
+ * set for one sample per chunk exceeds the array (about 3.7 Mpixel, e.g.
+ * 2560x1440, without bloom) are rejected with SING_ERR_ARG; sing_render has no
+ * such limit.  Synchronous on `stream` (NULL = the null stream) of `device`,
+ * which becomes the calling thread's current device.  The first call on a
+ * device also looks up the array and uploads the 8 KiB sample-offset tables
+ * (make it before any graph capture).  Calls that overlap in time on one GPU
+ * share the array and race.  Output pointers: rgba W*H*4, hdr W*H*3 floats,
+ * hdr16 W*H*4 halves, cls W*H, steps W*H u32, each required when its
+ * SING_OUT_* bit (or bloom, for rgba) is selected.  stats: rays, steps,
+ * fp64_rays, chunks, rejected (timings are 0).  Errors: sing_last_error(NULL). */
+typedef struct SingDeviceOutputs {
+    uint8_t* rgba;
+    float* hdr;
+    uint16_t* hdr16;
+    uint8_t* cls;
+    uint32_t* steps;
+} SingDeviceOutputs;
+int sing_render_noalloc(int device, const SingUniforms* u, const SingRenderOpts* opts, const SingDeviceOutputs* out,
+                        void* stream, SingRenderStats* stats);
+
 /* --- host-side scene ingestion (no GPU needed) ------------------------------ */
 /* render_backend.hpp:46-82 Scene (the fields cuda_cli uses) */
 typedef struct SingScene {

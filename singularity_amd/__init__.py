@@ -81,6 +81,13 @@ class RenderStats(C.Structure):
     ]
 
 
+class DeviceOutputs(C.Structure):
+    """Device output pointers of sing_render_noalloc (include/sing_abi.h)."""
+
+    _fields_ = [("rgba", C.c_void_p), ("hdr", C.c_void_p), ("hdr16", C.c_void_p), ("cls", C.c_void_p),
+                ("steps", C.c_void_p)]
+
+
 class Scene(C.Structure):
     """render_backend.hpp:46-82 Scene (the fields the offline path uses)."""
 
@@ -124,6 +131,8 @@ _SIGS = {
     "sing_destroy": (None, [C.c_void_p]),
     "sing_resize": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "sing_render": (C.c_int, [C.c_void_p, C.POINTER(Uniforms), C.POINTER(RenderOpts), C.POINTER(RenderStats)]),
+    "sing_render_noalloc": (C.c_int, [C.c_int, C.POINTER(Uniforms), C.POINTER(RenderOpts), C.POINTER(DeviceOutputs),
+                                      C.c_void_p, C.POINTER(RenderStats)]),
     "sing_capture_rgba8": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "sing_capture_hdr": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "sing_capture_class": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

@@ -41,7 +41,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-Wall"]
 
-HIP_SOURCES = ["trace.hip", "post.hip", "context.hip", "adjoint.hip", "batch.hip", "diag.hip"]
+HIP_SOURCES = ["trace.hip", "post.hip", "context.hip", "adjoint.hip", "batch.hip", "diag.hip", "noalloc.hip"]
 CXX_SOURCES = ["host_scene.cpp"]
 
 

@@ -154,7 +154,9 @@ std::uint32_t SingContext::chunk_samples(std::uint32_t spp) const {
     return std::uint32_t(std::max<std::size_t>(S, 1));
 }
 
-sing::LaunchShape SingContext::launch_shape(const SingRenderOpts& o) const {
+sing::LaunchShape SingContext::launch_shape(const SingRenderOpts& o) const { return sing::launch_shape_for(num_cu, o); }
+
+sing::LaunchShape sing::launch_shape_for(int num_cu, const SingRenderOpts& o) {
     sing::LaunchShape sh{};
     // defaults from a launch-shape sweep of the 4K Kerr frame: 32 steps
     // between refills and 24 waves per CU (the trace kernels' occupancy
@@ -165,6 +167,26 @@ sing::LaunchShape SingContext::launch_shape(const SingRenderOpts& o) const {
     sh.blocks64 = std::uint32_t(num_cu) * ((std::max(wpc / 2u, 4u) + 3u) / 4u);
     return sh;
 }
+
+namespace sing {
+
+bool render_args_ok(const SingUniforms& u, const SingRenderOpts& o, const char* who, char* msg, std::size_t len) {
+    char why[96] = "";
+    if (o.precision > SING_PREC_MIXED) std::snprintf(why, sizeof(why), "unknown precision %u", o.precision);
+    else if (o.post > SING_POST_BLOOM) std::snprintf(why, sizeof(why), "unknown post pass %u", o.post);
+    else if (o.sampler > SING_SAMPLER_GRID) std::snprintf(why, sizeof(why), "unknown sampler %u", o.sampler);
+    else if (u.metric_type > SING_METRIC_KERR) std::snprintf(why, sizeof(why), "unknown metric %u", u.metric_type);
+    else if (!(o.adapt_tol >= 0.0f && o.adapt_tol <= 0.1f)) std::snprintf(why, sizeof(why), "adapt_tol must be in [0, 0.1]");
+    else if (o.adapt_tol > 0.0f && u.metric_type != SING_METRIC_KERR)
+        std::snprintf(why, sizeof(why), "error-controlled stepping (adapt_tol) is Kerr-only");
+    else if (o.post == SING_POST_BLOOM && (u.width < 2 || u.height < 2))
+        std::snprintf(why, sizeof(why), "bloom needs at least 2x2 pixels");
+    if (!why[0]) return true;
+    std::snprintf(msg, len, "%s: %s", who, why);
+    return false;
+}
+
+}  // namespace sing
 
 extern "C" {
 
@@ -267,17 +289,8 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
                        c->width, c->height);
     SingRenderOpts o{};
     if (opts_in) o = *opts_in;
-    if (o.precision > SING_PREC_MIXED) return c->fail(SING_ERR_ARG, "sing_render: unknown precision %u", o.precision);
-    if (o.post > SING_POST_BLOOM) return c->fail(SING_ERR_ARG, "sing_render: unknown post pass %u", o.post);
-    if (o.sampler > SING_SAMPLER_GRID) return c->fail(SING_ERR_ARG, "sing_render: unknown sampler %u", o.sampler);
-    if (u->metric_type > SING_METRIC_KERR)
-        return c->fail(SING_ERR_ARG, "sing_render: unknown metric %u", u->metric_type);
-    if (!(o.adapt_tol >= 0.0f && o.adapt_tol <= 0.1f))
-        return c->fail(SING_ERR_ARG, "sing_render: adapt_tol must be in [0, 0.1]");
-    if (o.adapt_tol > 0.0f && u->metric_type != SING_METRIC_KERR)
-        return c->fail(SING_ERR_ARG, "sing_render: error-controlled stepping (adapt_tol) is Kerr-only");
-    if (o.post == SING_POST_BLOOM && (c->width < 2 || c->height < 2))
-        return c->fail(SING_ERR_ARG, "sing_render: bloom needs at least 2x2 pixels");
+    char msg[160];
+    if (!sing::render_args_ok(*u, o, "sing_render", msg, sizeof(msg))) return c->fail(SING_ERR_ARG, "%s", msg);
     std::uint32_t outs = o.outputs ? o.outputs : SING_OUT_RGBA8;
     if (o.post == SING_POST_BLOOM) outs |= SING_OUT_HDR16 | SING_OUT_RGBA8;
     hipError_t e = hipSetDevice(c->device);

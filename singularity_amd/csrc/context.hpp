@@ -16,6 +16,14 @@
 
 void sing_set_global_error(const std::string& m);
 
+namespace sing {
+// Checks the options and uniform block of a frame render; on failure writes
+// "<who>: <reason>" into msg and returns false.
+bool render_args_ok(const SingUniforms& u, const SingRenderOpts& o, const char* who, char* msg, std::size_t len);
+// Persistent grid and step batch of a frame's trace kernels on a device with num_cu CUs.
+LaunchShape launch_shape_for(int num_cu, const SingRenderOpts& o);
+}  // namespace sing
+
 struct SingContext {
     int device = 0;
     int num_cu = 256;

@@ -83,6 +83,12 @@ def _type_oracle(L: C.CDLL) -> None:
     L.orc_blackbody_f32.argtypes = [C.c_float, P]
     L.orc_blackbody_f64.argtypes = [C.c_double, P]
     L.orc_starfield_f32.argtypes = [C.c_float, C.c_float, C.c_float, P]
+    L.orc_schw_shift_f32.argtypes = [C.c_float] * 5 + [C.c_uint, P]
+    L.orc_kerr_shift_f32.argtypes = [C.c_float] * 5 + [C.c_uint, P]
+    L.orc_kerr_shift_f64.argtypes = [C.c_double] * 5 + [C.c_uint, P]
+    L.orc_disc_flux_f32.restype = C.c_float
+    L.orc_disc_flux_f32.argtypes = [C.c_float, C.c_float]
+    L.orc_disc_emit_f32.argtypes = [P] + [C.c_float] * 5 + [P]
     L.orc_mix32.restype = C.c_uint32
     L.orc_mix32.argtypes = [C.c_uint32, C.c_uint32]
     L.orc_aces_f32.argtypes = [P, P]

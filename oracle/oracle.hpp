@@ -18,9 +18,12 @@
 // -ffp-contract=off (checked against oracle/_ref by tests).
 //
 // Every function here is pinned by a test (tests/test_oracle_vs_reference.py,
-// tests/test_oracle_math.py, tests/test_golden_images.py) except the bloom
-// post passes, whose parity is unpinned: the reference's realtime backends
-// that own them are not buildable here and commit no bloom fixtures.
+// tests/test_oracle_math.py, tests/test_oracle_shading.py,
+// tests/test_golden_images.py): the physics bit-exact against the reference
+// headers, the shading against the reference's host twins
+// (core/include/physics/{disc,redshift}.hpp) and hand-derived known answers,
+// the bloom passes by their closed forms (uniform blocks, impulses, the knee
+// at the threshold, the weight sum).
 
 #pragma once
 
@@ -647,47 +650,66 @@ template <class T> inline T soft_alpha(T r, T r_in, T r_out, T w) {
     return vclamp(xfmin(lo, hi), T(0.0), T(1.0));
 }
 
-// Disc emission at a crossing for the Schwarzschild path (geodesic_kernel.cu:264-289).
-template <class T> inline V3<T> schw_disc_emit(const Uni<T>& u, T r_cross, T uphi_cross, T phi_cross, T f_cam, T Tk) {
-    const T f_emit = T(1.0) - u.rs / r_cross;
-    const T g_grav = (u.flags & FLAG_REDSHIFT) ? xsqrt(xfmax(f_emit / f_cam, T(0.0))) : T(1.0);
-    T g_dop = T(1.0);
-    if (u.flags & FLAG_DOPPLER) {
-        const T omega = xsqrt(u.mass_M / (r_cross * r_cross * r_cross));
-        const T v = r_cross * omega / xsqrt(xfmax(f_emit, T(1e-6)));
+// Frequency-shift factors at a disc crossing.  g_grav and g_dop follow the
+// kernel's Schwarzschild shading (geodesic_kernel.cu:266-279): the host twins
+// are schwarzschild_grav_redshift, schwarzschild_keplerian_omega and
+// schwarzschild_doppler_factor (core/include/physics/redshift.hpp:44-97),
+// here with the kernel's clamps.
+template <class T> struct Shift {
+    T g_grav, g_dop, omega;
+};
+template <class T> inline Shift<T> schw_shift(T rs, T M, T r_cross, T uphi_cross, T f_cam, unsigned flags) {
+    Shift<T> s{T(1.0), T(1.0), T(0.0)};
+    const T f_emit = T(1.0) - rs / r_cross;
+    if (flags & FLAG_REDSHIFT) s.g_grav = xsqrt(xfmax(f_emit / f_cam, T(0.0)));
+    if (flags & FLAG_DOPPLER) {
+        s.omega = xsqrt(M / (r_cross * r_cross * r_cross));
+        const T v = r_cross * s.omega / xsqrt(xfmax(f_emit, T(1e-6)));
         const T cos_psi = vclamp(-uphi_cross * r_cross, T(-1.0), T(1.0));
-        g_dop = xsqrt(xfmax(T(1.0) - v * v, T(0.0))) / xfmax(T(1.0) - v * cos_psi, T(1e-4));
+        s.g_dop = xsqrt(xfmax(T(1.0) - v * v, T(0.0))) / xfmax(T(1.0) - v * cos_psi, T(1e-4));
     }
-    const T g = g_grav * g_dop;
+    return s;
+}
+// Kerr path (geodesic_kernel.cu:368-386): Schwarzschild lapse ratio for g_grav,
+// prograde Keplerian omega_K = omega_0 / (1 + a omega_0), cos psi = L_z / r.
+template <class T> inline Shift<T> kerr_shift(T M, T a, T r_cross, T cam_r, T L_z, unsigned flags) {
+    Shift<T> s{T(1.0), T(1.0), T(0.0)};
+    const T f_emit = T(1.0) - T(2.0) * M / r_cross;
+    const T f_obs = T(1.0) - T(2.0) * M / cam_r;
+    if (flags & FLAG_REDSHIFT) s.g_grav = xsqrt(xfmax(f_emit, T(1e-6)) / xfmax(f_obs, T(1e-6)));
+    if (flags & FLAG_DOPPLER) {
+        const T om0 = xsqrt(M / (r_cross * r_cross * r_cross));
+        s.omega = om0 / (T(1.0) + a * om0);
+        const T v = r_cross * s.omega / xsqrt(xfmax(f_emit, T(1e-6)));
+        const T cos_psi = vclamp(L_z / r_cross, T(-1.0), T(1.0));
+        const T v2 = v * v;
+        s.g_dop = xsqrt(xfmax(T(1.0) - v2, T(0.0))) / xfmax(T(1.0) - v * cos_psi, T(1e-4));
+    }
+    return s;
+}
+// Flux clamp of the disc emission (geodesic_kernel.cu:281,388).
+template <class T> inline T disc_flux(T r_in, T r_cross) {
+    return vclamp(xpow(r_in / r_cross, T(3.0)), T(0.05), T(1.5));
+}
+// blackbody(g T) x flux x bands x g^4 (geodesic_kernel.cu:280-286,387-393; g^4 is
+// intensity_scaling, redshift.hpp:106-109).
+template <class T> inline V3<T> disc_emit(const Uni<T>& u, const Shift<T>& sh, T r_cross, T phi_cross, T Tk) {
+    const T g = sh.g_grav * sh.g_dop;
     const V3<T> col = blackbody_srgb(g * Tk);
-    const T flux = vclamp(xpow(u.disc_r_inner / r_cross, T(3.0)), T(0.05), T(1.5));
+    const T flux = disc_flux(u.disc_r_inner, r_cross);
     const T bands = disc_band_mask(r_cross, phi_cross, u.disc_r_inner, u.disc_turbulence, u.time_sec,
                                    u.ext.band_phase1, u.ext.band_phase2);
     const T g2 = g * g;
     return col * (flux * bands * g2 * g2);
 }
+
+// Disc emission at a crossing for the Schwarzschild path (geodesic_kernel.cu:264-289).
+template <class T> inline V3<T> schw_disc_emit(const Uni<T>& u, T r_cross, T uphi_cross, T phi_cross, T f_cam, T Tk) {
+    return disc_emit(u, schw_shift(u.rs, u.mass_M, r_cross, uphi_cross, f_cam, u.flags), r_cross, phi_cross, Tk);
+}
 // Disc emission at a crossing for the Kerr path (geodesic_kernel.cu:366-394).
 template <class T> inline V3<T> kerr_disc_emit(const Uni<T>& u, T r_cross, T phi_cross, T cam_r, T L_z, T Tk) {
-    const T M = u.mass_M, a = u.spin_a;
-    const T f_emit = T(1.0) - T(2.0) * M / r_cross;
-    const T f_obs = T(1.0) - T(2.0) * M / cam_r;
-    const T g_grav = (u.flags & FLAG_REDSHIFT) ? xsqrt(xfmax(f_emit, T(1e-6)) / xfmax(f_obs, T(1e-6))) : T(1.0);
-    T g_dop = T(1.0);
-    if (u.flags & FLAG_DOPPLER) {
-        const T om0 = xsqrt(M / (r_cross * r_cross * r_cross));
-        const T omega_K = om0 / (T(1.0) + a * om0);
-        const T v = r_cross * omega_K / xsqrt(xfmax(f_emit, T(1e-6)));
-        const T cos_psi = vclamp(L_z / r_cross, T(-1.0), T(1.0));
-        const T v2 = v * v;
-        g_dop = xsqrt(xfmax(T(1.0) - v2, T(0.0))) / xfmax(T(1.0) - v * cos_psi, T(1e-4));
-    }
-    const T g = g_grav * g_dop;
-    const V3<T> col = blackbody_srgb(g * Tk);
-    const T flux = vclamp(xpow(u.disc_r_inner / r_cross, T(3.0)), T(0.05), T(1.5));
-    const T bands = disc_band_mask(r_cross, phi_cross, u.disc_r_inner, u.disc_turbulence, u.time_sec,
-                                   u.ext.band_phase1, u.ext.band_phase2);
-    const T g2 = g * g;
-    return col * (flux * bands * g2 * g2);
+    return disc_emit(u, kerr_shift(u.mass_M, u.spin_a, r_cross, cam_r, L_z, u.flags), r_cross, phi_cross, Tk);
 }
 
 template <class T> inline V3<T> escape_color(const Uni<T>& u, T theta, T phi) {  // geodesic_kernel.cu:251-259

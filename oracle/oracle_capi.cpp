@@ -229,6 +229,27 @@ double orc_disc_temperature_f64(double r, double rin, double peak) { return disc
 float orc_band_mask_f32(float r, float phi, float rin, float turb, float t, float ph1, float ph2) {
     return disc_band_mask(r, phi, rin, turb, t, ph1, ph2);
 }
+void orc_schw_shift_f32(float rs, float M, float r, float uphi, float f_cam, unsigned flags, float* out) {
+    const Shift<float> sh = schw_shift(rs, M, r, uphi, f_cam, flags);
+    out[0] = sh.g_grav; out[1] = sh.g_dop; out[2] = sh.omega;
+}
+void orc_kerr_shift_f32(float M, float a, float r, float cam_r, float L_z, unsigned flags, float* out) {
+    const Shift<float> sh = kerr_shift(M, a, r, cam_r, L_z, flags);
+    out[0] = sh.g_grav; out[1] = sh.g_dop; out[2] = sh.omega;
+}
+void orc_kerr_shift_f64(double M, double a, double r, double cam_r, double L_z, unsigned flags, double* out) {
+    const Shift<double> sh = kerr_shift(M, a, r, cam_r, L_z, flags);
+    out[0] = sh.g_grav; out[1] = sh.g_dop; out[2] = sh.omega;
+}
+float orc_disc_flux_f32(float r_in, float r) { return disc_flux(r_in, r); }
+// Disc emission of the uniform block's scene at a crossing, Kerr or
+// Schwarzschild by metric_type: x = cam_r (Kerr) or f_cam (Schwarzschild),
+// y = L_z (Kerr) or u^phi at the crossing (Schwarzschild).
+void orc_disc_emit_f32(const void* uniforms, float r, float phi, float x, float y, float Tk, float* rgb) {
+    const Uni<float> u = load_uni<float>(uniforms, nullptr);
+    const V3<float> c = u.metric_type == 1u ? kerr_disc_emit(u, r, phi, x, y, Tk) : schw_disc_emit(u, r, y, phi, x, Tk);
+    rgb[0] = c.x; rgb[1] = c.y; rgb[2] = c.z;
+}
 void orc_starfield_f32(float x, float y, float z, float* rgb) {
     const V3<float> c = starfield_color(v3(x, y, z));
     rgb[0] = c.x; rgb[1] = c.y; rgb[2] = c.z;

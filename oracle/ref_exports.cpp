@@ -1,5 +1,5 @@
 // oracle/ref_exports.cpp — exports the reference's own host-compilable physics
-// headers (shared_shader/*.h, core/include/physics/kerr.hpp) as extern "C"
+// headers (shared_shader/*.h, core/include/physics/{kerr,disc,redshift}.hpp) as extern "C"
 // functions, so the oracle's restatement can be checked against the real
 // reference arithmetic.  Compiled ONLY from /root/reference by oracle/Makefile
 // into oracle/_ref/libref.so; test infrastructure, never part of the product.
@@ -13,7 +13,9 @@
 #include "geodesic_math.h"
 #include "kerr_hamilton.h"
 #include "kerr_math.h"
+#include "physics/disc.hpp"
 #include "physics/kerr.hpp"
+#include "physics/redshift.hpp"
 #include "uniforms.h"
 
 extern "C" {
@@ -113,5 +115,21 @@ float ref_kerr_closed_form(int which, float M, float a) {
     }
     return 0.0f;
 }
+
+// core/include/physics/disc.hpp:122-155: the host twin of the kernel's
+// blackbody (no [1000, 40000] K clamp, constants divided by 255 at run time).
+void ref_blackbody_th(float T, float* rgb) {
+    const singularity::physics::sRGBColor c = singularity::physics::blackbody_srgb_tanner_helland(T);
+    rgb[0] = c.r; rgb[1] = c.g; rgb[2] = c.b;
+}
+// core/include/physics/redshift.hpp:44-112
+float ref_grav_redshift(float rs, float r_emit, float r_obs) {
+    return singularity::physics::schwarzschild_grav_redshift(rs, r_emit, r_obs);
+}
+float ref_doppler_factor(float rs, float r_emit, float omega, float cos_psi) {
+    return singularity::physics::schwarzschild_doppler_factor(rs, r_emit, omega, cos_psi);
+}
+float ref_keplerian_omega(float M, float r) { return singularity::physics::schwarzschild_keplerian_omega(M, r); }
+float ref_intensity_scaling(float g) { return singularity::physics::intensity_scaling(g); }
 
 }  // extern "C"

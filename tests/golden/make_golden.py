@@ -13,7 +13,13 @@ they lie, see oracle/Makefile) and records input/output vectors of:
   * Kerr closed forms (horizons, ISCO, photon orbits) on a spin grid,
   * the Uniforms byte offsets.
 
-Usage: python tests/golden/make_golden.py
+With the argument `shading` it writes tests/golden/ref_shading.npz instead:
+the shading arithmetic's host twins in core/include/physics/disc.hpp:122-155
+(blackbody_srgb_tanner_helland) and redshift.hpp:44-112
+(schwarzschild_grav_redshift, _keplerian_omega, _doppler_factor,
+intensity_scaling) on temperature, radius, cos(psi) and g grids.
+
+Usage: python tests/golden/make_golden.py [shading]
 """
 
 from __future__ import annotations
@@ -159,5 +165,43 @@ def main() -> None:
     print("wrote", HERE / "ref_vectors.npz", {k: v.shape for k, v in out.items()})
 
 
+def shading() -> None:
+    subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "ref"], check=True)
+    if not oracle.ref_available():
+        raise SystemExit("reference tree not available: cannot generate vectors")
+    R = oracle.ref()
+    R.ref_blackbody_th.argtypes = [C.c_float, C.c_void_p]
+    for name in ("ref_grav_redshift", "ref_doppler_factor", "ref_keplerian_omega", "ref_intensity_scaling"):
+        getattr(R, name).restype = C.c_float
+    R.ref_grav_redshift.argtypes = [C.c_float] * 3
+    R.ref_doppler_factor.argtypes = [C.c_float] * 4
+    R.ref_keplerian_omega.argtypes = [C.c_float] * 2
+    R.ref_intensity_scaling.argtypes = [C.c_float]
+    out: dict[str, np.ndarray] = {}
+    T = np.unique(np.concatenate([np.geomspace(1000.0, 40000.0, 4001), np.linspace(1000.0, 40000.0, 391),
+                                  [6599.0, 6600.0, 6601.0, 1900.0, 1901.0]]).astype(np.float32))
+    rgb = np.zeros((T.size, 3), np.float32)
+    for i, t in enumerate(T):
+        R.ref_blackbody_th(float(t), f32p(rgb[i]))
+    out.update(bb_T=T, bb_rgb=rgb)
+    # Schwarzschild (M = 1, rs = 2): emitter radii outside the horizon, observers at 30 and 100 M
+    r = np.geomspace(2.05, 60.0, 64).astype(np.float32)
+    r_obs = np.array([30.0, 100.0], np.float32)
+    cpsi = np.linspace(-1.0, 1.0, 21).astype(np.float32)
+    grav = np.array([[R.ref_grav_redshift(2.0, float(x), float(o)) for o in r_obs] for x in r], np.float32)
+    omega = np.array([R.ref_keplerian_omega(1.0, float(x)) for x in r], np.float32)
+    dop = np.array([[R.ref_doppler_factor(2.0, float(x), float(w), float(c)) for c in cpsi]
+                    for x, w in zip(r, omega)], np.float32)
+    g = np.geomspace(0.05, 3.0, 97).astype(np.float32)
+    ints = np.array([R.ref_intensity_scaling(float(v)) for v in g], np.float32)
+    out.update(rs_r=r, rs_r_obs=r_obs, rs_cos_psi=cpsi, rs_grav=grav, rs_omega=omega, rs_doppler=dop,
+               rs_g=g, rs_intensity=ints)
+    np.savez_compressed(HERE / "ref_shading.npz", **out)
+    print("wrote", HERE / "ref_shading.npz", {k: v.shape for k, v in out.items()})
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["shading"]:
+        shading()
+    else:
+        main()

@@ -49,6 +49,10 @@ def headline_uniforms(width: int, height: int, spp: int) -> sa.Uniforms:
 
 
 def err_stats(a: np.ndarray, ref: np.ndarray, ca: np.ndarray, cref: np.ndarray) -> dict:
+    """Per-pixel radiance error of a frame against the all-FP64 GPU frame of the same
+    scene (the same discrete algorithm in double on the device).  The FP64 host
+    oracle checks of these scenes are the GPU tests (tests/test_gpu_headline.py,
+    tests/test_gpu_config3.py)."""
     a = a.reshape(-1, 3).astype(np.float64)
     b = ref.reshape(-1, 3).astype(np.float64)
     rel = np.abs(a - b).max(-1) / np.maximum(np.abs(b).max(-1), 1e-6)
@@ -62,30 +66,41 @@ def err_stats(a: np.ndarray, ref: np.ndarray, ca: np.ndarray, cref: np.ndarray) 
     }
 
 
-def cpu_baseline(u: sa.Uniforms, budget_s: float, seed: int = 1234) -> dict:
+def cpu_baseline(u: sa.Uniforms, rows_per_thread: int = 2, seed: int = 1234) -> dict:
+    """The C++ restatement in FP32 with the reference Release build's flags (-O3
+    -ffast-math), scheduled like the reference CPU renderer (cli/main.cpp:739-740,
+    861-872): one worker per hardware thread pulling whole image rows from an
+    atomic counter.  Workers are capped at the 16 cores a one-GPU share of the host
+    provides; the machine's own hardware_concurrency is reported beside it.  The
+    sample is rows_per_thread seeded whole rows of the same frame per worker (a
+    4K row at 256 spp is about 10^6 rays; rows differ in cost by 2-3x, so with
+    one row per worker the slowest row would set the time)."""
     import oracle  # CPU baseline leg only
 
-    threads = max(1, min(os.cpu_count() or 1, 16))
-    L = oracle.fast()
-    rng = np.random.default_rng(seed)
-    P = u.width * u.height
-    order = rng.permutation(P)
-    done, t0, batch = 0, time.perf_counter(), 256
-    rays_per_px = max(1, u.supersample)
-    while True:
-        px = order[done:done + batch]
-        oracle.render(u, 32, pixels=px, threads=threads, lib=L)
-        done += px.size
-        el = time.perf_counter() - t0
-        if el >= budget_s or done >= P:
-            break
-        batch = min(4096, batch * 2)
-    mrays = done * rays_per_px / el / 1e6
+    hw = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = hw
+    threads = max(1, min(avail, 16))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+    except OSError:
+        pass
+    rows = np.random.default_rng(seed).permutation(u.height)[: rows_per_thread * threads]
+    t0 = time.perf_counter()
+    _, steps = oracle.render_rows(u, rows, threads=threads, lib=oracle.fast())
+    el = time.perf_counter() - t0
+    rays = rows.size * u.width * max(1, u.supersample)
     return {
-        "value": round(mrays, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-        "sample": f"{done} seeded-random pixels x {rays_per_px} spp of the same scene "
-                  f"({done * rays_per_px} rays, {el:.1f} s); FP32 C++ restatement, -O3 -ffast-math, "
-                  f"{threads} std::threads pulling pixels from an atomic counter",
+        "value": round(rays / el / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+        "host_hardware_concurrency": hw, "cpu_model": model,
+        "rk4_steps_per_s": round(steps / el, 1),
+        "sample": f"{rows.size} seeded whole rows x {u.width} px x {max(1, u.supersample)} spp of the same frame "
+                  f"({rays} rays, {el:.1f} s); FP32 C++ restatement, -O3 -ffast-math, {threads} std::threads "
+                  f"pulling rows from an atomic counter (host reports {hw} hardware threads)",
     }
 
 
@@ -140,11 +155,11 @@ def c2_leg(frames: int = 3, seed: int = 20261203) -> dict:
         "post_ms": round(post_ms, 4),
         "post_hbm_gbs": round(FL.POST_BYTES_PER_PIXEL * W * H / (post_ms * 1e-3) / 1e9, 1),
         "post_bytes_per_pixel": FL.POST_BYTES_PER_PIXEL,
-        "plain_fp32": {"kernel_ms": round(s32.kernel_ms, 2), "error_vs_fp64": err_stats(f32.hdr, f64.hdr, f32.cls, f64.cls),
-                       "rgba_max_lsb_vs_fp64": int(np.abs(f32.rgba.astype(int) - f64.rgba.astype(int)).max())},
+        "plain_fp32": {"kernel_ms": round(s32.kernel_ms, 2), "error_vs_gpu_fp64": err_stats(f32.hdr, f64.hdr, f32.cls, f64.cls),
+                       "rgba_max_lsb_vs_gpu_fp64": int(np.abs(f32.rgba.astype(int) - f64.rgba.astype(int)).max())},
         "fp64": {"kernel_ms": round(s64.kernel_ms, 2)},
-        "mixed_error_vs_fp64": err_stats(fm.hdr, f64.hdr, fm.cls, f64.cls),
-        "mixed_rgba_max_lsb_vs_fp64": int(np.abs(fm.rgba.astype(int) - f64.rgba.astype(int)).max()),
+        "mixed_error_vs_gpu_fp64": err_stats(fm.hdr, f64.hdr, fm.cls, f64.cls),
+        "mixed_rgba_max_lsb_vs_gpu_fp64": int(np.abs(fm.rgba.astype(int) - f64.rgba.astype(int)).max()),
         "fixed_step_mixed": {"kernel_ms": round(sfix.kernel_ms, 2), "steps": int(sfix.steps)},
     }
     log(f"c2: {res['mrays_s']} Mrays/s, post {res['post_hbm_gbs']} GB/s")
@@ -170,7 +185,7 @@ def c3_leg() -> dict:
     f64 = frames["fp64"]
     for name in ("mixed", "fp32"):
         f = frames[name]
-        res[name]["error_vs_fp64"] = err_stats(f.hdr, f64.hdr, f.cls, f64.cls)
+        res[name]["error_vs_gpu_fp64"] = err_stats(f.hdr, f64.hdr, f.cls, f64.cls)
     res["mixed_over_plain_time"] = round(res["mixed"]["kernel_ms"] / res["fp32"]["kernel_ms"], 3)
     res["fp64_over_mixed_time"] = round(res["fp64"]["kernel_ms"] / res["mixed"]["kernel_ms"], 3)
     return res
@@ -304,7 +319,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--res", default="3840x2160")
     ap.add_argument("--spp", type=int, default=256)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-rows-per-thread", type=int, default=2)
     ap.add_argument("--no-compare", action="store_true", help="skip the plain-FP32 and FP64 comparison frames")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the grads/s leg (512x512 inverse render)")
@@ -341,7 +356,14 @@ def main() -> int:
     trace_ms = float(np.mean([s.trace_ms for s in stats]))
     kern_ms = float(np.mean([s.kernel_ms for s in stats]))
     steps_total = stats[-1].steps
-    achieved = FL.KERR_STEP_FLOPS * steps_total / (trace_ms * 1e-3) / 1e12
+    chunks = max(1, int(stats[-1].chunks))
+    # the dominant kernel is the FP32 trace pass (trace_kernel<1, 1>), launched once
+    # per sample chunk: its algorithmic flops per launch over its average launch time
+    t32 = float(np.mean([s.trace32_ms for s in stats]))
+    t64 = float(np.mean([s.trace64_ms for s in stats]))
+    steps32, steps64 = int(stats[-1].steps32), int(stats[-1].steps64)
+    achieved = FL.KERR_STEP_FLOPS * steps32 / (t32 * 1e-3) / 1e12
+    achieved64 = FL.KERR_STEP_FLOPS * steps64 / (t64 * 1e-3) / 1e12 if t64 > 0 else 0.0
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
@@ -358,13 +380,25 @@ def main() -> int:
         "roofline": {
             "bound": "valu", "achieved": round(achieved, 3), "peak": FL.PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FL.PEAK_FP32_TFLOPS, 4),
-            "traffic": round(FL.TRACE_HBM_BYTES_PER_RAY * rays_per_frame / max(1, int(stats[-1].chunks))),
-            "kernel": "trace (FP32 pass + FP64 band pass)",
-            "flops_per_step": FL.KERR_STEP_FLOPS, "steps_per_frame": int(steps_total),
+            "traffic": round(FL.TRACE_HBM_BYTES_PER_RAY * rays_per_frame / chunks),
+            "kernel": "trace_kernel<1, 1> (FP32 trace pass, one launch per sample chunk)",
+            "flops_per_step": FL.KERR_STEP_FLOPS, "divides_per_step": FL.KERR_STEP_DIVS,
+            "sincos_per_step": FL.KERR_STEP_TRIG, "launches_per_frame": chunks,
+            "steps_per_launch": round(steps32 / chunks), "launch_ms": round(t32 / chunks, 3),
+            "valu_insts_per_step": FL.KERR_STEP_VALU_INSTS,
+            "valu_issue_frac": round(FL.VALU_ISSUE_PER_CU_CLK / FL.VALU_ISSUE_PEAK_PER_CU_CLK, 3),
+        },
+        "roofline_fp64": {
+            "bound": "valu", "achieved": round(achieved64, 3), "peak": FL.PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved64 / FL.PEAK_FP64_TFLOPS, 4),
+            "kernel": "trace_kernel<1, 3> (FP64 pass over the rays routed to it)",
+            "flops_per_step": FL.KERR_STEP_FLOPS, "steps_per_launch": round(steps64 / chunks),
+            "launch_ms": round(t64 / chunks, 3),
         },
         "kernel_ms_per_frame": round(kern_ms, 2), "trace_ms_per_frame": round(trace_ms, 2),
         "shade_ms_per_frame": round(float(np.mean([s.shade_ms for s in stats])), 2),
         "rk4_steps_per_s": round(steps_total / (kern_ms * 1e-3), 1),
+        "rk4_steps_per_s_fp32_pass": round(steps32 / (t32 * 1e-3), 1),
         "fp64_rays_per_frame": int(stats[-1].fp64_rays),
         "chunks_per_frame": int(stats[-1].chunks),
     }
@@ -377,9 +411,9 @@ def main() -> int:
         log(f"fp64: {st64.kernel_ms:.0f} ms")
         out["plain_fp32"] = {"mrays_s": round(rays_per_frame / st32.kernel_ms / 1e3, 3),
                              "kernel_ms": round(st32.kernel_ms, 2),
-                             "error_vs_fp64": err_stats(f32.hdr, f64.hdr, f32.cls, f64.cls)}
+                             "error_vs_gpu_fp64": err_stats(f32.hdr, f64.hdr, f32.cls, f64.cls)}
         out["fp64"] = {"mrays_s": round(rays_per_frame / st64.kernel_ms / 1e3, 3), "kernel_ms": round(st64.kernel_ms, 2)}
-        out["mixed_error_vs_fp64"] = err_stats(f_mixed.hdr, f64.hdr, f_mixed.cls, f64.cls)
+        out["mixed_error_vs_gpu_fp64"] = err_stats(f_mixed.hdr, f64.hdr, f_mixed.cls, f64.cls)
         out["mixed_over_plain_time"] = round(kern_ms / st32.kernel_ms, 3)
         out["fp64_over_mixed_time"] = round(st64.kernel_ms / kern_ms, 3)
     be.shutdown()
@@ -393,7 +427,7 @@ def main() -> int:
         out["grads_per_s"] = adjoint["mixed"]["grads_per_s"]
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(u, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(u, args.cpu_rows_per_thread)
     print(json.dumps(out), flush=True)
     return 0
 

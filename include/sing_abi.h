@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SING_ABI_VERSION 1
+#define SING_ABI_VERSION 2
 
 /* --- the 160-byte uniform block, byte-for-byte shared_shader/uniforms.h:54-95 --- */
 typedef struct SingVec4 {
@@ -138,6 +138,11 @@ typedef struct SingRenderStats {
     uint64_t chunks;      /* sample chunks the frame was split into */
     uint64_t rejected;    /* error-controlled stepping: rejected step attempts (each is 3 RK4 steps of work;
                              an accepted step is also 3) */
+    double trace32_ms;    /* the FP32 trace kernels alone (HIP events around each launch) */
+    double trace64_ms;    /* the FP64 trace kernels alone */
+    uint64_t steps32;     /* RK4 steps executed by the FP32 trace kernels, including rays later handed
+                             to the FP64 pass */
+    uint64_t steps64;     /* RK4 steps executed by the FP64 trace kernels */
 } SingRenderStats;
 
 typedef struct SingContext SingContext;

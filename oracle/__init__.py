@@ -66,6 +66,8 @@ def _type_oracle(L: C.CDLL) -> None:
     P = C.c_void_p
     L.orc_render.restype = C.c_int
     L.orc_render.argtypes = [P, C.c_int, P, P, C.c_int64, C.c_int, P, P, P, P, P, P]
+    L.orc_render_rows.restype = C.c_longlong
+    L.orc_render_rows.argtypes = [P, C.c_int, P, C.c_int64, C.c_int, P]
     L.orc_trace_ray.restype = C.c_int
     L.orc_trace_ray.argtypes = [P, C.c_int, P, C.c_uint32, C.c_uint32, C.c_double, C.c_double, P]
     L.orc_halton.restype = C.c_float
@@ -169,6 +171,18 @@ def render(u, precision: int = 32, pixels: np.ndarray | None = None, threads: in
                       _ptr(out["ldr"]), _ptr(out["rgba"]), _ptr(out["cls"]), _ptr(out["steps"]), _ptr(out["drift"]))
     assert rc == 0
     return out
+
+
+def render_rows(u, rows, precision: int = 32, threads: int = 0, lib: C.CDLL | None = None) -> tuple[np.ndarray, int]:
+    """Render whole rows with workers pulling the next row from an atomic counter
+    (the reference CPU renderer's schedule); returns (rgba (n, W, 4), RK4 steps)."""
+    L = lib or strict()
+    width = int.from_bytes(bytes(u)[112:116], "little")
+    r = np.ascontiguousarray(rows, dtype=np.int32)
+    rgba = np.zeros((r.size, width, 4), np.uint8)
+    steps = L.orc_render_rows(_ubuf(u), precision, _ptr(r), r.size, threads or (os.cpu_count() or 1), _ptr(rgba))
+    assert steps >= 0
+    return rgba, int(steps)
 
 
 def boundary_band(u, pixels: np.ndarray | None = None, eps: float = 1e-4, threads: int = 0,

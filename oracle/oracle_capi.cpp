@@ -63,6 +63,32 @@ void render_impl(const void* raw, const OrcExtras* ex, const std::int64_t* px, s
     });
 }
 
+// Whole image rows, as the reference's CPU renderer schedules them
+// (cli/main.cpp:739-740,861-872: hardware_concurrency() workers pulling the next
+// row from an atomic counter): `rows` lists the rows to render, rgba receives
+// n_rows * W * 4 bytes (row-major, in list order).  Returns the RK4 steps.
+template <class T>
+std::uint64_t rows_impl(const void* raw, const std::int32_t* rows, std::int64_t n_rows, int threads,
+                        std::uint8_t* rgba) {
+    const Uni<T> u = load_uni<T>(raw, nullptr);
+    const CamFrame<T> f = camera_frame(u);
+    std::uint32_t spp = u.supersample < 1u ? 1u : (u.supersample > 1024u ? 1024u : u.supersample);
+    std::vector<float> hal(2 * spp);
+    halton_table(spp, hal.data());
+    const std::uint32_t W = u.width;
+    std::atomic<std::uint64_t> total{0};
+    parallel_for(n_rows, threads, [&](std::int64_t i) {
+        std::uint64_t steps = 0;
+        for (std::uint32_t x = 0; x < W; ++x) {
+            const PixelOut<T> o = render_pixel(u, f, x, std::uint32_t(rows[i]), hal.data());
+            if (rgba) std::memcpy(rgba + 4 * (std::size_t(i) * W + x), o.rgba, 4);
+            steps += o.steps;
+        }
+        total.fetch_add(steps, std::memory_order_relaxed);
+    });
+    return total.load();
+}
+
 template <class T> void ray_impl(const void* raw, const OrcExtras* ex, std::uint32_t gx, std::uint32_t gy, double ox,
                                  double oy, double* out) {
     const Uni<T> u = load_uni<T>(raw, ex);
@@ -137,6 +163,13 @@ int orc_render(const void* uniforms, int precision, const OrcExtras* ex, const s
     else if (precision == 64) render_impl<double>(uniforms, ex, px, n, threads, hdr, ldr, rgba, cls, steps, drift);
     else return -1;
     return 0;
+}
+
+long long orc_render_rows(const void* uniforms, int precision, const std::int32_t* rows, std::int64_t n_rows,
+                          int threads, std::uint8_t* rgba) {
+    if (precision == 32) return (long long)rows_impl<float>(uniforms, rows, n_rows, threads, rgba);
+    if (precision == 64) return (long long)rows_impl<double>(uniforms, rows, n_rows, threads, rgba);
+    return -1;
 }
 
 int orc_trace_ray(const void* uniforms, int precision, const OrcExtras* ex, std::uint32_t gx, std::uint32_t gy,

@@ -77,7 +77,8 @@ class RenderStats(C.Structure):
     _fields_ = [
         ("kernel_ms", C.c_double), ("trace_ms", C.c_double), ("shade_ms", C.c_double), ("post_ms", C.c_double),
         ("rays", C.c_uint64), ("steps", C.c_uint64), ("fp64_rays", C.c_uint64), ("chunks", C.c_uint64),
-        ("rejected", C.c_uint64),
+        ("rejected", C.c_uint64), ("trace32_ms", C.c_double), ("trace64_ms", C.c_double), ("steps32", C.c_uint64),
+        ("steps64", C.c_uint64),
     ]
 
 

@@ -312,7 +312,8 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
     const std::size_t P = std::size_t(c->width) * c->height;
     int rc = c->ensure_records(std::size_t(S) * P);
     if (rc != SING_OK) return rc;
-    if ((rc = c->ensure_events(4u * n_chunks + 4u)) != SING_OK) return rc;
+    // per chunk: start, trace done, shade done, unused, and the FP32/FP64 trace kernels' own brackets
+    if ((rc = c->ensure_events(8u * n_chunks + 4u)) != SING_OK) return rc;
 
     sing::Chunk ch{};
     ch.rec = c->d_rec;
@@ -345,16 +346,16 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
         // the work queues restart per chunk; the frame counters keep accumulating
         if (k > 0 && (e = hipMemsetAsync(c->d_ctl, 0, offsetof(sing::QueueCtl, steps), c->stream)) != hipSuccess)
             return c->hip_fail(e, "hipMemsetAsync(queue)");
-        hipEventRecord(ev[4 * k + 0], c->stream);
-        if ((e = sing::launch_trace_chunk(sf, sd, offsets, ch, c->d_ctl, o.precision, shape, c->stream)) !=
-            hipSuccess)
+        hipEventRecord(ev[8 * k + 0], c->stream);
+        if ((e = sing::launch_trace_chunk(sf, sd, offsets, ch, c->d_ctl, o.precision, shape, c->stream,
+                                          ev + 8 * k + 4)) != hipSuccess)
             return c->hip_fail(e, "trace kernel launch");
-        hipEventRecord(ev[4 * k + 1], c->stream);
+        hipEventRecord(ev[8 * k + 1], c->stream);
         if ((e = sing::launch_shade_chunk(sf, ch, c->d_acc, k == 0, k + 1 == n_chunks, to, c->stream)) != hipSuccess)
             return c->hip_fail(e, "shade kernel launch");
-        hipEventRecord(ev[4 * k + 2], c->stream);
+        hipEventRecord(ev[8 * k + 2], c->stream);
     }
-    hipEvent_t post0 = ev[4 * n_chunks], post1 = ev[4 * n_chunks + 1];
+    hipEvent_t post0 = ev[8 * n_chunks], post1 = ev[8 * n_chunks + 1];
     if (o.post == SING_POST_BLOOM) {
         hipEventRecord(post0, c->stream);
         e = sing::launch_bloom(c->d_hdr16, c->width, c->height, u->bloom_threshold, u->bloom_strength, u->exposure,
@@ -368,16 +369,20 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
         return c->hip_fail(e, "hipMemcpyAsync(stats)");
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return c->hip_fail(e, "hipStreamSynchronize");
     c->outputs_valid = outs;
-    double trace_ms = 0.0, shade_ms = 0.0, post_ms = 0.0;
+    double trace_ms = 0.0, shade_ms = 0.0, post_ms = 0.0, t32_ms = 0.0, t64_ms = 0.0;
     for (std::uint32_t k = 0; k < n_chunks; ++k) {
-        float a = 0.0f, b = 0.0f;
-        hipEventElapsedTime(&a, ev[4 * k + 0], ev[4 * k + 1]);
-        hipEventElapsedTime(&b, ev[4 * k + 1], ev[4 * k + 2]);
+        float a = 0.0f, b = 0.0f, t32 = 0.0f, t64 = 0.0f;
+        hipEventElapsedTime(&a, ev[8 * k + 0], ev[8 * k + 1]);
+        hipEventElapsedTime(&b, ev[8 * k + 1], ev[8 * k + 2]);
+        hipEventElapsedTime(&t32, ev[8 * k + 4], ev[8 * k + 5]);
+        hipEventElapsedTime(&t64, ev[8 * k + 6], ev[8 * k + 7]);
         trace_ms += a;
         shade_ms += b;
+        t32_ms += t32;
+        t64_ms += t64;
     }
     float total = 0.0f;
-    hipEventElapsedTime(&total, ev[0], ev[4 * (n_chunks - 1) + 2]);
+    hipEventElapsedTime(&total, ev[0], ev[8 * (n_chunks - 1) + 2]);
     if (o.post == SING_POST_BLOOM) {
         float p = 0.0f;
         hipEventElapsedTime(&p, post0, post1);
@@ -395,6 +400,10 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
         stats->fp64_rays = ctl.rays64;
         stats->chunks = n_chunks;
         stats->rejected = ctl.rejected;
+        stats->trace32_ms = t32_ms;
+        stats->trace64_ms = t64_ms;
+        stats->steps32 = ctl.steps32;
+        stats->steps64 = ctl.steps64;
     }
     const std::uint64_t expect = std::uint64_t(P) * spp;
     if (ctl.rays != expect)

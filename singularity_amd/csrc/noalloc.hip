@@ -258,6 +258,8 @@ extern "C" int sing_render_noalloc(int device, const SingUniforms* u, const Sing
         stats->fp64_rays = hc.rays64;
         stats->chunks = n_chunks;
         stats->rejected = hc.rejected;
+        stats->steps32 = hc.steps32;
+        stats->steps64 = hc.steps64;
     }
     const std::uint64_t expect = std::uint64_t(P) * spp;
     if (hc.rays != expect)

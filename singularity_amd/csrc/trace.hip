@@ -51,7 +51,7 @@ void trace_kernel(SceneConst<float> sf, SceneConst<double> sd,
 
     bool retired = false, live = false, need = true;
     std::uint32_t rec_idx = 0;
-    std::uint64_t lane_steps = 0, lane_rays = 0;
+    std::uint64_t lane_steps = 0, lane_rays = 0, lane_exec = 0;
     std::uint32_t lane_rej = 0;
     RayState<S> y{};
 
@@ -142,6 +142,7 @@ void trace_kernel(SceneConst<float> sf, SceneConst<double> sd,
 
         // (3) emit finished rays; in the mixed pass, hand ambiguous outcomes to FP64
         if (term != kTermLive) {
+            lane_exec += y.steps;
             bool defer = false;
             if (PASS == 1) defer = needs_fp64(y, term);
             if (defer) {
@@ -161,9 +162,12 @@ void trace_kernel(SceneConst<float> sf, SceneConst<double> sd,
     for (int off = 32; off > 0; off >>= 1) {
         lane_steps += shfl_u64(lane_steps, lane ^ off);
         lane_rays += shfl_u64(lane_rays, lane ^ off);
+        lane_exec += shfl_u64(lane_exec, lane ^ off);
         if (ADAPT) rej += shfl_u64(rej, lane ^ off);
     }
     if (ADAPT && lane == 0 && rej) atomicAdd((unsigned long long*)&ctl->rejected, (unsigned long long)rej);
+    if (lane == 0 && lane_exec)
+        atomicAdd((unsigned long long*)(F64 ? &ctl->steps64 : &ctl->steps32), (unsigned long long)lane_exec);
     if (lane == 0 && lane_rays) {
         atomicAdd((unsigned long long*)&ctl->steps, (unsigned long long)lane_steps);
         atomicAdd((unsigned long long*)&ctl->rays, (unsigned long long)lane_rays);
@@ -326,7 +330,10 @@ __global__ __launch_bounds__(256) void shade_kernel(SceneConst<float> sc, const 
 
 hipError_t launch_trace_chunk(const SceneConst<float>& sf, const SceneConst<double>& sd, const float2* halton,
                               const Chunk& ch, QueueCtl* ctl, std::uint32_t mode, const LaunchShape& shape,
-                              hipStream_t stream) {
+                              hipStream_t stream, const hipEvent_t* ev) {
+    auto mark = [&](int i) {
+        if (ev) (void)hipEventRecord(ev[i], stream);
+    };
     const dim3 block(256);
     const std::uint32_t sb = shape.step_batch ? shape.step_batch : 32u;
     const bool kerr = sf.metric == SING_METRIC_KERR;
@@ -351,15 +358,27 @@ hipError_t launch_trace_chunk(const SceneConst<float>& sf, const SceneConst<doub
     } while (0)
     if (mode == SING_PREC_FP32) {
         SING_SETUP(0);
+        mark(0);
         SING_TRACE(0, shape.blocks);
+        mark(1);
+        mark(2);
+        mark(3);
     } else if (mode == SING_PREC_FP64) {
+        mark(0);
+        mark(1);
+        mark(2);
         SING_TRACE(2, shape.blocks64);
+        mark(3);
     } else {
         SING_SETUP(1);
+        mark(0);
         SING_TRACE(1, shape.blocks);
+        mark(1);
         if (sf.flags & SING_FLAG_STARFIELD_ON)
             hipLaunchKernelGGL(audit_kernel, dim3(shape.blocks), block, 0, stream, sf, ch, ctl);
+        mark(2);
         SING_TRACE(3, shape.blocks64);
+        mark(3);
     }
 #undef SING_TRACE
 #undef SING_SETUP

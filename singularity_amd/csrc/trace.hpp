@@ -54,6 +54,8 @@ struct QueueCtl {
     std::uint64_t rays;
     std::uint64_t rays64;
     std::uint64_t rejected;    // error-controlled stepping: rejected step attempts
+    std::uint64_t steps32;     // RK4 steps executed by the FP32 trace pass (deferred rays included)
+    std::uint64_t steps64;     // RK4 steps executed by the FP64 trace pass
 };
 
 // Work item of an FP32 pass that needs no trace (tile padding outside the
@@ -86,9 +88,12 @@ struct LaunchShape {
 
 // mode: SING_PREC_FP32 -> FP32 trace; SING_PREC_FP64 -> FP64 trace of every
 // ray; SING_PREC_MIXED -> FP32 trace with band deferral, then FP64 over the list.
+// ev (optional, 4 events): recorded on `stream` before and after the FP32 trace
+// kernel (ev[0], ev[1]) and before and after the FP64 trace kernel (ev[2],
+// ev[3]); a pass the mode does not run gets its two events back to back.
 hipError_t launch_trace_chunk(const SceneConst<float>& sf, const SceneConst<double>& sd, const float2* halton,
                               const Chunk& ch, QueueCtl* ctl, std::uint32_t mode, const LaunchShape& shape,
-                              hipStream_t stream);
+                              hipStream_t stream, const hipEvent_t* ev = nullptr);
 hipError_t launch_shade_chunk(const SceneConst<float>& sf, const Chunk& ch, PixAcc* acc, bool first, bool last,
                               const TraceOutputs& out, hipStream_t stream);
 

@@ -5,14 +5,17 @@ Counts are the arithmetic of the reference algorithm as written
 shared_shader/geodesic_math.h:101-134 for one Schwarzschild RK4 step), taken
 by instantiating the CPU restatement with an operation-counting scalar
 (tests/test_oracle_math.py::test_op_counts_are_stable keeps them in sync).
-Every add, multiply, divide and sin/cos counts as one flop.
+Flops are adds and multiplies (an FMA would count 2); divides and sin/cos are
+counted separately and are not in the flop figure.
 """
 
 KERR_STEP_OPS = {"add": 142, "mul": 348, "div": 53, "sqrt": 0, "trig": 9, "other": 0}
 SCHW_STEP_OPS = {"add": 80, "mul": 162, "div": 29, "sqrt": 0, "trig": 8, "other": 0}
 
-KERR_STEP_FLOPS = sum(KERR_STEP_OPS.values())  # 552
-SCHW_STEP_FLOPS = sum(SCHW_STEP_OPS.values())  # 279
+KERR_STEP_FLOPS = KERR_STEP_OPS["add"] + KERR_STEP_OPS["mul"]  # 490
+KERR_STEP_DIVS = KERR_STEP_OPS["div"]  # 53
+KERR_STEP_TRIG = KERR_STEP_OPS["trig"]  # 9
+SCHW_STEP_FLOPS = SCHW_STEP_OPS["add"] + SCHW_STEP_OPS["mul"]  # 242
 
 # MI355X published peaks
 PEAK_FP32_TFLOPS = 157.3       # vector/matrix FP32 (packed issue)
@@ -27,8 +30,16 @@ POST_BYTES_PER_PIXEL = 32
 
 # Measured HBM traffic of one mixed-precision FP32 trace dispatch at 1920x1080 x
 # 16 spp (33 177 600 rays): rocprofv3 FETCH_SIZE 376424 KiB x 2.0 + WRITE_SIZE
-# 522196 KiB x 1.0 (factors from the 256 MiB copy of singularity_amd/hbm_copy),
-# per traced ray: the set-up kernel's 16-byte init entry and 4-byte record index
-# read, the 16-byte termination record, deferral markers and list entries
-# written.
+# 522196 KiB x 1.0 (the factors are what the two counters read for a plain
+# 256 MiB device-to-device copy), per traced ray: the set-up kernel's 16-byte
+# init entry and 4-byte record index read, the 16-byte termination record,
+# deferral markers and list entries written.
 TRACE_HBM_BYTES_PER_RAY = (522196.0 * 1024 + 376424.0 * 1024 * 2.0) / 33177600.0
+
+# VALU instructions the FP32 Kerr trace kernel issues per RK4 step (SQ_INSTS_VALU
+# x active lanes / steps of a 1080p x 16 spp chunk) and the kernel's issue rate
+# in wave64 instructions per CU per clock at 2.4 GHz, against the 1.9 the
+# dual-issue calibration reaches (FP32 instructions with all-vector operands).
+KERR_STEP_VALU_INSTS = 280
+VALU_ISSUE_PER_CU_CLK = 1.73
+VALU_ISSUE_PEAK_PER_CU_CLK = 1.9

@@ -31,8 +31,9 @@ def test_distributed_launch_is_refused_with_a_message():
 def test_roofline_constants():
     import singularity_amd.flops as fl
 
-    # the per-step flops are the op-counted reference step (add, mul, div, trig at 1 each)
-    assert fl.KERR_STEP_FLOPS == sum(fl.KERR_STEP_OPS.values()) == 552
+    # the per-step flops are the op-counted reference step's adds and multiplies; divides and trig apart
+    assert fl.KERR_STEP_FLOPS == fl.KERR_STEP_OPS["add"] + fl.KERR_STEP_OPS["mul"] == 490
+    assert (fl.KERR_STEP_DIVS, fl.KERR_STEP_TRIG) == (53, 9)
     # measured HBM bytes per traced ray: 16-byte init entry + 4-byte record index
     # read, 16-byte record written, plus deferral markers and list entries
     assert 36.0 < fl.TRACE_HBM_BYTES_PER_RAY < 42.0

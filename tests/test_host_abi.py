@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 20
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
-    assert L.sing_abi_version() == 1
+    assert L.sing_abi_version() == 2
     assert L.sing_backend_name() == b"HIP-gfx950"
 
 
@@ -60,7 +60,7 @@ def test_struct_sizes_match_header_layout():
     # sizes the C compiler gives the ABI structs (include/sing_abi.h), mirrored by ctypes
     assert C.sizeof(sa.Uniforms) == 160
     assert C.sizeof(sa.RenderOpts) == 64
-    assert C.sizeof(sa.RenderStats) == 72
+    assert C.sizeof(sa.RenderStats) == 104
     assert C.sizeof(sa.DiffOpts) == 40 and C.sizeof(sa.DiffStats) == 64
     assert C.sizeof(sa.DiffParams) == 40 and C.sizeof(sa.DiffCamera) == 24
     assert C.sizeof(sa.BatchItem) == 32 and C.sizeof(sa.BatchOpts) == 48 and C.sizeof(sa.BatchStats) == 64

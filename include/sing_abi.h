@@ -317,6 +317,13 @@ int sing_diff_loss_grad(SingContext* ctx, const SingUniforms* base, const SingDi
                         const SingDiffCamera* cam, const SingDiffOpts* opts, const float* target,
                         const float* weights, double* loss, double* grad, float* hdr_out, SingDiffStats* stats);
 
+/* Per-pixel results of the last sing_diff_* call on ctx (either may be NULL):
+ * grad_out W*H*5 doubles, the pixel's loss gradient wrt the 5 parameters (r_in
+ * partial; zero after sing_diff_forward); meta_out W*H*4 u32: RK4 steps,
+ * accepted disc crossings, termination (1 horizon, 2 escape, 3 disc, 4 budget,
+ * 5 init fail) and the precision the pixel ran in (0 FP32, 1 FP64). */
+int sing_diff_capture_pixels(SingContext* ctx, double* grad_out, uint32_t* meta_out);
+
 /* --- batched render ------------------------------------------------------------
  * No counterpart in the reference (cuda_cli renders one scene per launch,
  * cuda_cli/main.cpp:243-299).  Renders n scenes of any sizes and sample counts

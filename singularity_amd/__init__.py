@@ -372,6 +372,18 @@ _SIGS.update({
 })
 
 
+_SIGS["sing_diff_capture_pixels"] = (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p])
+
+
+def diff_capture_pixels(be: "HipBackend") -> tuple[np.ndarray, np.ndarray]:
+    """Per-pixel gradient (H, W, 5) and (steps, crossings, termination, precision) (H, W, 4)
+    of the last differentiable call."""
+    g = np.empty((be.height, be.width, 5), np.float64)
+    m = np.empty((be.height, be.width, 4), np.uint32)
+    be._check(lib().sing_diff_capture_pixels(be._ctx, g.ctypes.data, m.ctypes.data))
+    return g, m
+
+
 def diff_base_uniforms(width: int, height: int, starfield: bool = False, peak_T: float = 4500.0,
                        h_step: float = 0.4, max_steps: int = 2500, escape_M: float = 100.0) -> Uniforms:
     """Base block of the differentiable Kerr scene: resolution, integrator and shading settings."""
@@ -426,17 +438,12 @@ def adam_fit(be: "HipBackend", base: Uniforms, start, target: np.ndarray, cam: D
              clip: float = 2.0, lr_floor: float = 0.003, callback=None):
     """Adam over the 5 scene parameters (entries with free == False stay fixed).
 
-    The learning rate follows a cosine decay from lr to lr_floor * lr (the
-    floor of 0.003 lets the last steps settle within the noise of the
-    photon-orbit pixels described below; 0.01 left the 512x512 Kerr fit
-    wandering at ~100x the loss on some starts).  After 5
+    The learning rate follows a cosine decay from lr to lr_floor * lr.  After 5
     steps each gradient entry is clipped to clip x its running RMS: pixels whose
     rays pass near the photon orbit have steep but finite derivatives, and one
     such spike would otherwise inflate Adam's second moment for hundreds of
-    steps.  Near the optimum those pixels keep the normalised step from
-    vanishing, so the iterates wander within the noise; the result is the
-    evaluated iterate of lowest loss.  Returns (params, history of (loss,
-    params, grad)).
+    steps.  Returns (the parameters after the last update, history of (loss,
+    params, grad) of every evaluated iterate).
     """
     x = np.array(start, np.float64)
     lr = np.array([0.03, 0.015, 0.075, 0.3, 0.015]) if lr is None else np.asarray(lr, np.float64)
@@ -464,8 +471,7 @@ def adam_fit(be: "HipBackend", base: Uniforms, start, target: np.ndarray, cam: D
         step = sched * lr * (m / (1 - beta1**k)) / (np.sqrt(v / (1 - beta2**k)) + 1e-300)
         x = x - np.where(free, step, 0.0)
         x[0] = min(max(x[0], 0.0), 0.99)
-    best = min(range(len(history)), key=lambda i: history[i][0])
-    return history[best][1].copy(), history
+    return x, history
 
 
 # --- batched render (include/sing_abi.h: sing_render_batch) -------------------

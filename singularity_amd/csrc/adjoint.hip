@@ -706,6 +706,17 @@ int sing_diff_loss_grad(SingContext* ctx, const SingUniforms* base, const SingDi
     return diff_run(ctx, base, p, cam, opts, target, weights, true, loss, grad, hdr_out, stats);
 }
 
+int sing_diff_capture_pixels(SingContext* c, double* grad_out, uint32_t* meta_out) {
+    if (!c) return SING_ERR_ARG;
+    if (!c->diff_P || !c->d_dset) return c->fail(SING_ERR_STATE, "sing_diff_capture_pixels: no differentiable frame");
+    hipError_t e = hipSetDevice(c->device);
+    const std::size_t P = c->diff_P;
+    if (e == hipSuccess && grad_out) e = hipMemcpy(grad_out, c->diff.grad, P * 5 * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && meta_out) e = hipMemcpy(meta_out, c->diff.meta, P * 4 * sizeof(std::uint32_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return c->hip_fail(e, "sing_diff_capture_pixels");
+    return SING_OK;
+}
+
 // ---- diagnostics: the adjoint math evaluated on the host
 void sing_diag_kerr_rhs(const double* y, const double* c, double* out) {
     const sing::KY<double> d = sing::kerr_rhs<double>({y[0], y[1], y[2], y[3], y[4]}, {c[0], c[1], c[2], c[3]});

@@ -8,12 +8,25 @@ and its target is rendered at (0.7, 70 deg, ISCO, 20, 1.0).
 Bars (tolerances stated here):
   FP64 forward vs FP64 oracle          |dI| <= 1e-9 + 6e-8 |I| (float32 output rounding) on >= 99.9 %
   FP64 gradient vs oracle dual numbers |dg_i| <= 1e-6 max_j |g_j|  (same discrete model, exact derivative)
-  FP64 gradient vs central FD h=1e-3   |dg_i| <= 1e-3 |g_i| + 1e-6 max_j |g_j|, on pixels whose
+  FP64 gradient vs central FD h=1e-3   |dg_i| <= 1e-3 |g_i| + 1e-6 max_j |g_j|, on the smooth pixels:
                                        FD at h and h/2 agree to 3e-4 (no event inside the stencil and
-                                       the O(h^2) truncation below 4e-4)
-  FP32 / mixed gradient vs FP64        relative L2 error <= 5e-2 / 2e-2
+                                       the O(h^2) truncation below 4e-4) and the second difference
+                                       I(x+h) - 2 I(x) + I(x-h) is below 0.05 h |dI/dx| (no event
+                                       narrower than h/2 at x itself: a ray that grazes the equator
+                                       crosses it only in a window of the parameters that can be
+                                       narrower than the stencil)
+  mixed gradient vs FP64               relative L2 error <= 1e-4, each entry within 1e-3 relative
+                                       (+1e-6 of the largest); plain FP32 (not the shipped form:
+                                       its few photon-orbit and near-hole rays carry ~100 % of its
+                                       error) relative L2 <= 5e-2
   checkpoint interval                  gradient bit-identical for C = 1, 16, 64, 4096
   determinism                          loss and gradient bit-identical on repeat
+  512 x 512 inverse-render workload (r_in = ISCO(a), mixed precision):
+    mixed vs FP64                      relative L2 <= 1e-4, each entry within 1e-3 relative (+1e-6)
+    mixed vs central FD h = 1e-3       each entry within 1e-3 relative (+1e-6 of the largest), on the
+                                       smooth-pixel mask (FD of the FP64 forward at h and h/2 agree)
+    Adam, 200 steps                    the last iterate: |da| <= 0.01, |di| <= 0.5 deg,
+                                       |dr_out| <= 0.25 M, |dTs| <= 0.01, loss below 0.1 % of the start
 """
 
 from __future__ import annotations
@@ -130,17 +143,25 @@ def test_gradient_fp64_matches_oracle_dual_numbers(be, base, orc, target):
         assert st.rays == W * H and st.steps > 0
 
 
+def smooth(i0, ip, im, ip2, im2, h):
+    """Pixels whose radiance is smooth over [x - h, x + h] in one parameter: the
+    central differences at h and h/2 agree, and the second difference is small."""
+    d1 = (ip - im) / (2 * h)
+    d2 = (ip2 - im2) / h
+    bad = np.abs(d1 - d2) > 3e-4 * (np.abs(d1) + 1e-3)
+    bad |= np.abs(ip - 2.0 * i0 + im) > 0.05 * h * (np.abs(d1) + 1e-3)
+    return ~bad.any(-1)
+
+
 def smooth_pixel_mask(be, base, x, h=1e-3):
-    """Pixels whose radiance is smooth in every parameter over [x - h, x + h]:
-    the central differences at h and h/2 agree (no event inside the stencil)."""
+    """Pixels whose radiance is smooth in every parameter over [x - h, x + h]."""
     mask = np.ones((H, W), bool)
+    i0 = gpu_hdr(be, base, x)
     for i in range(5):
         e = np.zeros(5)
         e[i] = h
-        d1 = (gpu_hdr(be, base, x + e) - gpu_hdr(be, base, x - e)) / (2 * h)
-        d2 = (gpu_hdr(be, base, x + e / 2) - gpu_hdr(be, base, x - e / 2)) / h
-        bad = np.abs(d1 - d2) > 3e-4 * (np.abs(d1) + 1e-3)
-        mask &= ~bad.any(-1)
+        mask &= smooth(i0, gpu_hdr(be, base, x + e), gpu_hdr(be, base, x - e), gpu_hdr(be, base, x + e / 2),
+                       gpu_hdr(be, base, x - e / 2), h)
     return mask
 
 
@@ -163,16 +184,21 @@ def test_gradient_fp64_matches_central_differences(be, base, target):
     assert np.all(np.abs(g - fd) <= 1e-3 * np.abs(fd) + 1e-6 * np.abs(fd).max()), (g, fd)
 
 
+def close_entries(g, ref, rel=1e-3, floor=1e-6):
+    return np.all(np.abs(g - ref) <= rel * np.abs(ref) + floor * np.abs(ref).max())
+
+
 def test_gradient_fp32_and_mixed_close_to_fp64(be, base, target):
-    x = 0.5 * (START + TARGET)
-    _, g64, _, _ = gpu_grad(be, base, x, target)
-    for prec, tol in ((sa.PREC_FP32, 5e-2), (sa.PREC_MIXED, 2e-2)):
-        _, g, _, st = gpu_grad(be, base, x, target, prec)
-        err = np.linalg.norm(g - g64) / np.linalg.norm(g64)
-        print(prec, g, "rel", err, "fp64 rays", st.fp64_rays, flush=True)
-        assert err <= tol, (prec, err, g, g64)
-        if prec == sa.PREC_MIXED:
-            assert 0 < st.fp64_rays < W * H
+    for x in (START, 0.5 * (START + TARGET)):
+        _, g64, _, _ = gpu_grad(be, base, x, target)
+        for prec, tol in ((sa.PREC_FP32, 5e-2), (sa.PREC_MIXED, 1e-4)):
+            _, g, _, st = gpu_grad(be, base, x, target, prec)
+            err = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+            print(prec, g, "rel", err, "fp64 rays", st.fp64_rays, flush=True)
+            assert err <= tol, (prec, err, g, g64)
+            if prec == sa.PREC_MIXED:
+                assert 0 < st.fp64_rays < W * H
+                assert close_entries(g, g64), (g, g64)
 
 
 def test_checkpoint_interval_does_not_change_gradient(be, base, target):
@@ -245,25 +271,63 @@ def test_gradients_finite_and_consistent_at_512(be512, workload):
         assert st.rays == WL * WL
         out[prec] = g
     g64 = out[sa.PREC_FP64]
-    assert np.linalg.norm(out[sa.PREC_MIXED] - g64) <= 2e-2 * np.linalg.norm(g64)
+    rel = np.linalg.norm(out[sa.PREC_MIXED] - g64) / np.linalg.norm(g64)
+    print("512 mixed vs fp64: rel L2", rel, out[sa.PREC_MIXED], g64, flush=True)
+    assert rel <= 1e-4
+    assert close_entries(out[sa.PREC_MIXED], g64)
     assert np.linalg.norm(out[sa.PREC_FP32] - g64) <= 5e-2 * np.linalg.norm(g64)
+
+
+def test_mixed_gradient_matches_central_differences_at_512(be512, workload):
+    """The shipped form (mixed precision, r_in = ISCO(a)) against central differences
+    of the FP64 forward, h = 1e-3 on each parameter, on the pixels whose radiance is
+    smooth over the stencil (smooth(): the FD at h and h/2 agree to 3e-4, the second
+    difference is small)."""
+    base, tgt = workload
+    x = START.copy()
+    h = 1e-3
+    t = tgt.astype(np.float64)
+
+    def fwd(xx):
+        print(f"diff_forward {WL}x{WL} fp64 x={np.round(xx, 5)}", flush=True)
+        hdr, _ = sa.diff_forward(be512, base, sa.DiffParams.from_array(xx), cam(), opts(sa.PREC_FP64, isco=1))
+        return hdr.astype(np.float64)
+
+    free = [0, 1, 3, 4]  # r_in follows a (ISCO), so it is not a free parameter here
+    plus, minus, mask = {}, {}, np.ones((WL, WL), bool)
+    i0 = fwd(x)
+    for i in free:
+        e = np.zeros(5)
+        e[i] = h
+        plus[i], minus[i] = fwd(x + e), fwd(x - e)
+        mask &= smooth(i0, plus[i], minus[i], fwd(x + e / 2), fwd(x - e / 2), h)
+    print("smooth-pixel fraction", mask.mean(), flush=True)
+    assert mask.mean() > 0.8
+    wts = mask.astype(np.float32)
+    _, g, _, st = sa.diff_loss_grad(be512, base, sa.DiffParams.from_array(x), cam(), tgt, wts,
+                                    opts(sa.PREC_MIXED, isco=1))
+    assert st.fp64_rays > 0
+    fd = np.zeros(5)
+    for i in free:
+        lp = (wts[..., None] * (plus[i] - t) ** 2).sum()
+        lm = (wts[..., None] * (minus[i] - t) ** 2).sum()
+        fd[i] = (lp - lm) / (2 * h)
+    print("adjoint", g[free], "\nFD     ", fd[free], "\nrel", np.abs(g[free] - fd[free]) / np.abs(fd[free]), flush=True)
+    assert close_entries(g[free], fd[free]), (g[free], fd[free])
 
 
 def test_adam_recovers_parameters(be512, workload):
     """200 Adam steps from (0.5, 60 deg, 15, 0.8) to the target (0.7, 70 deg, 20, 1.0),
-    r_in = ISCO(a), mixed precision.  Bars on the returned (lowest-loss) iterate:
-    |da| <= 0.01, |di| <= 0.5 deg, |dr_out| <= 0.25 M, |dTs| <= 0.01, loss below
-    0.1 % of the start; the last iterate stays within 2x the parameter bars."""
+    r_in = ISCO(a), mixed precision.  Bars on the last iterate: |da| <= 0.01,
+    |di| <= 0.5 deg, |dr_out| <= 0.25 M, |dTs| <= 0.01, loss below 0.1 % of the start."""
     base, tgt = workload
     print(f"adam {WL}x{WL} mixed, 200 steps", flush=True)
     x, hist = sa.adam_fit(be512, base, START, tgt, cam(), steps=200, opts=opts(sa.PREC_MIXED, isco=1))
-    best = min(h[0] for h in hist)
-    print("best", x, "loss", hist[0][0], "->", best, "last", hist[-1][1], hist[-1][0], flush=True)
+    loss, _, _, _ = sa.diff_loss_grad(be512, base, sa.DiffParams.from_array(x), cam(), tgt, None,
+                                      opts(sa.PREC_MIXED, isco=1))
+    print("last", x, "loss", hist[0][0], "->", loss, flush=True)
     assert abs(x[0] - 0.7) <= 0.01
     assert abs(np.degrees(x[1]) - 70.0) <= 0.5
     assert abs(x[3] - 20.0) <= 0.25
     assert abs(x[4] - 1.0) <= 0.01
-    assert best <= 1e-3 * hist[0][0]
-    last = hist[-1][1]
-    assert abs(last[0] - 0.7) <= 0.02 and abs(np.degrees(last[1]) - 70.0) <= 1.0
-    assert abs(last[3] - 20.0) <= 0.5 and abs(last[4] - 1.0) <= 0.02
+    assert loss <= 1e-3 * hist[0][0]

@@ -166,6 +166,27 @@ def c2_leg(frames: int = 3, seed: int = 20261203) -> dict:
     return res
 
 
+def post_leg(reps: int = 5) -> dict:
+    """The bloom post pass (extract, 7-tap H and V blurs, composite, ACES, sRGB) on
+    1-spp FP32 frames of the 1080p Kerr scene at 1920x1080 and 3840x2160: HIP-event
+    time of the post kernel per frame (best of reps) and HBM GB/s in the four-pass
+    algorithm's bytes (32 B per full-res pixel)."""
+    res = {}
+    for W, H in ((1920, 1080), (3840, 2160)):
+        u = kerr_uniforms(W, H, 1, 0.94, 25.0, 40.0, 80.0, 30.0)
+        be = sa.HipBackend(W, H)
+        o = sa.RenderOpts(precision=sa.PREC_FP32, outputs=sa.OUT_RGBA8, post=sa.POST_BLOOM)
+        be.render(u, o)
+        ms = min(be.render(u, o).post_ms for _ in range(reps))
+        be.shutdown()
+        res[f"{W}x{H}"] = {"post_ms": round(ms, 4),
+                           "hbm_gbs": round(FL.POST_BYTES_PER_PIXEL * W * H / (ms * 1e-3) / 1e9, 1),
+                           "frac_of_hbm_peak": round(FL.POST_BYTES_PER_PIXEL * W * H / (ms * 1e-3) / 1e9
+                                                     / FL.PEAK_HBM_GBS, 3)}
+        log(f"post {W}x{H}: {res[f'{W}x{H}']}")
+    return res
+
+
 def c3_leg() -> dict:
     """Kerr a=0.998 3840x2160 256 spp, incl 60 deg, disc [ISCO, 30 M], cuda_cli camera
     defaults (30 M, FOV 60), fixed h = 0.4: plain FP32, mixed and all-FP64 frames."""
@@ -419,6 +440,7 @@ def main() -> int:
     be.shutdown()
     if not args.no_extra:
         out["adaptive_bloom_1080p"] = c2_leg()
+        out["post_passes"] = post_leg()
         out["kerr_a0998_4k"] = c3_leg()
         out["batch_64_scenes"] = c4_leg()
         out["integrator_fan"] = fan_leg()

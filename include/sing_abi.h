@@ -186,8 +186,8 @@ int sing_capture_hdr16(SingContext* ctx, uint16_t* dst, size_t bytes);
  * device also looks up the array and uploads the 8 KiB sample-offset tables
  * (make it before any graph capture).  Calls that overlap in time on one GPU
  * share the array and race.  Output pointers: rgba W*H*4, hdr W*H*3 floats,
- * hdr16 W*H*4 halves, cls W*H, steps W*H u32, each required when its
- * SING_OUT_* bit (or bloom, for rgba) is selected.  stats: rays, steps,
+ * hdr16 W*H*4 halves (16-byte aligned), cls W*H, steps W*H u32, each required
+ * when its SING_OUT_* bit (or bloom, for rgba) is selected.  stats: rays, steps,
  * fp64_rays, chunks, rejected (timings are 0).  Errors: sing_last_error(NULL). */
 typedef struct SingDeviceOutputs {
     uint8_t* rgba;

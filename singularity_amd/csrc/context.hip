@@ -58,28 +58,22 @@ void SingContext::free_frame() {
     hipFree(d_hdr16);
     hipFree(d_cls);
     hipFree(d_steps);
-    hipFree(d_bloom_a);
-    hipFree(d_bloom_b);
     hipFree(d_acc);
     d_rgba = nullptr;
     d_hdr = nullptr;
     d_hdr16 = nullptr;
     d_cls = nullptr;
     d_steps = nullptr;
-    d_bloom_a = nullptr;
-    d_bloom_b = nullptr;
     d_acc = nullptr;
     outputs_valid = 0;
 }
 
 int SingContext::alloc_frame(std::uint32_t w, std::uint32_t h) {
     const std::size_t n = std::size_t(w) * h;
-    const std::size_t nb = std::size_t(std::max(w / 2u, 1u)) * std::max(h / 2u, 1u);
     hipError_t e;
     if ((e = hipMalloc(&d_rgba, n * 4)) != hipSuccess || (e = hipMalloc(&d_hdr, n * 12)) != hipSuccess ||
         (e = hipMalloc(&d_hdr16, n * 8)) != hipSuccess || (e = hipMalloc(&d_cls, n)) != hipSuccess ||
-        (e = hipMalloc(&d_steps, n * 4)) != hipSuccess || (e = hipMalloc(&d_bloom_a, nb * 8)) != hipSuccess ||
-        (e = hipMalloc(&d_bloom_b, nb * 8)) != hipSuccess ||
+        (e = hipMalloc(&d_steps, n * 4)) != hipSuccess ||
         (e = hipMalloc(&d_acc, n * sizeof(sing::PixAcc))) != hipSuccess) {
         free_frame();
         return fail(SING_ERR_NOMEM, "hipMalloc of %ux%u frame buffers failed: %s", w, h, hipGetErrorString(e));
@@ -359,7 +353,6 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
     if (o.post == SING_POST_BLOOM) {
         hipEventRecord(post0, c->stream);
         e = sing::launch_bloom(c->d_hdr16, c->width, c->height, u->bloom_threshold, u->bloom_strength, u->exposure,
-                               reinterpret_cast<uint2*>(c->d_bloom_a), reinterpret_cast<uint2*>(c->d_bloom_b),
                                c->d_rgba, c->stream);
         if (e != hipSuccess) return c->hip_fail(e, "bloom launch");
         hipEventRecord(post1, c->stream);

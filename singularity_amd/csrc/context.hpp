@@ -38,8 +38,6 @@ struct SingContext {
     uint2* d_hdr16 = nullptr;
     std::uint8_t* d_cls = nullptr;
     std::uint32_t* d_steps = nullptr;
-    float* d_bloom_a = nullptr;  // half-res ping-pong (RGBA16F stored as uint2)
-    float* d_bloom_b = nullptr;
     sing::PixAcc* d_acc = nullptr;
     // per-chunk ray records and the deferred (FP64) list
     sing::Rec* d_rec = nullptr;

@@ -4,8 +4,8 @@
 // capture, serving loops, harnesses that forbid hipMalloc.  The caller passes
 // the uniform block and device output buffers; every working buffer (queue
 // counters, FP64 scene constants, sample-offset tables, per-pixel
-// accumulators, ray records, the FP64 list, the set-up arrays, bloom
-// ping-pong) is carved from `scratch`, one file-scope device array of
+// accumulators, ray records, the FP64 list, the set-up arrays, the bloom
+// pass's half-float HDR input) is carved from `scratch`, one file-scope device array of
 // kScratchBytes per GPU that lives until the process ends.
 //
 // Host code never passes `scratch` itself to a kernel or copy: in host code the
@@ -138,10 +138,10 @@ extern "C" int sing_render_noalloc(int device, const SingUniforms* u, const Sing
         ((outs & SING_OUT_STEPS) && !out->steps))
         return fail(SING_ERR_ARG, "sing_render_noalloc: an output selected in opts->outputs has no buffer");
     auto misaligned = [](const void* p, std::uintptr_t a) { return (reinterpret_cast<std::uintptr_t>(p) & (a - 1u)) != 0; };
-    if (misaligned(out->rgba, 4) || misaligned(out->hdr, 4) || misaligned(out->hdr16, 8) ||
+    if (misaligned(out->rgba, 4) || misaligned(out->hdr, 4) || misaligned(out->hdr16, 16) ||
         misaligned(out->steps, 4))
-        return fail(SING_ERR_ARG, "sing_render_noalloc: output buffers must be aligned to their element size "
-                                  "(rgba 4, hdr 4, hdr16 8, steps 4 bytes)");
+        return fail(SING_ERR_ARG, "sing_render_noalloc: output buffers must be aligned to rgba 4, hdr 4, hdr16 16, "
+                                  "steps 4 bytes");
 
     const sing::SceneConst<float> sf = sing::make_scene_const<float>(*u, o);
     const sing::SceneConst<double> sd = sing::make_scene_const<double>(*u, o);
@@ -149,7 +149,6 @@ extern "C" int sing_render_noalloc(int device, const SingUniforms* u, const Sing
     const std::size_t P = std::size_t(W) * H;
     const bool bloom = o.post == SING_POST_BLOOM;
     const bool hdr16_in_scratch = bloom && !(outs & SING_OUT_HDR16);
-    const std::size_t nb = std::size_t(std::max(W / 2u, 1u)) * std::max(H / 2u, 1u);
 
     sing::Chunk ch{};
     ch.P = std::uint32_t(P);
@@ -157,10 +156,9 @@ extern "C" int sing_render_noalloc(int device, const SingUniforms* u, const Sing
     ch.n_tiles = ch.tiles_x * ((H + 7u) / 8u);
     const std::size_t items_per_sample = std::size_t(ch.n_tiles) * 64u;
 
-    // layout: head | accumulators | [hdr16] | [bloom a, b] | records | [FP64 list] | [set-up arrays]
+    // layout: head | accumulators | [hdr16] | records | [FP64 list] | [set-up arrays]
     std::size_t fixed = kHeadBytes + round256(P * sizeof(sing::PixAcc));
     if (hdr16_in_scratch) fixed += round256(P * 8u);
-    if (bloom) fixed += 2u * round256(nb * 8u);
     const bool mixed = o.precision == SING_PREC_MIXED, fp32_items = o.precision != SING_PREC_FP64;
     const std::size_t per_sample = P * sizeof(sing::Rec) + (mixed ? P * 4u : 0u) +
                                    (fp32_items ? items_per_sample * (sizeof(float4) + 4u) : 0u);
@@ -192,8 +190,6 @@ extern "C" int sing_render_noalloc(int device, const SingUniforms* u, const Sing
     sing::PixAcc* acc = reinterpret_cast<sing::PixAcc*>(take(P * sizeof(sing::PixAcc)));
     uint2* hdr16 = reinterpret_cast<uint2*>(out->hdr16);
     if (hdr16_in_scratch) hdr16 = reinterpret_cast<uint2*>(take(P * 8u));
-    uint2* bloom_a = bloom ? reinterpret_cast<uint2*>(take(nb * 8u)) : nullptr;
-    uint2* bloom_b = bloom ? reinterpret_cast<uint2*>(take(nb * 8u)) : nullptr;
     ch.rec = reinterpret_cast<sing::Rec*>(take(S * P * sizeof(sing::Rec)));
     ch.band = mixed ? reinterpret_cast<std::uint32_t*>(take(S * P * 4u)) : nullptr;
     ch.init = fp32_items ? reinterpret_cast<float4*>(take(S * items_per_sample * sizeof(float4))) : nullptr;
@@ -243,7 +239,7 @@ extern "C" int sing_render_noalloc(int device, const SingUniforms* u, const Sing
             return fail(SING_ERR_HIP, "sing_render_noalloc: shade launch: %s", hipGetErrorString(e));
     }
     if (bloom) {
-        e = sing::launch_bloom(hdr16, W, H, u->bloom_threshold, u->bloom_strength, u->exposure, bloom_a, bloom_b,
+        e = sing::launch_bloom(hdr16, W, H, u->bloom_threshold, u->bloom_strength, u->exposure,
                                reinterpret_cast<uchar4*>(out->rgba), stream);
         if (e != hipSuccess) return fail(SING_ERR_HIP, "sing_render_noalloc: bloom launch: %s", hipGetErrorString(e));
     }

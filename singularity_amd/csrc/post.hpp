@@ -7,9 +7,9 @@
 
 namespace sing {
 
-// hdr16: W*H RGBA16F.  bloom_a/bloom_b: floor(W/2)*floor(H/2) RGBA16F scratch.
+// hdr16: W*H RGBA16F (16-byte aligned); the half-res bloom texels live in LDS.
 // rgba: W*H RGBA8 composite (hdr + bloom*strength)*exposure -> ACES -> sRGB.
 hipError_t launch_bloom(const uint2* hdr16, std::uint32_t W, std::uint32_t H, float threshold, float strength,
-                        float exposure, uint2* bloom_a, uint2* bloom_b, uchar4* rgba, hipStream_t stream);
+                        float exposure, uchar4* rgba, hipStream_t stream);
 
 }  // namespace sing

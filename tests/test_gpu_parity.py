@@ -175,8 +175,14 @@ def test_starfield_off_gives_black_sky(backends, orc):
     assert o["hdr"].max() == 0.0
 
 
-def test_bloom_post_pass_matches_oracle(backends, orc):
-    u = config2(160, 90, 2)
+@pytest.mark.parametrize("size", [(160, 90, 2), (301, 173, 1), (1920, 1080, 1), (3, 3, 1)])
+def test_bloom_post_pass_matches_oracle(backends, orc, size):
+    """The one-launch tiled post pass against the oracle's four full-image passes
+    (half-float storage between them): multi-tile frames, odd sizes (the half-res
+    image is floor(W/2) x floor(H/2)), a frame smaller than one tile."""
+    W, H, spp = size
+    print(f"bloom {W}x{H} spp {spp}", flush=True)
+    u = config2(W, H, spp)
     f = render(backends, u, outputs=sa.OUT_HDR16 | sa.OUT_RGBA8, post=sa.POST_BLOOM)
     hdr = f.hdr16[..., :3].astype(np.float32)
     ref = orc.bloom_pipeline(hdr, u.bloom_threshold, u.bloom_strength, u.exposure, half_store=True)

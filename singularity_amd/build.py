@@ -7,6 +7,8 @@ source tree:
     singularity_amd/libsing_hip.so   the C-ABI library (include/sing_abi.h)
     singularity_amd/hip_cli          flag-compatible replacement of cuda_cli
     singularity_amd/hip_diag         the CPU driver's kerr-geometry / photon-orbit / benchmark modes
+    singularity_amd/hip_backend_check  the HipBackend : RenderBackend adapter (include/sing_hip_backend.hpp)
+                                     built against the reference's render_backend.hpp, where that tree exists
 
 Usage: python -m singularity_amd.build  (or import and call build()).
 """
@@ -27,6 +29,7 @@ OBJ = PKG / "_build"
 LIB = PKG / "libsing_hip.so"
 CLI = PKG / "hip_cli"
 DIAG = PKG / "hip_diag"
+ADAPTER = PKG / "hip_backend_check"  # include/sing_hip_backend.hpp driven through the reference's RenderBackend
 
 ARCH = "gfx950"
 HIP_FLAGS = [
@@ -109,6 +112,20 @@ def build(verbose: bool = False) -> Path:
             ]
         )
         os.replace(tmpd, DIAG)
+    # the RenderBackend adapter (include/sing_hip_backend.hpp) compiled against the
+    # reference's own interface header, where the reference tree is present
+    ref_hdr = Path("/root/reference/render/include")
+    if (ref_hdr / "render_backend.hpp").exists():
+        tmpa = ADAPTER.with_suffix(".tmp")
+        _run(
+            [
+                "g++", "-O2", "-std=c++20", "-Wall", "-Wextra", "-o", str(tmpa), str(ROOT / "tests" / "hip_backend_check.cpp"),
+                f"-I{ref_hdr}", f"-I{ROOT / 'include'}",
+                f"-L{PKG}", "-lsing_hip", "-Wl,-rpath,$ORIGIN",
+                "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib",
+            ]
+        )
+        os.replace(tmpa, ADAPTER)
     return LIB
 
 

@@ -247,7 +247,8 @@ def c4_leg(seed: int = 20261203, reps: int = 2) -> dict:
     be = sa.HipBackend(128, 128)
     opts = sa.BatchOpts(precision=sa.PREC_MIXED, use_graph=1, autotune=1)
     _, st = sa.render_batch(be, scenes, opts)  # tunes, records the graph
-    log(f"c4: tuned step_batch={st.step_batch} waves_per_cu={st.waves_per_cu}, {st.kernel_ms:.0f} ms")
+    log(f"c4: tuned step_batch={st.step_batch} waves_per_cu={st.waves_per_cu} rays_per_refill={st.rays_per_refill} "
+        f"queue_chunk={st.queue_chunk}, {st.kernel_ms:.0f} ms")
     opts.autotune = 0
     times, walls = [], []
     for _ in range(reps):
@@ -269,7 +270,8 @@ def c4_leg(seed: int = 20261203, reps: int = 2) -> dict:
         "wall_ms_incl_output_copies": round(float(np.mean(walls)) * 1e3, 1), "copy_ms": round(st.copy_ms, 1),
         "steps": int(st.steps), "fp64_rays": int(st.fp64_rays), "groups": int(st.groups),
         "graph_replayed": int(st.graph_replayed), "step_batch": int(st.step_batch),
-        "waves_per_cu": int(st.waves_per_cu),
+        "waves_per_cu": int(st.waves_per_cu), "rays_per_refill": int(st.rays_per_refill),
+        "queue_chunk": int(st.queue_chunk),
         "sequential_single_scene_ms": round(seq, 1), "batch_speedup_vs_sequential": round(seq / kern, 3),
     }
     log(f"c4: {res['mrays_s']} Mrays/s, batch {kern:.0f} ms vs sequential {seq:.0f} ms")

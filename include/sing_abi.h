@@ -343,15 +343,20 @@ typedef struct SingBatchOpts {
     uint32_t precision;     /* SING_PREC_* */
     uint32_t use_graph;     /* 1: record the batch's launches once as a hipGraph and replay it for batches
                                of the same layout (sizes, sample counts, metrics, options) */
-    uint32_t autotune;      /* 1: time launch-shape candidates on a 1/16 sample of this batch and keep the
-                               fastest for this (metric, precision, stepping) in the context */
+    uint32_t autotune;      /* 1: time candidates for waves per CU, step batch, rays per refill and queue
+                               chunk on a 1/16 sample of this batch and keep the fastest for this (metric,
+                               precision, stepping) in the context */
     uint32_t step_batch;    /* RK4 steps between lane refills (0 = tuned or 32) */
     uint32_t waves_per_cu;  /* persistent waves per CU (0 = tuned or 24) */
     float adapt_tol;        /* > 0: error-controlled stepping (applies to the Kerr scenes) */
     uint32_t turb_seed;     /* seeded disc-band phases, as SingRenderOpts */
     uint32_t record_budget_mib; /* device memory for one group's termination records (0 = 32 GiB); scenes are
                                    split into groups that fit, traced one group after another */
-    uint32_t reserved[4];
+    uint32_t rays_per_refill;   /* a wave claims new rays once at least this many of its lanes are idle (or
+                                   none is live); 0 = tuned or 1, at most 64 */
+    uint32_t queue_chunk;       /* work items a wave takes from its scene's queue per atomic and hands to its
+                                   lanes as they free up; 1 = one per idle lane; 0 = tuned or 1 */
+    uint32_t reserved[2];
 } SingBatchOpts;
 
 typedef struct SingBatchStats {
@@ -365,6 +370,8 @@ typedef struct SingBatchStats {
     uint32_t waves_per_cu;
     uint32_t graph_replayed;   /* 1 if an existing graph was replayed */
     uint32_t groups;           /* record-memory groups the batch was split into */
+    uint32_t rays_per_refill;  /* claim policy used */
+    uint32_t queue_chunk;
 } SingBatchStats;
 
 int sing_render_batch(SingContext* ctx, const SingBatchItem* items, uint32_t n, const SingBatchOpts* opts,

@@ -482,13 +482,15 @@ class BatchItem(C.Structure):
 class BatchOpts(C.Structure):
     _fields_ = [("precision", C.c_uint32), ("use_graph", C.c_uint32), ("autotune", C.c_uint32),
                 ("step_batch", C.c_uint32), ("waves_per_cu", C.c_uint32), ("adapt_tol", C.c_float),
-                ("turb_seed", C.c_uint32), ("record_budget_mib", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+                ("turb_seed", C.c_uint32), ("record_budget_mib", C.c_uint32), ("rays_per_refill", C.c_uint32),
+                ("queue_chunk", C.c_uint32), ("reserved", C.c_uint32 * 2)]
 
 
 class BatchStats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("copy_ms", C.c_double), ("rays", C.c_uint64), ("steps", C.c_uint64),
                 ("fp64_rays", C.c_uint64), ("rejected", C.c_uint64), ("step_batch", C.c_uint32),
-                ("waves_per_cu", C.c_uint32), ("graph_replayed", C.c_uint32), ("groups", C.c_uint32)]
+                ("waves_per_cu", C.c_uint32), ("graph_replayed", C.c_uint32), ("groups", C.c_uint32),
+                ("rays_per_refill", C.c_uint32), ("queue_chunk", C.c_uint32)]
 
 
 _SIGS["sing_render_batch"] = (C.c_int, [C.c_void_p, C.POINTER(BatchItem), C.c_uint32, C.POINTER(BatchOpts),

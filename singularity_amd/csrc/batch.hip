@@ -37,6 +37,11 @@ __device__ __forceinline__ std::uint64_t load_relaxed(const std::uint64_t* p) {
 __device__ __forceinline__ std::uint32_t load_relaxed(const std::uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// A wave-uniform 64-bit value in scalar registers.
+__device__ __forceinline__ std::uint64_t uniform_u64(std::uint64_t v) {
+    return (std::uint64_t(__builtin_amdgcn_readfirstlane(std::uint32_t(v >> 32))) << 32) |
+           std::uint64_t(__builtin_amdgcn_readfirstlane(std::uint32_t(v)));
+}
 __device__ __forceinline__ std::uint64_t item_limit(std::uint64_t n_items, std::uint32_t sample_div) {
     if (sample_div <= 1u) return n_items;
     const std::uint64_t part = ((n_items / sample_div + 63u) / 64u) * 64u;
@@ -90,7 +95,8 @@ template <int METRIC, int PASS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(METRIC != 1 ? 1 : PASS < 2 ? 6 : 3)))
 void trace_batch_kernel(const BatchScene* __restrict__ scenes, BatchLaunch L,
                                                           std::uint32_t first, std::uint32_t count,
-                                                          std::uint32_t step_batch, std::uint32_t sample_div) {
+                                                          std::uint32_t step_batch, std::uint32_t sample_div,
+                                                          BatchClaim claim) {
     constexpr bool KERR = (METRIC >= 1);
     constexpr bool ADAPT = (METRIC == 2);
     constexpr bool F64 = (PASS >= 2);
@@ -104,84 +110,110 @@ void trace_batch_kernel(const BatchScene* __restrict__ scenes, BatchLaunch L,
     std::uint64_t lane_steps = 0, lane_rays = 0;
     std::uint32_t lane_rej = 0;
     RayState<S> y{};
+    // the wave's private run of the current scene's work items (queue_chunk > 0)
+    std::uint64_t pnext = 0, pend = 0;
 
     for (;;) {
         const BatchScene& bs = scenes[sid];
         SceneQueue& q = L.queues[sid];
-        // (1) claim rays of the current scene for idle lanes
+        // (1) claim rays of the current scene for idle lanes: when at least
+        // rays_per_refill lanes are idle (or none is live), in up to two rounds
+        // (lanes whose item needed no trace, or that the private run could not
+        // cover, go round again)
         if (!dry) {
-            const std::uint64_t want = __ballot(need);
-            if (want) {
-                const std::uint64_t n_items = (PASS == 3) ? std::uint64_t(q.band_count) : item_limit(bs.n_items, sample_div);
-                const int leader = __ffsll((unsigned long long)want) - 1;
-                std::uint64_t base = 0;
-                if (lane == leader)
-                    base = atomicAdd((unsigned long long*)((PASS == 3) ? &q.next_band : &q.next_item),
-                                     (unsigned long long)__popcll(want));
-                base = shfl_u64(base, leader);
-                bool got_dry = false;
-                if (need) {
-                    const std::uint64_t item = base + __popcll(want & ((1ull << lane) - 1ull));
-                    if (item >= n_items) {
-                        got_dry = true;
+            std::uint64_t want = __ballot(need);
+            if (want && (std::uint32_t(__popcll(want)) >= claim.rays_per_refill || __ballot(live) == 0ull)) {
+                const std::uint64_t n_items =
+                    (PASS == 3) ? std::uint64_t(q.band_count) : item_limit(bs.n_items, sample_div);
+                std::uint64_t* counter = (PASS == 3) ? &q.next_band : &q.next_item;
+                for (int round = 0; round < 2 && want != 0ull && !dry; ++round) {
+                    const std::uint32_t k = std::uint32_t(__popcll(want));
+                    const int leader = __ffsll((unsigned long long)want) - 1;
+                    std::uint64_t base;
+                    std::uint32_t avail;
+                    if (claim.queue_chunk <= 1u) {
+                        base = 0;
+                        if (lane == leader) base = atomicAdd((unsigned long long*)counter, (unsigned long long)k);
+                        base = uniform_u64(shfl_u64(base, leader));
+                        avail = k;
                     } else {
-                        const std::uint32_t W = bs.sf.width, H = bs.sf.height;
-                        std::uint32_t gx, gy, s;
-                        bool ok;
-                        if (PASS == 3) {
-                            rec_idx = L.band[bs.rec_base + item];
-                            const std::uint32_t sl = rec_idx / bs.P, pix = rec_idx - sl * bs.P;
-                            gx = pix % W;
-                            gy = pix / W;
-                            s = sl;
-                            ok = true;
-                        } else {
-                            Chunk ch{};
-                            ch.s0 = 0;
-                            ch.S = bs.spp;
-                            ch.P = bs.P;
-                            ch.tiles_x = bs.tiles_x;
-                            ch.n_tiles = bs.n_tiles;
-                            ok = item_to_ray(item, ch, W, H, rec_idx, gx, gy, s);
+                        if (pnext >= pend) {
+                            std::uint64_t b = 0;
+                            if (lane == leader)
+                                b = atomicAdd((unsigned long long*)counter, (unsigned long long)claim.queue_chunk);
+                            b = uniform_u64(shfl_u64(b, leader));
+                            pnext = b;
+                            pend = b + claim.queue_chunk;
                         }
-                        if (ok) {
-                            float ox = 0.5f, oy = 0.5f;
-                            if (bs.spp > 1u) {
-                                const float2 hv = L.halton[s];
-                                ox = hv.x;
-                                oy = hv.y;
-                            }
-                            bool started, deferred = false;
-                            if (F64) {
-                                started = init_ray<double, S, KERR>(y, bs.sd, gx, gy, double(ox), double(oy),
-                                                                    (bool*)nullptr);
-                            } else if (PASS == 1) {
-                                bool hard = false;
-                                started = init_ray<float, S, KERR>(y, bs.sf, gx, gy, ox, oy, &hard);
-                                deferred = started && hard;
+                        const std::uint64_t left = pend - pnext;
+                        avail = left < k ? std::uint32_t(left) : k;
+                        base = pnext;
+                        pnext += avail;
+                    }
+                    bool got_dry = false;
+                    const std::uint32_t rank = std::uint32_t(__popcll(want & ((1ull << lane) - 1ull)));
+                    if (need && rank < avail) {
+                        const std::uint64_t item = base + rank;
+                        if (item >= n_items) {
+                            got_dry = true;
+                        } else {
+                            const std::uint32_t W = bs.sf.width, H = bs.sf.height;
+                            std::uint32_t gx, gy, s;
+                            bool ok;
+                            if (PASS == 3) {
+                                rec_idx = L.band[bs.rec_base + item];
+                                const std::uint32_t sl = rec_idx / bs.P, pix = rec_idx - sl * bs.P;
+                                gx = pix % W;
+                                gy = pix / W;
+                                s = sl;
+                                ok = true;
                             } else {
-                                started = init_ray<float, S, KERR>(y, bs.sf, gx, gy, ox, oy, (bool*)nullptr);
+                                Chunk ch{};
+                                ch.s0 = 0;
+                                ch.S = bs.spp;
+                                ch.P = bs.P;
+                                ch.tiles_x = bs.tiles_x;
+                                ch.n_tiles = bs.n_tiles;
+                                ok = item_to_ray(item, ch, W, H, rec_idx, gx, gy, s);
                             }
-                            need = false;
-                            if (deferred) {
-                                const std::uint32_t slot = atomicAdd(&q.band_count, 1u);
-                                L.band[bs.rec_base + slot] = rec_idx;
-                                L.rec[bs.rec_base + rec_idx] = deferred_marker();
-                                need = true;
-                            } else if (!started) {
-                                Rec rc;
-                                rc.a = rc.b = rc.c = 0.0f;
-                                rc.meta = kTermInitFail;
-                                L.rec[bs.rec_base + rec_idx] = rc;
-                                lane_rays += 1;
-                                need = true;
-                            } else {
-                                live = true;
+                            if (ok) {
+                                float ox = 0.5f, oy = 0.5f;
+                                if (bs.spp > 1u) {
+                                    const float2 hv = L.halton[s];
+                                    ox = hv.x;
+                                    oy = hv.y;
+                                }
+                                bool started, deferred = false;
+                                if (F64) {
+                                    started = init_ray<double, S, KERR>(y, bs.sd, gx, gy, double(ox), double(oy),
+                                                                        (bool*)nullptr);
+                                } else if (PASS == 1) {
+                                    bool hard = false;
+                                    started = init_ray<float, S, KERR>(y, bs.sf, gx, gy, ox, oy, &hard);
+                                    deferred = started && hard;
+                                } else {
+                                    started = init_ray<float, S, KERR>(y, bs.sf, gx, gy, ox, oy, (bool*)nullptr);
+                                }
+                                if (deferred) {
+                                    const std::uint32_t slot = atomicAdd(&q.band_count, 1u);
+                                    L.band[bs.rec_base + slot] = rec_idx;
+                                    L.rec[bs.rec_base + rec_idx] = deferred_marker();
+                                } else if (!started) {
+                                    Rec rc;
+                                    rc.a = rc.b = rc.c = 0.0f;
+                                    rc.meta = kTermInitFail;
+                                    L.rec[bs.rec_base + rec_idx] = rc;
+                                    lane_rays += 1;
+                                } else {
+                                    need = false;
+                                    live = true;
+                                }
                             }
                         }
                     }
+                    if (__ballot(got_dry)) dry = true;
+                    want = __ballot(need);
                 }
-                if (__ballot(got_dry)) dry = true;
             }
         }
         // (2) a dry scene with no live ray left: move to another scene with work
@@ -202,13 +234,21 @@ void trace_batch_kernel(const BatchScene* __restrict__ scenes, BatchLaunch L,
             if (!found) break;
             dry = false;
             need = true;
+            pnext = pend = 0;
             continue;
         }
 
-        // (3) integrate a batch of steps
+        // (3) integrate a batch of steps.  The step loop reads the scene through a
+        // local copy: with the table reference, the compiler (at its SGPR limit)
+        // re-loaded a constant from the table on every step and waited for it.
         std::uint32_t term;
-        if constexpr (F64) term = run_steps<double, S, KERR, false, ADAPT>(y, bs.sd, live, step_batch, &lane_rej);
-        else term = run_steps<float, S, KERR, PASS == 1, ADAPT>(y, bs.sf, live, step_batch, &lane_rej);
+        if constexpr (F64) {
+            const SceneConst<double> scl = bs.sd;
+            term = run_steps<double, S, KERR, false, ADAPT>(y, scl, live, step_batch, &lane_rej);
+        } else {
+            const SceneConst<float> scl = bs.sf;
+            term = run_steps<float, S, KERR, PASS == 1, ADAPT>(y, scl, live, step_batch, &lane_rej);
+        }
 
         // (4) emit finished rays; in the mixed pass, ambiguous outcomes go to FP64
         if (term != kTermLive) {
@@ -321,8 +361,8 @@ hipError_t launch_batch_reset(const BatchLaunch& L, const BatchGroup& g, bool re
 }
 
 hipError_t launch_batch_trace(const BatchLaunch& L, const BatchGroup& g, std::uint32_t mode, bool adaptive,
-                              const LaunchShape& shape, std::uint32_t sample_div, bool starfield_any,
-                              hipStream_t stream) {
+                              const LaunchShape& shape, const BatchClaim& claim, std::uint32_t sample_div,
+                              bool starfield_any, hipStream_t stream) {
     const dim3 block(256);
     const std::uint32_t sb = shape.step_batch ? shape.step_batch : 32u;
     const std::uint32_t n_kerr = g.count - g.n_schw, k0 = g.first + g.n_schw;
@@ -330,13 +370,13 @@ hipError_t launch_batch_trace(const BatchLaunch& L, const BatchGroup& g, std::ui
     do {                                                                                                        \
         if (g.n_schw)                                                                                           \
             hipLaunchKernelGGL((trace_batch_kernel<0, P>), dim3(G), block, 0, stream, L.scenes, L, g.first,       \
-                               g.n_schw, sb, sample_div);                                                       \
+                               g.n_schw, sb, sample_div, claim);                                                \
         if (n_kerr && adaptive)                                                                                 \
             hipLaunchKernelGGL((trace_batch_kernel<2, P>), dim3(G), block, 0, stream, L.scenes, L, k0, n_kerr, sb, \
-                               sample_div);                                                                     \
+                               sample_div, claim);                                                              \
         else if (n_kerr)                                                                                        \
             hipLaunchKernelGGL((trace_batch_kernel<1, P>), dim3(G), block, 0, stream, L.scenes, L, k0, n_kerr, sb, \
-                               sample_div);                                                                     \
+                               sample_div, claim);                                                              \
     } while (0)
     if (mode == SING_PREC_FP32) {
         SING_BTRACE(0, shape.blocks);
@@ -393,13 +433,14 @@ std::uint64_t tune_key(bool has_schw, bool has_kerr, std::uint32_t precision, bo
 }
 
 int run_groups(SingContext* c, const sing::BatchLaunch& L, const std::vector<sing::BatchGroup>& groups,
-               std::uint32_t precision, bool adaptive, const sing::LaunchShape& shape, bool starfield_any,
-               const sing::BatchOutputs& out) {
+               std::uint32_t precision, bool adaptive, const sing::LaunchShape& shape, const sing::BatchClaim& claim,
+               bool starfield_any, const sing::BatchOutputs& out) {
     hipError_t e = hipSuccess;
     for (std::size_t k = 0; k < groups.size() && e == hipSuccess; ++k) {
         const sing::BatchGroup& g = groups[k];
         e = sing::launch_batch_reset(L, g, k == 0, c->stream);
-        if (e == hipSuccess) e = sing::launch_batch_trace(L, g, precision, adaptive, shape, 1u, starfield_any, c->stream);
+        if (e == hipSuccess)
+            e = sing::launch_batch_trace(L, g, precision, adaptive, shape, claim, 1u, starfield_any, c->stream);
         if (e == hipSuccess) e = sing::launch_batch_shade(L, g, out, c->stream);
     }
     return e == hipSuccess ? SING_OK : c->hip_fail(e, "batch launch");
@@ -450,6 +491,9 @@ extern "C" int sing_render_batch(SingContext* c, const SingBatchItem* items, uin
     if (!(o.adapt_tol >= 0.0f && o.adapt_tol <= 0.1f))
         return c->fail(SING_ERR_ARG, "sing_render_batch: adapt_tol must be in [0, 0.1]");
     if (o.use_graph > 1u || o.autotune > 1u) return c->fail(SING_ERR_ARG, "sing_render_batch: flags must be 0 or 1");
+    if (o.rays_per_refill > 64u) return c->fail(SING_ERR_ARG, "sing_render_batch: rays_per_refill must be in [0, 64]");
+    if (o.queue_chunk > (1u << 20))
+        return c->fail(SING_ERR_ARG, "sing_render_batch: queue_chunk must be at most 2^20");
 
     // scene order: Schwarzschild first, then Kerr (stable), so each metric is one
     // contiguous run per group and one kernel instantiation traces it
@@ -564,54 +608,73 @@ extern "C" int sing_render_batch(SingContext* c, const SingBatchItem* items, uin
     sing::BatchOutputs out{want_rgba ? c->d_brgba : nullptr, want_hdr ? c->d_bhdr : nullptr,
                            want_cls ? c->d_bcls : nullptr};
 
-    // launch shape: explicit, else tuned for this kind of batch, else the default
+    // launch shape and claim policy: explicit, else tuned for this kind of batch, else the defaults
     const std::uint64_t tkey = tune_key(has_schw, has_kerr, o.precision, adaptive);
-    std::uint32_t sb = o.step_batch, wpc = o.waves_per_cu;
+    sing::BatchTune tu{o.step_batch, o.waves_per_cu, o.rays_per_refill, o.queue_chunk};
+    const bool explicit_rpr = o.rays_per_refill != 0u, explicit_qc = o.queue_chunk != 0u;
     for (const auto& kv : c->b_tuned)
         if (kv.first == tkey) {
-            if (!sb) sb = std::uint32_t(kv.second >> 32);
-            if (!wpc) wpc = std::uint32_t(kv.second & 0xffffffffu);
+            if (!tu.step_batch) tu.step_batch = kv.second.step_batch;
+            if (!tu.waves_per_cu) tu.waves_per_cu = kv.second.waves_per_cu;
+            if (!explicit_rpr) tu.rays_per_refill = kv.second.rays_per_refill;
+            if (!explicit_qc) tu.queue_chunk = kv.second.queue_chunk;
         }
-    if (o.autotune && (!o.step_batch || !o.waves_per_cu)) {
-        // coordinate search on the first 1/16 of the first group's work items
-        auto time_shape = [&](std::uint32_t s_b, std::uint32_t w_c, float* ms) -> int {
-            const sing::LaunchShape sh = shape_of(c, s_b, w_c);
+    if (o.autotune && (!o.step_batch || !o.waves_per_cu || !explicit_rpr || !explicit_qc)) {
+        // coordinate search on the first 1/16 of the first group's work items:
+        // waves per CU, step batch, rays per refill, queue chunk, each with the
+        // best of the knobs before it
+        auto time_shape = [&](const sing::BatchTune& t, float* ms) -> int {
+            const sing::LaunchShape sh = shape_of(c, t.step_batch, t.waves_per_cu);
+            const sing::BatchClaim cl{t.rays_per_refill, t.queue_chunk};
             hipError_t r = sing::launch_batch_reset(L, groups[0], true, c->stream);
             if (r == hipSuccess) r = hipEventRecord(c->events[0], c->stream);
             if (r == hipSuccess)
-                r = sing::launch_batch_trace(L, groups[0], o.precision, adaptive, sh, 16u, false, c->stream);
+                r = sing::launch_batch_trace(L, groups[0], o.precision, adaptive, sh, cl, 16u, false, c->stream);
             if (r == hipSuccess) r = hipEventRecord(c->events[1], c->stream);
             if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
             if (r == hipSuccess) r = hipEventElapsedTime(ms, c->events[0], c->events[1]);
             return r == hipSuccess ? SING_OK : c->hip_fail(r, "batch autotune");
         };
+        sing::BatchTune best_t{tu.step_batch ? tu.step_batch : 32u, tu.waves_per_cu ? tu.waves_per_cu : 24u,
+                         tu.rays_per_refill ? tu.rays_per_refill : 1u, tu.queue_chunk ? tu.queue_chunk : 1u};
         float best = 0.0f, ms = 0.0f;
-        std::uint32_t best_sb = sb ? sb : 32u, best_w = wpc ? wpc : 24u;
-        if ((rc = time_shape(best_sb, best_w, &best)) != SING_OK) return rc;  // also warms the code up
-        if ((rc = time_shape(best_sb, best_w, &best)) != SING_OK) return rc;
-        if (!o.waves_per_cu)
-            for (std::uint32_t w_c : {12u, 16u, 32u}) {
-                if ((rc = time_shape(best_sb, w_c, &ms)) != SING_OK) return rc;
-                if (ms < best) best = ms, best_w = w_c;
+        if ((rc = time_shape(best_t, &best)) != SING_OK) return rc;  // also warms the code up
+        if ((rc = time_shape(best_t, &best)) != SING_OK) return rc;
+        auto sweep = [&](std::uint32_t sing::BatchTune::*field, std::initializer_list<std::uint32_t> vals) -> int {
+            for (std::uint32_t v : vals) {
+                sing::BatchTune t = best_t;
+                t.*field = v;
+                if (t.*field == best_t.*field) continue;
+                const int r = time_shape(t, &ms);
+                if (r != SING_OK) return r;
+                if (ms < best) best = ms, best_t = t;
             }
-        if (!o.step_batch)
-            for (std::uint32_t s_b : {8u, 16u, 64u}) {
-                if ((rc = time_shape(s_b, best_w, &ms)) != SING_OK) return rc;
-                if (ms < best) best = ms, best_sb = s_b;
-            }
-        sb = best_sb;
-        wpc = best_w;
+            return SING_OK;
+        };
+        if (!o.waves_per_cu && (rc = sweep(&sing::BatchTune::waves_per_cu, {12u, 16u, 24u, 32u})) != SING_OK) return rc;
+        if (!o.step_batch && (rc = sweep(&sing::BatchTune::step_batch, {8u, 16u, 32u, 64u})) != SING_OK) return rc;
+        if (!explicit_rpr && (rc = sweep(&sing::BatchTune::rays_per_refill, {1u, 4u, 8u, 16u, 32u})) != SING_OK)
+            return rc;
+        if (!explicit_qc && (rc = sweep(&sing::BatchTune::queue_chunk, {1u, 64u, 256u})) != SING_OK) return rc;
+        // the step batch again: fewer refills shift its best value
+        if (!o.step_batch && (rc = sweep(&sing::BatchTune::step_batch, {8u, 16u, 32u, 64u})) != SING_OK) return rc;
+        tu = best_t;
         bool stored = false;
         for (auto& kv : c->b_tuned)
-            if (kv.first == tkey) kv.second = (std::uint64_t(sb) << 32) | wpc, stored = true;
-        if (!stored) c->b_tuned.push_back({tkey, (std::uint64_t(sb) << 32) | wpc});
+            if (kv.first == tkey) kv.second = tu, stored = true;
+        if (!stored) c->b_tuned.push_back({tkey, tu});
     }
-    if (!sb) sb = 32u;
-    if (!wpc) wpc = 24u;
+    if (!tu.step_batch) tu.step_batch = 32u;
+    if (!tu.waves_per_cu) tu.waves_per_cu = 24u;
+    if (!tu.rays_per_refill) tu.rays_per_refill = 1u;
+    if (!tu.queue_chunk) tu.queue_chunk = 1u;
+    const std::uint32_t sb = tu.step_batch, wpc = tu.waves_per_cu;
     const sing::LaunchShape shape = shape_of(c, sb, wpc);
+    const sing::BatchClaim claim{tu.rays_per_refill, tu.queue_chunk};
 
     // graph key: everything the recorded launches depend on
-    std::vector<std::uint64_t> key = {n, o.precision, adaptive ? 1u : 0u, sb, wpc, starfield_any ? 1u : 0u,
+    std::vector<std::uint64_t> key = {n, o.precision, adaptive ? 1u : 0u, sb, wpc, claim.rays_per_refill,
+                                      claim.queue_chunk, starfield_any ? 1u : 0u,
                                       reinterpret_cast<std::uint64_t>(c->d_bscenes),
                                       reinterpret_cast<std::uint64_t>(c->d_brec),
                                       reinterpret_cast<std::uint64_t>(c->d_bband),
@@ -633,7 +696,7 @@ extern "C" int sing_render_batch(SingContext* c, const SingBatchItem* items, uin
             c->drop_batch_graph();
             if ((e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
                 return c->hip_fail(e, "hipStreamBeginCapture");
-            const int r1 = run_groups(c, L, groups, o.precision, adaptive, shape, starfield_any, out);
+            const int r1 = run_groups(c, L, groups, o.precision, adaptive, shape, claim, starfield_any, out);
             hipGraph_t g = nullptr;
             e = hipStreamEndCapture(c->stream, &g);
             if (r1 != SING_OK) {
@@ -651,7 +714,8 @@ extern "C" int sing_render_batch(SingContext* c, const SingBatchItem* items, uin
         (void)hipEventRecord(c->events[0], c->stream);
         if ((e = hipGraphLaunch(c->b_graph_exec, c->stream)) != hipSuccess) return c->hip_fail(e, "hipGraphLaunch");
     } else {
-        if ((rc = run_groups(c, L, groups, o.precision, adaptive, shape, starfield_any, out)) != SING_OK) return rc;
+        if ((rc = run_groups(c, L, groups, o.precision, adaptive, shape, claim, starfield_any, out)) != SING_OK)
+            return rc;
     }
     (void)hipEventRecord(c->events[1], c->stream);
     sing::QueueCtl ctl{};
@@ -686,6 +750,8 @@ extern "C" int sing_render_batch(SingContext* c, const SingBatchItem* items, uin
         stats->rejected = ctl.rejected;
         stats->step_batch = sb;
         stats->waves_per_cu = wpc;
+        stats->rays_per_refill = claim.rays_per_refill;
+        stats->queue_chunk = claim.queue_chunk;
         stats->graph_replayed = replayed;
         stats->groups = std::uint32_t(groups.size());
     }

@@ -58,6 +58,20 @@ struct BatchLaunch {
     QueueCtl* ctl;
 };
 
+// How waves claim rays from a scene's queue: only when at least
+// rays_per_refill of their lanes are idle (or none is live), and, with
+// queue_chunk > 1, from a private run of queue_chunk work items per atomic
+// (<= 1: exactly one item per idle lane per atomic).
+struct BatchClaim {
+    std::uint32_t rays_per_refill;
+    std::uint32_t queue_chunk;
+};
+
+// A batch launch's tunables (0 = not set).
+struct BatchTune {
+    std::uint32_t step_batch, waves_per_cu, rays_per_refill, queue_chunk;
+};
+
 // Zero the queues of scenes [first, first + count) and the ray counters
 // (a kernel, so the sequence can be recorded in a graph).
 hipError_t launch_batch_reset(const BatchLaunch& L, const BatchGroup& g, bool reset_totals, hipStream_t stream);
@@ -65,8 +79,8 @@ hipError_t launch_batch_reset(const BatchLaunch& L, const BatchGroup& g, bool re
 // sample_div > 1 traces only the first 1/sample_div of each scene's work
 // items (autotuning).  adaptive: Kerr scenes use error-controlled steps.
 hipError_t launch_batch_trace(const BatchLaunch& L, const BatchGroup& g, std::uint32_t mode, bool adaptive,
-                              const LaunchShape& shape, std::uint32_t sample_div, bool starfield_any,
-                              hipStream_t stream);
+                              const LaunchShape& shape, const BatchClaim& claim, std::uint32_t sample_div,
+                              bool starfield_any, hipStream_t stream);
 hipError_t launch_batch_shade(const BatchLaunch& L, const BatchGroup& g, const BatchOutputs& out,
                               hipStream_t stream);
 

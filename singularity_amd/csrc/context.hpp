@@ -77,7 +77,7 @@ struct SingContext {
     hipGraph_t b_graph = nullptr;
     hipGraphExec_t b_graph_exec = nullptr;
     std::vector<std::uint64_t> b_graph_key;
-    std::vector<std::pair<std::uint64_t, std::uint64_t>> b_tuned;  // key -> step_batch << 32 | waves_per_cu
+    std::vector<std::pair<std::uint64_t, sing::BatchTune>> b_tuned;  // kind of batch -> tuned launch
 
     int fail(int code, const char* fmt, ...);
     int hip_fail(hipError_t e, const char* where);

@@ -104,12 +104,16 @@ def test_batch_adaptive_and_seeded_equals_single(be):
 def test_launch_shapes_and_groups_do_not_change_outputs(be):
     scenes = scenes_mixed()
     ref, _ = sa.render_batch(be, scenes, sa.BatchOpts(precision=sa.PREC_MIXED), hdr=True, cls=True)
-    for sb, wpc, budget in ((1, 4, 0), (32, 24, 0), (8, 16, 2)):
-        print(f"shape step_batch={sb} waves_per_cu={wpc} budget_mib={budget}", flush=True)
-        o = sa.BatchOpts(precision=sa.PREC_MIXED, step_batch=sb, waves_per_cu=wpc, record_budget_mib=budget)
+    for sb, wpc, budget, rpr, qc in ((1, 4, 0, 1, 1), (32, 24, 0, 16, 1), (8, 16, 2, 1, 64), (32, 24, 0, 4, 256),
+                                     (16, 12, 0, 64, 7)):
+        print(f"shape step_batch={sb} waves_per_cu={wpc} budget_mib={budget} rays_per_refill={rpr} "
+              f"queue_chunk={qc}", flush=True)
+        o = sa.BatchOpts(precision=sa.PREC_MIXED, step_batch=sb, waves_per_cu=wpc, record_budget_mib=budget,
+                         rays_per_refill=rpr, queue_chunk=qc)
         frames, st = sa.render_batch(be, scenes, o, hdr=True, cls=True)
         check_equal(frames, ref)
-        assert (st.step_batch, st.waves_per_cu) == (sb, wpc)
+        assert (st.step_batch, st.waves_per_cu, st.rays_per_refill, st.queue_chunk) == (sb, wpc, rpr, qc)
+        assert st.rays == sum(u.width * u.height * max(1, u.supersample) for u in scenes)
         if budget:
             assert st.groups > 1
 
@@ -146,12 +150,14 @@ def test_autotune_picks_a_candidate_and_keeps_outputs(be):
     ref, _ = sa.render_batch(be, scenes, sa.BatchOpts(precision=sa.PREC_FP32), hdr=True)
     frames, st = sa.render_batch(be, scenes, sa.BatchOpts(precision=sa.PREC_FP32, autotune=1), hdr=True)
     assert st.step_batch in (8, 16, 32, 64) and st.waves_per_cu in (12, 16, 24, 32)
+    assert st.rays_per_refill in (1, 4, 8, 16, 32) and st.queue_chunk in (1, 64, 256)
     for f, r in zip(frames, ref):
         np.testing.assert_array_equal(f.rgba, r.rgba)
         np.testing.assert_array_equal(f.hdr, r.hdr)
     # the tuned shape is kept for this kind of batch
     _, st2 = sa.render_batch(be, scenes, sa.BatchOpts(precision=sa.PREC_FP32), hdr=True)
-    assert (st2.step_batch, st2.waves_per_cu) == (st.step_batch, st.waves_per_cu)
+    assert (st2.step_batch, st2.waves_per_cu, st2.rays_per_refill, st2.queue_chunk) == (
+        st.step_batch, st.waves_per_cu, st.rays_per_refill, st.queue_chunk)
 
 
 def test_batch_argument_errors(be):
@@ -163,6 +169,8 @@ def test_batch_argument_errors(be):
         sa.render_batch(be, [bad], sa.BatchOpts())
     with pytest.raises(sa.SingError):
         sa.render_batch(be, [scene_uniforms(width=4, height=4)], sa.BatchOpts(precision=9))
+    with pytest.raises(sa.SingError):
+        sa.render_batch(be, [scene_uniforms(width=4, height=4)], sa.BatchOpts(rays_per_refill=65))
 
 
 def test_c4_workload_against_oracle_totals_and_graph(be, orc):

@@ -63,7 +63,7 @@ def test_struct_sizes_match_header_layout():
     assert C.sizeof(sa.RenderStats) == 104
     assert C.sizeof(sa.DiffOpts) == 40 and C.sizeof(sa.DiffStats) == 64
     assert C.sizeof(sa.DiffParams) == 40 and C.sizeof(sa.DiffCamera) == 24
-    assert C.sizeof(sa.BatchItem) == 32 and C.sizeof(sa.BatchOpts) == 48 and C.sizeof(sa.BatchStats) == 64
+    assert C.sizeof(sa.BatchItem) == 32 and C.sizeof(sa.BatchOpts) == 48 and C.sizeof(sa.BatchStats) == 72
 
 
 def test_scene_defaults_match_reference():

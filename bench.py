@@ -187,6 +187,30 @@ def post_leg(reps: int = 5) -> dict:
     return res
 
 
+def refine_leg(be: sa.HipBackend, u: sa.Uniforms, base_mixed_ms: float) -> dict:
+    """The headline frame with the precision refinements of the fixed-step march
+    (SING_REFINE_CROSS64: disc crossing at the FP64 cubic dense-output root;
+    SING_REFINE_REPROJECT: null-constraint re-projection after every step): the
+    mixed frame's time against the same frame without them, and its error
+    against the all-FP64 GPU frame under the same rule."""
+    outs = sa.OUT_HDR | sa.OUT_CLASS
+    res = {}
+    for name, rf in (("cross64", sa.REFINE_CROSS64), ("reproject", sa.REFINE_REPROJECT),
+                     ("cross64+reproject", sa.REFINE_CROSS64 | sa.REFINE_REPROJECT)):
+        stm = be.render(u, sa.RenderOpts(precision=sa.PREC_MIXED, outputs=outs, refine=rf))
+        fm = be.capture(outs)
+        st64 = be.render(u, sa.RenderOpts(precision=sa.PREC_FP64, outputs=outs, refine=rf))
+        f64 = be.capture(outs)
+        res[name] = {"mixed_kernel_ms": round(stm.kernel_ms, 2),
+                     "mixed_trace32_ms": round(stm.trace32_ms, 2), "mixed_trace64_ms": round(stm.trace64_ms, 2),
+                     "mixed_over_unrefined_mixed_time": round(stm.kernel_ms / base_mixed_ms, 3),
+                     "fp64_kernel_ms": round(st64.kernel_ms, 2),
+                     "fp64_over_mixed_time": round(st64.kernel_ms / stm.kernel_ms, 3),
+                     "mixed_error_vs_gpu_fp64_same_rule": err_stats(fm.hdr, f64.hdr, fm.cls, f64.cls)}
+        log(f"refine {name}: {res[name]}")
+    return res
+
+
 def c3_leg() -> dict:
     """Kerr a=0.998 3840x2160 256 spp, incl 60 deg, disc [ISCO, 30 M], cuda_cli camera
     defaults (30 M, FOV 60), fixed h = 0.4: plain FP32, mixed and all-FP64 frames."""
@@ -347,7 +371,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the grads/s leg (512x512 inverse render)")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the 1080p adaptive+bloom and the a=0.998 4K three-precision legs")
+                    help="skip the refinement, 1080p adaptive+bloom, a=0.998 4K three-precision, post, "
+                         "batch and integrator-fan legs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; the renderer has no multi-GPU path "
@@ -439,6 +464,8 @@ def main() -> int:
         out["mixed_error_vs_gpu_fp64"] = err_stats(f_mixed.hdr, f64.hdr, f_mixed.cls, f64.cls)
         out["mixed_over_plain_time"] = round(kern_ms / st32.kernel_ms, 3)
         out["fp64_over_mixed_time"] = round(st64.kernel_ms / kern_ms, 3)
+    if not args.no_extra:
+        out["refine"] = refine_leg(be, u, kern_ms)
     be.shutdown()
     if not args.no_extra:
         out["adaptive_bloom_1080p"] = c2_leg()

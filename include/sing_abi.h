@@ -111,6 +111,16 @@ typedef struct SingUniforms {
 #define SING_SAMPLER_GRID 1u   /* ss x ss grid, ss = clamp(supersample, 1, 4), ACES of the mean radiance
                                   (render/metal/shaders/geodesic_kernel.metal:416-433, blit.metal:54-68) */
 
+/* --- precision refinements of the fixed-step Kerr march (bit mask) ------------
+ * Neither has a counterpart in the reference; both change its discrete
+ * algorithm, and the oracle implements the same rules. */
+#define SING_REFINE_CROSS64 (1u << 0)   /* disc crossing at the root of the FP64 cubic Hermite dense output of
+                                           the step (three Newton iterations), not linear in theta
+                                           (shared_shader/disc_intersection.h:30-44); phi from the same output */
+#define SING_REFINE_REPROJECT (1u << 1) /* after every RK4 step, one Newton step of the null constraint
+                                           2H = 0 (shared_shader/kerr_hamilton.h:67-94) along its gradient
+                                           in (p_r, p_theta) */
+
 /* Options beyond the 160-byte block.  Zero-initialised = reference semantics,
  * FP32, RGBA8 only. */
 typedef struct SingRenderOpts {
@@ -124,7 +134,8 @@ typedef struct SingRenderOpts {
     uint32_t step_batch;  /* RK4 steps between lane-refill checks (0 = tuned default) */
     uint32_t waves_per_cu;/* persistent grid size in waves per CU (0 = tuned default) */
     uint32_t sampler;     /* SING_SAMPLER_* */
-    uint32_t reserved[6];
+    uint32_t refine;      /* SING_REFINE_* mask; Kerr with the fixed step (adapt_tol = 0) only */
+    uint32_t reserved[5];
 } SingRenderOpts;
 
 typedef struct SingRenderStats {

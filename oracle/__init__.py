@@ -27,7 +27,8 @@ _LIBS: dict[str, C.CDLL] = {}
 class Extras(C.Structure):
     _fields_ = [("soft_edge", C.c_double), ("band_phase1", C.c_double), ("band_phase2", C.c_double),
                 ("adapt_tol", C.c_double), ("peak_T_scale", C.c_double), ("jitter_x", C.c_double),
-                ("jitter_y", C.c_double), ("grid_sampler", C.c_double)]
+                ("jitter_y", C.c_double), ("grid_sampler", C.c_double), ("reproject", C.c_double),
+                ("cross_refine", C.c_double)]
 
 
 def build() -> None:
@@ -105,6 +106,10 @@ def _type_oracle(L: C.CDLL) -> None:
     L.orc_kerr_two_H_f32.argtypes = [P, P]
     L.orc_kerr_two_H_f64.restype = C.c_double
     L.orc_kerr_two_H_f64.argtypes = [P, P]
+    L.orc_kerr_reproject_f32.argtypes = [P, P, P]
+    L.orc_kerr_reproject_f64.argtypes = [P, P, P]
+    L.orc_kerr_refined_crossing_f32.argtypes = [P, P, C.c_float, P, P]
+    L.orc_kerr_refined_crossing_f64.argtypes = [P, P, C.c_double, P, P]
     L.orc_kerr_rk4_f32.argtypes = [P, C.c_float, P, C.c_int, P]
     L.orc_kerr_rk4_f64.argtypes = [P, C.c_double, P, C.c_int, P]
     L.orc_schw_rhs_f32.argtypes = [P, C.c_float, P]

@@ -247,6 +247,8 @@ template <class T> struct Extras {
     T jitter_x = T(0.0);    // added to every sample offset (boundary-band probes)
     T jitter_y = T(0.0);
     bool grid = false;      // realtime sampler: ss x ss grid, ss = clamp(spp, 1, 4), tone map of the mean
+    bool reproject = false;     // Kerr: null-constraint re-projection after every fixed RK4 step
+    bool cross_refine = false;  // Kerr fixed step: crossing root from the FP64 cubic dense output
 };
 
 template <class T> struct Uni {
@@ -817,6 +819,47 @@ inline DenseCross<T> kerr_dense_crossing(const KState<T>& y0, const KState<T>& y
     const T s = dense_root(y0.theta, h * f0.theta, y1.theta, h * f1.theta, mid, frac);
     return {hermite(s, y0.r, h * f0.r, y1.r, h * f1.r), hermite(s, y0.phi, h * f0.phi, y1.phi, h * f1.phi)};
 }
+// Null-constraint re-projection (an option with no counterpart in the reference,
+// whose constraint kerr_ham_two_H, kerr_hamilton.h:67-94, only its tests
+// evaluate): one Newton step on 2H(r, theta, p_r, p_theta) = 0 along the gradient
+// of 2H in the momenta, g = (2 Delta p_r, 2 p_theta) / Sigma, i.e.
+// p -= (2H / |g|^2) g.  2H is quadratic in (p_r, p_theta), so what remains is
+// O((2H)^2); r, theta, phi, E and L_z are untouched.
+template <class T> inline KState<T> kerr_reproject(KState<T> s, const KConst<T>& c) {
+    const T ct = xcos(s.theta);
+    const T r2 = s.r * s.r, a2 = c.a * c.a;
+    const T Sigma = r2 + a2 * ct * ct;
+    const T Delta = r2 - T(2.0) * c.M * s.r + a2;
+    const T h2 = kerr_two_H(s.r, s.theta, s.p_r, s.p_theta, c);
+    const T gr = T(2.0) * Delta * s.p_r / Sigma, gt = T(2.0) * s.p_theta / Sigma;
+    const T n = gr * gr + gt * gt;
+    if (val(n) > 0.0) {
+        const T lam = h2 / n;
+        s.p_r = s.p_r - lam * gr;
+        s.p_theta = s.p_theta - lam * gt;
+    }
+    return s;
+}
+// The precision a crossing refinement runs in: double for a float march.
+template <class T> struct Wide { using type = T; };
+template <> struct Wide<float> { using type = double; };
+// Equator crossing of a fixed step with the root refined in FP64 (an option; the
+// reference interpolates linearly in theta, disc_intersection.h:30-44): the
+// cubic Hermite dense output of the step the march took (h = h_step *
+// pole_scale(theta_prev)) from its end states and their RHS derivatives in the
+// march's precision T, then, widened to FP64, three Newton iterations on
+// theta(s) = pi/2 from the linear fraction and r, phi at the root.
+template <class T>
+inline DenseCross<T> kerr_refined_crossing(const KState<T>& y0, const KState<T>& y1, T h, const KConst<T>& c) {
+    using W = typename Wide<T>::type;
+    const KState<T> f0 = kerr_rhs(y0, c), f1 = kerr_rhs(y1, c);
+    const W mid = W(1.5707963267948966192313216916397514);
+    const W th0 = W(y0.theta), th1 = W(y1.theta);
+    const W s = dense_root(th0, W(h * f0.theta), th1, W(h * f1.theta), mid, (mid - th0) / (th1 - th0));
+    return {T(hermite(s, W(y0.r), W(h * f0.r), W(y1.r), W(h * f1.r))),
+            T(hermite(s, W(y0.phi), W(h * f0.phi), W(y1.phi), W(h * f1.phi)))};
+}
+
 // Escape of an error-controlled step: the direction (theta, phi) where the dense
 // output reaches r = escape_r (the fixed-step algorithm takes the first state
 // beyond it, which moves by a whole step when the step sequence changes).
@@ -914,6 +957,10 @@ inline RayOut<T> trace_kerr(const Uni<T>& u, const CamFrame<T>& f, const CameraR
                 const DenseCross<T> dc = kerr_dense_crossing(prev, s, h_used, x.frac, c);
                 r_cross = dc.r;
                 phi_cross = dc.phi;
+            } else if (u.ext.cross_refine && o.steps > 0) {
+                const DenseCross<T> dc = kerr_refined_crossing(prev, s, u.h_step * kerr_pole_scale(prev.theta), c);
+                r_cross = dc.r;
+                phi_cross = dc.phi;
             }
             const bool stop = disc_hit(u, r_cross, acc, trans, o, [&](T Tk) {
                 return kerr_disc_emit(u, r_cross, phi_cross, cam_r, c.L_z, Tk);
@@ -925,6 +972,7 @@ inline RayOut<T> trace_kerr(const Uni<T>& u, const CamFrame<T>& f, const CameraR
         prev_phi = s.phi;
         prev = s;
         s = kerr_step(s, c, u, h_cur, &h_used);
+        if (u.ext.reproject && !adaptive) s = kerr_reproject(s, c);
         o.steps = step + 1;
         o.r_min = std::min(o.r_min, val(s.r));
         o.th_min = std::min(o.th_min, std::fabs(val(s.theta) - 1.5707963267948966));

@@ -19,6 +19,8 @@ struct OrcExtras {  // mirrors orc::Extras in double; all zero = reference seman
     double jitter_x;      // added to every sample offset
     double jitter_y;
     double grid_sampler;  // non-zero: the realtime backends' grid sampler
+    double reproject;     // non-zero: Kerr null-constraint re-projection after every fixed step
+    double cross_refine;  // non-zero: Kerr fixed-step crossing root from the FP64 dense output
 };
 
 template <class T> Uni<T> load_uni(const void* raw, const OrcExtras* ex) {
@@ -34,6 +36,8 @@ template <class T> Uni<T> load_uni(const void* raw, const OrcExtras* ex) {
         o.ext.jitter_x = T(ex->jitter_x);
         o.ext.jitter_y = T(ex->jitter_y);
         o.ext.grid = ex->grid_sampler != 0.0;
+        o.ext.reproject = ex->reproject != 0.0;
+        o.ext.cross_refine = ex->cross_refine != 0.0;
     }
     return o;
 }
@@ -198,6 +202,34 @@ float orc_kerr_two_H_f32(const float* s, const float* c) {
 }
 double orc_kerr_two_H_f64(const double* s, const double* c) {
     return kerr_two_H(s[1], s[2], s[4], s[5], KConst<double>{c[0], c[1], c[2], c[3], c[4]});
+}
+// Null-constraint re-projection (kerr_reproject) of s[6] = (t, r, theta, phi,
+// p_r, p_theta) with c[5] = (E, L_z, Q, a, M); out[6] = the projected state.
+void orc_kerr_reproject_f32(const float* s, const float* c, float* out) {
+    const KState<float> o = kerr_reproject(KState<float>{s[0], s[1], s[2], s[3], s[4], s[5]},
+                                           KConst<float>{c[0], c[1], c[2], c[3], c[4]});
+    out[0] = o.t; out[1] = o.r; out[2] = o.theta; out[3] = o.phi; out[4] = o.p_r; out[5] = o.p_theta;
+}
+void orc_kerr_reproject_f64(const double* s, const double* c, double* out) {
+    const KState<double> o = kerr_reproject(KState<double>{s[0], s[1], s[2], s[3], s[4], s[5]},
+                                            KConst<double>{c[0], c[1], c[2], c[3], c[4]});
+    out[0] = o.t; out[1] = o.r; out[2] = o.theta; out[3] = o.phi; out[4] = o.p_r; out[5] = o.p_theta;
+}
+// FP64-refined equator crossing (kerr_refined_crossing) of the step y0 -> y1 of
+// size h; out[2] = (r, phi) at the root.
+void orc_kerr_refined_crossing_f32(const float* y0, const float* y1, float h, const float* c, double* out) {
+    const DenseCross<float> d = kerr_refined_crossing(KState<float>{y0[0], y0[1], y0[2], y0[3], y0[4], y0[5]},
+                                                      KState<float>{y1[0], y1[1], y1[2], y1[3], y1[4], y1[5]}, h,
+                                                      KConst<float>{c[0], c[1], c[2], c[3], c[4]});
+    out[0] = d.r;
+    out[1] = d.phi;
+}
+void orc_kerr_refined_crossing_f64(const double* y0, const double* y1, double h, const double* c, double* out) {
+    const DenseCross<double> d = kerr_refined_crossing(KState<double>{y0[0], y0[1], y0[2], y0[3], y0[4], y0[5]},
+                                                       KState<double>{y1[0], y1[1], y1[2], y1[3], y1[4], y1[5]}, h,
+                                                       KConst<double>{c[0], c[1], c[2], c[3], c[4]});
+    out[0] = d.r;
+    out[1] = d.phi;
 }
 // n steps of the damped RK4; traj receives 6*(n+1) values (state after each step, incl. start).
 void orc_kerr_rk4_f32(const float* s, float h, const float* c, int n, float* traj) {

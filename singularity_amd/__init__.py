@@ -35,6 +35,7 @@ PREC_FP32, PREC_FP64, PREC_MIXED = 0, 1, 2
 OUT_RGBA8, OUT_HDR, OUT_CLASS, OUT_STEPS, OUT_HDR16 = 1, 2, 4, 8, 16
 POST_NONE, POST_BLOOM = 0, 1
 SAMPLER_HALTON, SAMPLER_GRID = 0, 1
+REFINE_CROSS64, REFINE_REPROJECT = 1, 2
 SING_OK = 0
 ERR_NAMES = {1: "ARG", 2: "HIP", 3: "STATE", 4: "NOMEM", 5: "IO", 6: "PARSE", 7: "KEY"}
 
@@ -69,7 +70,7 @@ class RenderOpts(C.Structure):
         ("precision", C.c_uint32), ("outputs", C.c_uint32), ("post", C.c_uint32), ("turb_seed", C.c_uint32),
         ("adapt_tol", C.c_float), ("band_width", C.c_float), ("peak_T_scale", C.c_float),
         ("step_batch", C.c_uint32), ("waves_per_cu", C.c_uint32), ("sampler", C.c_uint32),
-        ("reserved", C.c_uint32 * 6),
+        ("refine", C.c_uint32), ("reserved", C.c_uint32 * 5),
     ]
 
 

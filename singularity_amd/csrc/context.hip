@@ -173,6 +173,10 @@ bool render_args_ok(const SingUniforms& u, const SingRenderOpts& o, const char* 
     else if (!(o.adapt_tol >= 0.0f && o.adapt_tol <= 0.1f)) std::snprintf(why, sizeof(why), "adapt_tol must be in [0, 0.1]");
     else if (o.adapt_tol > 0.0f && u.metric_type != SING_METRIC_KERR)
         std::snprintf(why, sizeof(why), "error-controlled stepping (adapt_tol) is Kerr-only");
+    else if (o.refine & ~(SING_REFINE_CROSS64 | SING_REFINE_REPROJECT))
+        std::snprintf(why, sizeof(why), "unknown refine bits 0x%x", o.refine);
+    else if (o.refine && (u.metric_type != SING_METRIC_KERR || o.adapt_tol > 0.0f))
+        std::snprintf(why, sizeof(why), "refine options apply to the fixed-step Kerr march only");
     else if (o.post == SING_POST_BLOOM && (u.width < 2 || u.height < 2))
         std::snprintf(why, sizeof(why), "bloom needs at least 2x2 pixels");
     if (!why[0]) return true;

@@ -345,6 +345,32 @@ __device__ __forceinline__ void kerr_rk4(R& r, R& th, R& ph, R& pr, R& pth, R hs
     kerr_rk4_raw<R, T_OUT, true, K>(r, th, ph, pr, pth, hs * kerr_pole_scale(st), st, ct, sc, k, t);
 }
 
+// Null-constraint re-projection (SING_REFINE_REPROJECT; the reference defines
+// the constraint, kerr_hamilton.h:67-94, but never applies it): one Newton
+// step on 2H(p_r, p_theta) = 0 along the gradient of 2H in the momenta,
+// g = (2 Delta p_r, 2 p_theta) / Sigma.  With 2H = F / Sigma (F the separated
+// numerator of kerr_rhs_sc) the step p -= (2H / |g|^2) g reads
+//   p_r *= 1 - c Delta,  p_theta *= 1 - c,  c = F / (2 (Delta^2 p_r^2 + p_theta^2)),
+// Sigma cancelling; F Delta s2 and the denominator share one reciprocal.
+template <class R>
+__device__ __forceinline__ void kerr_reproject(R r, R& pr, R& pth, R st, const SceneConst<R>& sc, const KerrK<R>& k) {
+    const R a2 = sc.a2;
+    const R s2 = st * st;
+    const R safe = pos_max(s2, R(1e-10));
+    const R r2a2 = r * r + a2;
+    const R Del = r * (r - sc.twoM) + a2;
+    const R P = r2a2 * k.E - k.aL;
+    const R pr2 = pr * pr, pt2 = pth * pth;
+    const R G = Del * Del * pr2 + pt2;
+    const R Ds = Del * safe;
+    const R den = R(2) * G * Ds;
+    if (!(den != R(0))) return;
+    const R FDs = (((Del * pr2 + pt2) + k.c2aEL) + k.a2E2 * s2) * Ds + (k.L2 * Del - P * P * safe);
+    const R c = FDs * drcp(den);
+    pr = pr * (R(1) - c * Del);
+    pth = pth * (R(1) - c);
+}
+
 // Error-controlled step by step doubling: one step of h against two of h/2
 // (h = h_cur * pole scale).  The error norm is max(|dr|/(1+|r|), |dtheta|,
 // |dphi|)/15.  Rejected (err > tol, fewer than 11 retries, h_cur above

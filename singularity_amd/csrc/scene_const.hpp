@@ -42,6 +42,8 @@ template <class R> struct SceneConst {
     // error-controlled stepping (Kerr): per-step tolerance, 0 = the reference's fixed step
     R tol;
     std::uint32_t width, height, metric, flags, max_steps, spp;
+    // SING_REFINE_* (fixed-step Kerr): FP64 crossing root, null-constraint re-projection
+    std::uint32_t refine;
     // grid sampler: spp = ss * ss offsets from the grid table, ACES of the mean radiance
     std::uint32_t grid;
 };
@@ -135,6 +137,7 @@ template <class R> inline SceneConst<R> make_scene_const(const SingUniforms& u, 
     s.max_steps = u.max_steps;
     std::uint32_t spp = u.supersample;
     s.spp = spp < 1u ? 1u : (spp > 1024u ? 1024u : spp);
+    s.refine = o.refine;
     s.grid = o.sampler == SING_SAMPLER_GRID ? 1u : 0u;
     if (s.grid) {
         const std::uint32_t ss = s.spp < 4u ? s.spp : 4u;

@@ -29,11 +29,14 @@ namespace {
 
 using namespace core;
 
-// METRIC 0: Schwarzschild; 1: Kerr; 2: Kerr with error-controlled steps.
+// METRIC 0: Schwarzschild; 1: Kerr; 2: Kerr with error-controlled steps; 3: Kerr
+// with the fixed step and SING_REFINE_* options (FP64 crossing root, null-
+// constraint re-projection).
 // PASS 0: FP32 trace of every ray; 1: FP32 trace deferring photon-band rays;
 // 2: FP64 trace of every ray; 3: FP64 trace of the deferred list.
 // The fixed-step Kerr FP64 passes are held to 168 VGPRs, 3 waves per SIMD (the
-// compiler's own choice, 170, leaves room for 2), without spills.
+// compiler's own choice, 170, leaves room for 2), without spills; METRIC 3
+// keeps the compiler's choice (held to 168 it spills).
 template <int METRIC, int PASS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((METRIC == 1 && PASS >= 2) ? 3 : 1)))
 void trace_kernel(SceneConst<float> sf, SceneConst<double> sd,
@@ -41,6 +44,7 @@ void trace_kernel(SceneConst<float> sf, SceneConst<double> sd,
                                                     QueueCtl* __restrict__ ctl, std::uint32_t step_batch) {
     constexpr bool KERR = (METRIC >= 1);
     constexpr bool ADAPT = (METRIC == 2);
+    constexpr bool XTRA = (METRIC == 3);
     constexpr bool F64 = (PASS >= 2);
     using S = typename std::conditional<F64, double, float>::type;
     const int lane = __lane_id();
@@ -137,8 +141,8 @@ void trace_kernel(SceneConst<float> sf, SceneConst<double> sd,
 
         // (2) integrate a batch of steps
         std::uint32_t term;
-        if constexpr (F64) term = run_steps<double, S, KERR, false, ADAPT>(y, sd, live, step_batch, &lane_rej);
-        else term = run_steps<float, S, KERR, PASS == 1, ADAPT>(y, sf, live, step_batch, &lane_rej);
+        if constexpr (F64) term = run_steps<double, S, KERR, false, ADAPT, XTRA>(y, sd, live, step_batch, &lane_rej);
+        else term = run_steps<float, S, KERR, PASS == 1, ADAPT, XTRA>(y, sf, live, step_batch, &lane_rej);
 
         // (3) emit finished rays; in the mixed pass, hand ambiguous outcomes to FP64
         if (term != kTermLive) {
@@ -150,7 +154,7 @@ void trace_kernel(SceneConst<float> sf, SceneConst<double> sd,
                 ch.band[slot] = rec_idx;
                 ch.rec[rec_idx] = deferred_marker();
             } else {
-                ch.rec[rec_idx] = make_rec<S, KERR, ADAPT>(y, term);
+                ch.rec[rec_idx] = make_rec<S, KERR, ADAPT, ADAPT || XTRA>(y, term);
                 lane_steps += y.steps;
                 lane_rays += 1;
             }
@@ -342,6 +346,8 @@ hipError_t launch_trace_chunk(const SceneConst<float>& sf, const SceneConst<doub
     do {                                                                                                  \
         if (adapt)                                                                                        \
             hipLaunchKernelGGL((trace_kernel<2, P>), dim3(G), block, 0, stream, sf, sd, halton, ch, ctl, sb); \
+        else if (kerr && sf.refine)                                                                       \
+            hipLaunchKernelGGL((trace_kernel<3, P>), dim3(G), block, 0, stream, sf, sd, halton, ch, ctl, sb); \
         else if (kerr)                                                                                    \
             hipLaunchKernelGGL((trace_kernel<1, P>), dim3(G), block, 0, stream, sf, sd, halton, ch, ctl, sb); \
         else                                                                                              \

@@ -61,6 +61,26 @@ __device__ __forceinline__ KerrK<R> ray_k(const RayState<S>& y, const SceneConst
     else return KerrK<R>{};
 }
 
+// Equator crossing of a fixed step with the root refined in FP64
+// (SING_REFINE_CROSS64): the cubic Hermite dense output of the step from its
+// end states and their RHS derivatives (in the march's precision R; the step
+// is the damped one the march took, kept in hprev), then, widened to double,
+// three Newton iterations on theta(s) = pi/2 from the linear fraction and r,
+// phi at the root.  The derivatives' rounding moves the root by ~1 ulp of s;
+// the FP32 subtraction theta(s) - pi/2 would move it by ~1e-7 / |dtheta|.
+template <class R, class S>
+__device__ __forceinline__ void refined_crossing(const RayState<S>& y, const SceneConst<R>& sc, const KerrK<R>& k,
+                                                 R& rc, R& phc) {
+    using W = double;
+    const KerrDense<R> d = kerr_dense<R>(R(y.prev_r), R(y.prev_th), R(y.prev_uph), R(y.prev_v0), R(y.prev_v1),
+                                         R(y.r), R(y.th), R(y.ph), R(y.v0), R(y.v1), R(y.hprev), sc, k);
+    const W mid = W(1.5707963267948966192313216916397514);
+    const W th0 = W(d.th0), th1 = W(d.th1);
+    const W s = dense_root<W>(th0, W(d.dth0), th1, W(d.dth1), mid, (mid - th0) / (th1 - th0));
+    rc = R(hermite<W>(s, W(d.r0), W(d.dr0), W(d.r1), W(d.dr1)));
+    phc = R(hermite<W>(s, W(d.ph0), W(d.dph0), W(d.ph1), W(d.dph1)));
+}
+
 // Termination checks in the reference's order (geodesic_kernel.cu:351-401)
 // followed by one RK4 step.  Arithmetic in R, storage in S; k = ray_k(y, sc)
 // (Kerr).  With ADAPT (Kerr) the step is one attempt of the error-controlled
@@ -68,8 +88,10 @@ __device__ __forceinline__ KerrK<R> ray_k(const RayState<S>& y, const SceneConst
 // are idempotent) and counts in *rejected.  The four exit conditions are
 // tested together so that a marching ray takes one branch; which one fired
 // (budget, horizon, escape, then a crossing inside the annulus) is sorted out
-// only on the rare exit path.
-template <class R, class S, bool KERR, bool TRACK = false, bool ADAPT = false>
+// only on the rare exit path.  XTRA (fixed-step Kerr): the step keeps the
+// state before it for the crossing refinement, and sc.refine selects the FP64
+// crossing root and the null-constraint re-projection.
+template <class R, class S, bool KERR, bool TRACK = false, bool ADAPT = false, bool XTRA = false>
 __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneConst<R>& sc, const KerrK<R>& k,
                                                  std::uint32_t* rejected = nullptr) {
     R r = R(y.r), th = R(y.th);
@@ -107,6 +129,10 @@ __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneCons
                                               R(y.v0), R(y.v1), R(y.hprev), sc, k);
             kerr_dense_crossing(d, frac, rc, phc);
         }
+        if (XTRA) {
+            phc = R(y.ph);  // the reference shades at phi of the step after the crossing
+            if ((sc.refine & SING_REFINE_CROSS64) && y.steps > 0u) refined_crossing(y, sc, k, rc, phc);
+        }
         if (TRACK) {
             const R tol = R(kEdgeRel) * rc;
             const R d_in = rc - sc.r_in, d_out = rc - sc.r_out;
@@ -114,7 +140,7 @@ __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneCons
         }
         if (rc >= sc.r_in && rc <= sc.r_out) {
             y.aux0 = S(rc);
-            if (ADAPT) y.aux1 = S(phc);
+            if (ADAPT || XTRA) y.aux1 = S(phc);
             if (!KERR) {
                 const R pu = R(y.prev_uph);
                 y.aux1 = S(pu + frac * (R(y.v3) - pu));
@@ -153,8 +179,20 @@ __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneCons
                 ct = R(y.cth);
             }
             const R th0 = th;
-            kerr_rk4_raw<R>(r, th, ph, pr, pt, sc.h * kerr_pole_scale(st), st, ct, sc, k);
-            if (((y.steps + 1u) & 7u) != 0u) {
+            const R hs = sc.h * kerr_pole_scale(st);
+            if (XTRA) {
+                y.prev_uph = y.ph;
+                y.prev_v0 = y.v0;
+                y.prev_v1 = y.v1;
+                y.hprev = S(hs);
+            }
+            kerr_rk4_raw<R>(r, th, ph, pr, pt, hs, st, ct, sc, k);
+            if (XTRA) {
+                sincos_shift(th0, th - th0, st, ct, &st, &ct);
+                if (sc.refine & SING_REFINE_REPROJECT) kerr_reproject(r, pr, pt, st, sc, k);
+                y.sth = S(st);
+                y.cth = S(ct);
+            } else if (((y.steps + 1u) & 7u) != 0u) {
                 sincos_shift(th0, th - th0, st, ct, &st, &ct);
                 y.sth = S(st);
                 y.cth = S(ct);
@@ -181,14 +219,14 @@ __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneCons
 
 // Up to n steps of a live ray (advance); returns its termination code, or
 // kTermLive if it is still marching, and clears live when it ends.
-template <class R, class S, bool KERR, bool TRACK, bool ADAPT>
+template <class R, class S, bool KERR, bool TRACK, bool ADAPT, bool XTRA = false>
 __device__ __forceinline__ std::uint32_t run_steps(RayState<S>& y, const SceneConst<R>& sc, bool& live,
                                                    std::uint32_t n, std::uint32_t* rejected) {
     std::uint32_t term = kTermLive;
     const KerrK<R> k = ray_k<R, S, KERR>(y, sc);
     for (std::uint32_t i = 0; i < n; ++i) {
         if (live) {
-            const std::uint32_t t = advance<R, S, KERR, TRACK, ADAPT>(y, sc, k, rejected);
+            const std::uint32_t t = advance<R, S, KERR, TRACK, ADAPT, XTRA>(y, sc, k, rejected);
             if (t != kTermLive) {
                 live = false;
                 term = t;
@@ -284,13 +322,14 @@ __device__ __forceinline__ void start_ray(RayState<S>& y, const SceneConst<float
     if (!KERR) y.prev_uph = y.v3;
 }
 
-template <class S, bool KERR, bool ADAPT = false>
+// DISC_AUX: a disc record's phi is the one kept at the crossing (aux1).
+template <class S, bool KERR, bool ADAPT = false, bool DISC_AUX = ADAPT>
 __device__ __forceinline__ Rec make_rec(const RayState<S>& y, std::uint32_t term) {
     Rec rc;
     rc.meta = term | (y.steps << 4);
     if (term == kTermDisc) {
         rc.a = float(y.aux0);
-        rc.b = ADAPT ? float(y.aux1) : float(y.ph);
+        rc.b = DISC_AUX ? float(y.aux1) : float(y.ph);
         rc.c = KERR ? float(y.v3) : float(y.aux1);
     } else if (ADAPT && term == kTermEscape) {
         rc.a = float(y.aux0);  // dense-output direction at the escape radius

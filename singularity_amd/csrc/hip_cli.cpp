@@ -143,6 +143,9 @@ void print_help() {
               "  --adapt-tol T        Error-controlled RK4 with per-step tolerance T (Kerr scenes)\n"
               "  --sampler S          halton (default: Halton offsets, mean of tone-mapped samples) |\n"
               "                       grid (NxN grid, N = min(spp, 4), tone map of the mean radiance)\n"
+              "  --refine LIST        Fixed-step Kerr refinements, comma-separated: cross64 (disc crossing\n"
+              "                       at the FP64 dense-output root), reproject (null-constraint\n"
+              "                       re-projection after every step)\n"
               "  --stats              Print per-frame timing and step counts.\n");
 }
 
@@ -232,6 +235,19 @@ int main(int argc, char** argv) {
             else {
                 std::fprintf(stderr, "hip_cli: --sampler must be halton or grid\n");
                 return 2;
+            }
+        } else if (a == "--refine" && has) {
+            std::string_view rest = argv[++i];
+            while (!rest.empty()) {
+                const std::size_t comma = rest.find(',');
+                const std::string_view item = rest.substr(0, comma);
+                if (item == "cross64") opts.refine |= SING_REFINE_CROSS64;
+                else if (item == "reproject") opts.refine |= SING_REFINE_REPROJECT;
+                else {
+                    std::fprintf(stderr, "hip_cli: --refine takes cross64 and/or reproject\n");
+                    return 2;
+                }
+                rest = comma == std::string_view::npos ? std::string_view() : rest.substr(comma + 1);
             }
         }
         else {

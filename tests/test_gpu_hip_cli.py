@@ -68,6 +68,18 @@ def test_png_equals_library_frame(tmp_path):
     np.testing.assert_array_equal(read_pgm(tmp_path / "a.pgm"), f.cls)
 
 
+def test_refine_png_equals_library_frame(tmp_path):
+    scene = GOLDEN / "kerr_a094.scene"
+    out = tmp_path / "r.png"
+    r = run("--scene", scene, "--output", out, "--res", f"{W}x{H}", "--precision", "mixed",
+            "--refine", "cross64,reproject", "--hdr-output", tmp_path / "r.pfm")
+    assert r.returncode == 0, r.stderr
+    f = library_frame(scene, prec=sa.PREC_MIXED, refine=sa.REFINE_CROSS64 | sa.REFINE_REPROJECT)
+    np.testing.assert_array_equal(load_png(out), f.rgba)
+    np.testing.assert_array_equal(read_pfm(tmp_path / "r.pfm"), f.hdr)
+    assert not np.array_equal(f.hdr, library_frame(scene, prec=sa.PREC_MIXED).hdr)
+
+
 def test_grid_sampler_png_equals_library_frame(tmp_path):
     scene = GOLDEN / "kerr_a094.scene"
     out = tmp_path / "g.png"

@@ -26,7 +26,7 @@ def test_help_lists_the_reference_flags():
     for flag in ("--scene", "--output", "--res", "--samples-per-pixel", "--camera-distance", "--camera-elevation",
                  "--camera-azimuth", "--fov-y-deg", "--frames", "--output-pattern", "--encode-mp4", "--mp4-fps",
                  "--precision", "--hdr-output", "--class-output", "--bloom", "--adapt-tol", "--turb-seed",
-                 "--sampler"):
+                 "--sampler", "--refine"):
         assert flag in r.stdout, flag
 
 
@@ -39,6 +39,7 @@ def test_help_lists_the_reference_flags():
     (("--mp4-fps", "0"), "--mp4-fps must be in [1, 240]"),
     (("--precision", "fp16"), "--precision must be fp32, fp64 or mixed"),
     (("--sampler", "stratified"), "--sampler must be halton or grid"),
+    (("--refine", "cross64,newton"), "--refine takes cross64 and/or reproject"),
     (("--bogus",), "unknown argument: --bogus"),
     (("--frames", "3"), "--frames > 1 requires --output-pattern"),
     (("--frames", "3", "--output-pattern", "f.png"), "must contain a printf placeholder"),

@@ -302,11 +302,13 @@ def c4_leg(seed: int = 20261203, reps: int = 2) -> dict:
     return res
 
 
-def adjoint_leg(evals: int = 50) -> dict:
-    """grads/s of the 512x512 inverse render: one evaluation = forward with
+def adjoint_leg(evals: int = 50, W: int = 512) -> dict:
+    """grads/s of the WxW (512x512) inverse render: one evaluation = forward with
     checkpoints + discrete adjoint + reduction of dL/d(a/M, incl, r_in, r_out, Ts),
-    at the optimisation's start point against a target rendered at the goal."""
-    W = 512
+    at the optimisation's start point against a target rendered at the goal.
+    Also the longest ray's RK4 steps per precision: each launch lasts as long as
+    its longest ray, so forward_ms / longest steps is the per-step time on the
+    critical path."""
     be = sa.HipBackend(W, W)
     base = sa.diff_base_uniforms(W, W)
     cam = sa.DiffCamera(50.0, 0.0, 50.0)
@@ -331,9 +333,14 @@ def adjoint_leg(evals: int = 50) -> dict:
             fwd.append(st.forward_ms)
             bwd.append(st.backward_ms)
         wall = time.perf_counter() - t0
+        _, meta = sa.diff_capture_pixels(be)
+        steps, on64 = meta[..., 0], meta[..., 3] == 1
         res[name] = {"grads_per_s": round(evals / wall, 1), "forward_ms": round(float(np.mean(fwd)), 3),
                      "adjoint_ms": round(float(np.mean(bwd)), 3), "fp64_rays": int(st.fp64_rays),
-                     "forward_steps": int(st.steps)}
+                     "forward_steps": int(st.steps),
+                     "forward_steps_per_s": round(int(st.steps) / (float(np.mean(fwd)) * 1e-3), 1),
+                     "longest_ray_steps_fp32": int(steps[~on64].max()) if (~on64).any() else 0,
+                     "longest_ray_steps_fp64": int(steps[on64].max()) if on64.any() else 0}
         log(f"adjoint {name}: {res[name]}")
     be.shutdown()
     return res

@@ -158,22 +158,26 @@ template <class T> SING_HD void hsincos(T x, T* s, T* c) {
     const T k = rint(x * T(0.636619772367581343075535053490057448));
     T r = fma(-k, T(1.5707963267948966), x);
     r = fma(-k, T(6.123233995736766e-17), r);
-    const T r2 = r * r;
-    T ps = fma(r2, T(2.8114572543455206e-15), T(-7.647163731819816e-13));
-    ps = fma(r2, ps, T(1.6059043836821613e-10));
-    ps = fma(r2, ps, T(-2.505210838544172e-08));
-    ps = fma(r2, ps, T(2.7557319223985893e-06));
-    ps = fma(r2, ps, T(-0.0001984126984126984));
-    ps = fma(r2, ps, T(0.008333333333333333));
-    ps = fma(r2, ps, T(-0.16666666666666666));
+    // sums of coefficient x power, highest first (each FMA accumulates into
+    // the running sum; see dsincos in physics.hpp)
+    const T r2 = r * r, r4 = r2 * r2, r6 = r4 * r2, r8 = r4 * r4, r10 = r8 * r2, r12 = r8 * r4, r14 = r8 * r6;
+    T ps = T(2.8114572543455206e-15) * r14;
+    ps = fma(T(-7.647163731819816e-13), r12, ps);
+    ps = fma(T(1.6059043836821613e-10), r10, ps);
+    ps = fma(T(-2.505210838544172e-08), r8, ps);
+    ps = fma(T(2.7557319223985893e-06), r6, ps);
+    ps = fma(T(-0.0001984126984126984), r4, ps);
+    ps = fma(T(0.008333333333333333), r2, ps);
+    ps = ps - T(0.16666666666666666);
     const T sr = fma(r * r2, ps, r);
-    T pc = fma(r2, T(4.779477332387385e-14), T(-1.1470745597729725e-11));
-    pc = fma(r2, pc, T(2.08767569878681e-09));
-    pc = fma(r2, pc, T(-2.755731922398589e-07));
-    pc = fma(r2, pc, T(2.48015873015873e-05));
-    pc = fma(r2, pc, T(-0.001388888888888889));
-    pc = fma(r2, pc, T(0.041666666666666664));
-    pc = fma(r2, pc, T(-0.5));
+    T pc = T(4.779477332387385e-14) * r14;
+    pc = fma(T(-1.1470745597729725e-11), r12, pc);
+    pc = fma(T(2.08767569878681e-09), r10, pc);
+    pc = fma(T(-2.755731922398589e-07), r8, pc);
+    pc = fma(T(2.48015873015873e-05), r6, pc);
+    pc = fma(T(-0.001388888888888889), r4, pc);
+    pc = fma(T(0.041666666666666664), r2, pc);
+    pc = pc - T(0.5);
     const T cr = fma(r2, pc, T(1));
     const int q = int(k);
     const T ss = (q & 1) ? cr : sr;
@@ -222,16 +226,18 @@ SING_HD void hsincos_shift(float th0, float d, float s0, float c0, float* s, flo
 }
 SING_HD void hsincos_shift(double th0, double d, double s0, double c0, double* s, double* c) {
     if (fabs(d) <= 0.125) {
-        const double d2 = d * d;
-        double ps = fma(d2, 2.505210838544172e-08, -2.755731922398589e-06);
-        ps = fma(d2, ps, 1.984126984126984e-04);
-        ps = fma(d2, ps, -8.333333333333333e-03);
-        ps = fma(d2, ps, 1.6666666666666666e-01);
+        const double d2 = d * d, d4 = d2 * d2, d6 = d4 * d2, d8 = d4 * d4;
+        double ps = 2.505210838544172e-08 * d8;
+        ps = fma(-2.755731922398589e-06, d6, ps);
+        ps = fma(1.984126984126984e-04, d4, ps);
+        ps = fma(-8.333333333333333e-03, d2, ps);
+        ps = ps + 1.6666666666666666e-01;
         const double sd = fma(-(d * d2), ps, d);
-        double pc = fma(d2, -2.755731922398589e-07, 2.48015873015873e-05);
-        pc = fma(d2, pc, -1.388888888888889e-03);
-        pc = fma(d2, pc, 4.1666666666666664e-02);
-        pc = fma(d2, pc, -0.5);
+        double pc = -2.755731922398589e-07 * d8;
+        pc = fma(2.48015873015873e-05, d6, pc);
+        pc = fma(-1.388888888888889e-03, d4, pc);
+        pc = fma(4.1666666666666664e-02, d2, pc);
+        pc = pc - 0.5;
         const double cd = fma(d2, pc, 1.0);
         *s = fma(s0, cd, c0 * sd);
         *c = fma(c0, cd, -(s0 * sd));

@@ -262,8 +262,8 @@ template <class T> struct KG {  // adjoints of the constants
 // F = Delta p_r^2 + p_th^2 - P^2/Delta + L^2/s2 - 2 a E L + a^2 E^2 sin^2,
 // P = (r^2 + a^2) E - a L (s2 = max(sin^2, 1e-10); r^2 + a^2 - Delta = 2 M r
 // makes this the reference's B/(Sigma Delta) + L^2/(Sigma s2) + (Delta p_r^2 +
-// p_th^2)/Sigma), with one reciprocal.  kerr_rhs_vjp differentiates the
-// reference's grouping of the same function.
+// p_th^2)/Sigma), with one reciprocal.  kerr_rhs_vjp_sc differentiates this
+// form.
 template <class T> SING_HD KY<T> kerr_rhs_sc(const KY<T>& y, const KC<T>& c, T st, T ct) {
     const T a = c.a, a2 = a * a, E = c.E, L = c.L;
     const T aE = a * E;
@@ -296,190 +296,119 @@ template <class T> SING_HD KY<T> kerr_rhs(const KY<T>& y, const KC<T>& c) {
     return kerr_rhs_sc(y, c, st, ct);
 }
 
-// v^T (d rhs / d(y, E, L, a)): accumulates into gy (no phi dependence) and gc.
-template <class T> SING_HD void kerr_rhs_vjp(const KY<T>& y, const KC<T>& c, const KY<T>& v, KY<T>& gy, KG<T>& gc) {
-    // ---- forward recompute
-    T st, ct;
-    hsincos(y.th, &st, &ct);
-    const T twoM = T(2) * c.M;
-    const T a2 = c.a * c.a, EE = c.E * c.E, L2 = c.L * c.L;
-    const T cB1 = T(2) * twoM * c.a * c.E * c.L, cB0 = a2 * L2, cphi = twoM * c.a * c.E;
-    const T s2 = st * st, c2 = ct * ct;
+// v^T (d rhs / d(y, E, L, a)) of kerr_rhs_sc, the function the forward
+// evaluates: accumulates into gy (no phi dependence) and gc.  (st, ct) =
+// sin/cos of y.th as the forward evaluated them.  A reverse sweep of the
+// separated form: with 1/Sigma, 1/Delta and 1/s2 as independent quantities
+// (d(1/x) = -(1/x)^2 dx), about half the operations of differentiating the
+// reference's grouping (B/(Sigma Delta) + ...) of the same function.
+template <class T>
+SING_HD void kerr_rhs_vjp_sc(const KY<T>& y, const KC<T>& c, T st, T ct, const KY<T>& v, KY<T>& gy, KG<T>& gc) {
+    // ---- forward recompute (as kerr_rhs_sc)
+    const T a = c.a, a2 = a * a, E = c.E, L = c.L;
+    const T aE = a * E;
+    const T s2 = st * st;
     const bool clamped = s2 < T(1e-10);
     const T safe = clamped ? T(1e-10) : s2;
-    const T r = y.r, r2 = r * r;
-    const T Sig = r2 + a2 * c2;
-    const T Del = r2 - twoM * r + a2;
-    const T r2a2 = r2 + a2;
-    const T A = r2a2 * r2a2 - a2 * Del * s2;
-    const T iS = hrcp(Sig), iD = hrcp(Del), iss = hrcp(safe);
-    const T iSD = iS * iD;
-    const T Nph = cphi * r * s2 + (Del - a2 * s2) * c.L;
-    const T B = cB1 * r - A * EE - cB0;
-    const T twor = T(2) * r;
-    const T Del_r = twor - twoM;
-    const T A_r = T(2) * twor * r2a2 - a2 * Del_r * s2;
-    const T B_r = cB1 - A_r * EE;
-    const T sin2t = T(2) * st * ct;
-    const T Sig_t = -a2 * sin2t;
-    const T B_t = a2 * Del * sin2t * EE;
-    const T iS2 = iS * iS;
+    const T r = y.r;
+    const T r2a2 = r * r + a2;
+    const T Sig = r2a2 - a2 * s2;
+    const T Del = r * (r - T(2) * c.M) + a2;
+    const T P = r2a2 * E - a * L;
+    const T SD = Sig * Del;
+    const T q = hrcp(SD * safe);
+    const T iS = q * (Del * safe), iD = q * (Sig * safe), u = q * SD;
+    const T PiD = P * iD;
     const T pr2 = y.pr * y.pr;
-    const T Dp2 = Del * pr2 + y.pt * y.pt;
-    const T X = twor * Del + Sig * Del_r;
-    const T Y = Sig_t * safe + Sig * sin2t;
-    const T iSD2 = iSD * iSD, iss2 = iss * iss;
-
+    const T F = ((Del * pr2 + y.pt * y.pt) - P * PiD) + (L * L * u - T(2) * aE * L) + aE * aE * s2;
+    const T FiS = F * iS;
+    const T rmM = r - c.M;
+    const T sc = st * ct;
     // ---- reverse sweep
-    T g_r = T(0), g_pr = T(0), g_pt = T(0), g_st = T(0), g_ct = T(0);
-    T g_a2 = T(0), g_EE = T(0), g_L2 = T(0), g_cB1 = T(0), g_cB0 = T(0), g_cphi = T(0), g_L = T(0);
-    T g_s2 = T(0), g_c2 = T(0), g_safe = T(0), g_r2 = T(0), g_Sig = T(0), g_Del = T(0), g_r2a2 = T(0), g_A = T(0);
-    T g_iS = T(0), g_iD = T(0), g_iss = T(0), g_iSD = T(0), g_B = T(0), g_twor = T(0), g_Del_r = T(0);
-    T g_A_r = T(0), g_B_r = T(0), g_sin2t = T(0), g_Sig_t = T(0), g_B_t = T(0), g_iS2 = T(0), g_pr2 = T(0);
-    T g_Dp2 = T(0), g_X = T(0), g_Y = T(0), g_Nph = T(0);
-
-    // d.pr = -0.5 dHdr ; d.pt = -0.5 dHdth
-    const T gHr = T(-0.5) * v.pr;
-    const T gHt = T(-0.5) * v.pt;
-    // dHdth = B_t iSD - B Sig_t iS2 iD - L2 Y iS2 iss^2 - Dp2 Sig_t iS2
-    g_B_t += gHt * iSD;
-    g_iSD += gHt * B_t;
-    g_B -= gHt * Sig_t * iS2 * iD;
-    g_Sig_t -= gHt * B * iS2 * iD;
-    g_iS2 -= gHt * B * Sig_t * iD;
-    g_iD -= gHt * B * Sig_t * iS2;
-    g_L2 -= gHt * Y * iS2 * iss2;
-    g_Y -= gHt * L2 * iS2 * iss2;
-    g_iS2 -= gHt * L2 * Y * iss2;
-    g_iss -= gHt * L2 * Y * iS2 * T(2) * iss;
-    g_Dp2 -= gHt * Sig_t * iS2;
-    g_Sig_t -= gHt * Dp2 * iS2;
-    g_iS2 -= gHt * Dp2 * Sig_t;
-    // dHdr = B_r iSD - B X iSD^2 - L2 twor iS2 iss + Del_r pr2 iS - Dp2 twor iS2
-    g_B_r += gHr * iSD;
-    g_iSD += gHr * B_r;
-    g_B -= gHr * X * iSD2;
-    g_X -= gHr * B * iSD2;
-    g_iSD -= gHr * B * X * T(2) * iSD;
-    g_L2 -= gHr * twor * iS2 * iss;
-    g_twor -= gHr * L2 * iS2 * iss;
-    g_iS2 -= gHr * L2 * twor * iss;
-    g_iss -= gHr * L2 * twor * iS2;
-    g_Del_r += gHr * pr2 * iS;
-    g_pr2 += gHr * Del_r * iS;
-    g_iS += gHr * Del_r * pr2;
-    g_Dp2 -= gHr * twor * iS2;
-    g_twor -= gHr * Dp2 * iS2;
-    g_iS2 -= gHr * Dp2 * twor;
-    // Y = Sig_t safe + Sig sin2t
-    g_Sig_t += g_Y * safe;
-    g_safe += g_Y * Sig_t;
-    g_Sig += g_Y * sin2t;
-    g_sin2t += g_Y * Sig;
-    // X = twor Del + Sig Del_r
-    g_twor += g_X * Del;
-    g_Del += g_X * twor;
-    g_Sig += g_X * Del_r;
-    g_Del_r += g_X * Sig;
-    // Dp2 = Del pr2 + pt^2
-    g_Del += g_Dp2 * pr2;
-    g_pr2 += g_Dp2 * Del;
-    g_pt += g_Dp2 * T(2) * y.pt;
-    // pr2 = pr^2
+    // dpt = (sc iS) (L^2 u^2 - (aE)^2 - a2 FiS)
+    const T B1 = (L * L * (u * u) - aE * aE) - a2 * FiS;
+    const T gB1 = v.pt * (sc * iS);
+    T g_sc = v.pt * B1 * iS;
+    T g_iS = v.pt * B1 * sc;
+    T g_u = gB1 * T(2) * L * L * u;
+    T g_FiS = -gB1 * a2;
+    T g_L = gB1 * T(2) * L * (u * u);
+    T g_aE = -gB1 * T(2) * aE;
+    T g_a2 = -gB1 * FiS;
+    // dpr = C1 iS, C1 = 2E r PiD - rmM (PiD^2 + pr2) + r FiS
+    const T C1 = ((E + E) * r) * PiD - rmM * (PiD * PiD + pr2) + r * FiS;
+    const T gC1 = v.pr * iS;
+    g_iS += v.pr * C1;
+    T g_r = gC1 * (((E + E) * PiD - (PiD * PiD + pr2)) + FiS);
+    T g_PiD = gC1 * ((E + E) * r - T(2) * rmM * PiD);
+    T g_pr2 = -gC1 * rmM;
+    g_FiS += gC1 * r;
+    T g_E = gC1 * T(2) * r * PiD;
+    // dph = D1 iS, D1 = a PiD + L u - aE
+    const T D1 = a * PiD + (L * u - aE);
+    const T gD1 = v.ph * iS;
+    g_iS += v.ph * D1;
+    g_PiD += gD1 * a;
+    g_u += gD1 * L;
+    g_L += gD1 * u;
+    T g_a = gD1 * PiD;
+    g_aE -= gD1;
+    // dth = pt iS, dr = Del pr iS
+    T g_pt = v.th * iS;
+    g_iS += v.th * y.pt + v.r * Del * y.pr;
+    T g_Del = v.r * y.pr * iS;
+    T g_pr = v.r * Del * iS;
+    // FiS = F iS
+    const T g_F = g_FiS * iS;
+    g_iS += g_FiS * F;
+    // F = Del pr2 + pt^2 - P PiD + L^2 u - 2 aE L + aE^2 s2
+    g_Del += g_F * pr2;
+    g_pr2 += g_F * Del;
+    g_pt += g_F * T(2) * y.pt;
+    T g_P = -g_F * PiD;
+    g_PiD -= g_F * P;
+    g_u += g_F * L * L;
+    g_L += g_F * (T(2) * L * u - T(2) * aE);
+    g_aE += g_F * (T(2) * aE * s2 - T(2) * L);
+    T g_s2 = g_F * aE * aE;
+    // pr2, PiD = P iD, and the reciprocals iS = 1/Sig, iD = 1/Del, u = 1/safe
     g_pr += g_pr2 * T(2) * y.pr;
-    // iS2 = iS^2
-    g_iS += g_iS2 * T(2) * iS;
-    // B_t = a2 Del sin2t EE
-    g_a2 += g_B_t * Del * sin2t * EE;
-    g_Del += g_B_t * a2 * sin2t * EE;
-    g_sin2t += g_B_t * a2 * Del * EE;
-    g_EE += g_B_t * a2 * Del * sin2t;
-    // Sig_t = -a2 sin2t
-    g_a2 -= g_Sig_t * sin2t;
-    g_sin2t -= g_Sig_t * a2;
-    // sin2t = 2 st ct
-    g_st += g_sin2t * T(2) * ct;
-    g_ct += g_sin2t * T(2) * st;
-    // B_r = cB1 - A_r EE
-    g_cB1 += g_B_r;
-    g_A_r -= g_B_r * EE;
-    g_EE -= g_B_r * A_r;
-    // A_r = 2 twor r2a2 - a2 Del_r s2
-    g_twor += g_A_r * T(2) * r2a2;
-    g_r2a2 += g_A_r * T(2) * twor;
-    g_a2 -= g_A_r * Del_r * s2;
-    g_Del_r -= g_A_r * a2 * s2;
-    g_s2 -= g_A_r * a2 * Del_r;
-    // Del_r = twor - twoM
-    g_twor += g_Del_r;
-    // twor = 2 r
-    g_r += g_twor * T(2);
-    // B = cB1 r - A EE - cB0
-    g_cB1 += g_B * r;
-    g_r += g_B * cB1;
-    g_A -= g_B * EE;
-    g_EE -= g_B * A;
-    g_cB0 -= g_B;
-    // d.ph = Nph iSD iss
-    g_Nph += v.ph * iSD * iss;
-    g_iSD += v.ph * Nph * iss;
-    g_iss += v.ph * Nph * iSD;
-    // Nph = cphi r s2 + (Del - a2 s2) L
-    g_cphi += g_Nph * r * s2;
-    g_r += g_Nph * cphi * s2;
-    g_s2 += g_Nph * (cphi * r - a2 * c.L);
-    g_Del += g_Nph * c.L;
-    g_a2 -= g_Nph * s2 * c.L;
-    g_L += g_Nph * (Del - a2 * s2);
-    // d.th = pt iS
-    g_pt += v.th * iS;
-    g_iS += v.th * y.pt;
-    // d.r = Del pr iS
-    g_Del += v.r * y.pr * iS;
-    g_pr += v.r * Del * iS;
-    g_iS += v.r * Del * y.pr;
-    // iSD = iS iD
-    g_iS += g_iSD * iD;
-    g_iD += g_iSD * iS;
-    // reciprocals
-    g_Sig -= g_iS * iS * iS;
+    g_P += g_PiD * iD;
+    const T g_iD = g_PiD * P;
+    T g_Sig = -g_iS * iS * iS;
     g_Del -= g_iD * iD * iD;
-    g_safe -= g_iss * iss * iss;
-    // A = r2a2^2 - a2 Del s2
-    g_r2a2 += g_A * T(2) * r2a2;
-    g_a2 -= g_A * Del * s2;
-    g_Del -= g_A * a2 * s2;
-    g_s2 -= g_A * a2 * Del;
-    // r2a2 = r2 + a2
-    g_r2 += g_r2a2;
-    g_a2 += g_r2a2;
-    // Del = r2 - twoM r + a2
-    g_r2 += g_Del;
-    g_r -= g_Del * twoM;
+    const T g_safe = -g_u * u * u;
+    // P = r2a2 E - a L
+    T g_r2a2 = g_P * E;
+    g_E += g_P * r2a2;
+    g_a -= g_P * L;
+    g_L -= g_P * a;
+    // Del = r (r - 2M) + a2, Sig = r2a2 - a2 s2, r2a2 = r^2 + a2
+    g_r += g_Del * T(2) * rmM;
     g_a2 += g_Del;
-    // Sig = r2 + a2 c2
-    g_r2 += g_Sig;
-    g_a2 += g_Sig * c2;
-    g_c2 += g_Sig * a2;
-    // r2 = r^2
-    g_r += g_r2 * T(2) * r;
-    // safe = max(s2, 1e-10)
+    g_r2a2 += g_Sig;
+    g_a2 -= g_Sig * s2;
+    g_s2 -= g_Sig * a2;
+    g_r += g_r2a2 * T(2) * r;
+    g_a2 += g_r2a2;
+    // safe = max(s2, 1e-10), s2 = st^2, sc = st ct; aE = a E, a2 = a^2
     if (!clamped) g_s2 += g_safe;
-    // s2 = st^2, c2 = ct^2
-    g_st += g_s2 * T(2) * st;
-    g_ct += g_c2 * T(2) * ct;
-    // constants: a2 = a^2, EE = E^2, L2 = L^2, cB1 = 2 twoM a E L, cB0 = a2 L2, cphi = twoM a E
-    g_a2 += g_cB0 * L2;
-    g_L2 += g_cB0 * a2;
-    g_L += g_L2 * T(2) * c.L + g_cB1 * T(2) * twoM * c.a * c.E;
+    const T g_st = g_s2 * T(2) * st + g_sc * ct;
+    const T g_ct = g_sc * st;
+    g_a += g_aE * E + g_a2 * T(2) * a;
+    g_E += g_aE * a;
+    gc.E += g_E;
     gc.L += g_L;
-    gc.E += g_EE * T(2) * c.E + g_cB1 * T(2) * twoM * c.a * c.L + g_cphi * twoM * c.a;
-    gc.a += g_a2 * T(2) * c.a + g_cB1 * T(2) * twoM * c.E * c.L + g_cphi * twoM * c.E;
-    // theta through (st, ct)
+    gc.a += g_a;
     gy.r += g_r;
     gy.th += g_st * ct - g_ct * st;
     gy.pr += g_pr;
     gy.pt += g_pt;
+}
+template <class T> SING_HD void kerr_rhs_vjp(const KY<T>& y, const KC<T>& c, const KY<T>& v, KY<T>& gy, KG<T>& gc) {
+    T st, ct;
+    hsincos(y.th, &st, &ct);
+    kerr_rhs_vjp_sc(y, c, st, ct, v, gy, gc);
 }
 
 template <class T> SING_HD KY<T> ky_axpy(const KY<T>& y, const KY<T>& k, T h) {
@@ -543,17 +472,23 @@ template <class T> SING_HD KY<T> kerr_step(const KY<T>& y, const KC<T>& c, T hs)
 
 // Discrete adjoint of one step: given lam = dJ/d y_{n+1}, returns dJ/d y_n and
 // accumulates dJ/d(E, L, a) into gc.
+// The stages are recomputed exactly as kerr_step evaluates them (sin/cos of
+// each stage's theta included), and the reverse sweep reuses those sin/cos.
 template <class T> SING_HD KY<T> kerr_step_vjp(const KY<T>& y, const KC<T>& c, T hs, const KY<T>& lam, KG<T>& gc) {
-    T dh;
-    const T h = pole_step(y.th, hs, &dh);
+    T dh, st1, ct1, st2, ct2, st3, ct3, st4, ct4;
+    hsincos(y.th, &st1, &ct1);
+    const T h = pole_step_sc(st1, ct1, hs, &dh);
     const T hh = T(0.5) * h;
-    const KY<T> k1 = kerr_rhs(y, c);
+    const KY<T> k1 = kerr_rhs_sc(y, c, st1, ct1);
+    stage_sincos(y.th, hh * k1.th, st1, ct1, &st2, &ct2);
     const KY<T> y2 = ky_axpy(y, k1, hh);
-    const KY<T> k2 = kerr_rhs(y2, c);
+    const KY<T> k2 = kerr_rhs_sc(y2, c, st2, ct2);
+    stage_sincos(y.th, hh * k2.th, st1, ct1, &st3, &ct3);
     const KY<T> y3 = ky_axpy(y, k2, hh);
-    const KY<T> k3 = kerr_rhs(y3, c);
+    const KY<T> k3 = kerr_rhs_sc(y3, c, st3, ct3);
+    stage_sincos(y.th, h * k3.th, st1, ct1, &st4, &ct4);
     const KY<T> y4 = ky_axpy(y, k3, h);
-    const KY<T> k4 = kerr_rhs(y4, c);
+    const KY<T> k4 = kerr_rhs_sc(y4, c, st4, ct4);
     // y' = y + h/6 (k1 + 2 k2 + 2 k3 + k4)
     KY<T> gy = lam;
     const T h6 = h * T(1.0 / 6.0), h3 = h * T(1.0 / 3.0);
@@ -570,25 +505,25 @@ template <class T> SING_HD KY<T> kerr_step_vjp(const KY<T>& y, const KC<T>& c, T
     T gh = ky_dot(lam, sum) * T(1.0 / 6.0);
     // k4 = f(y4), y4 = y + h k3
     KY<T> gy4 = {T(0), T(0), T(0), T(0), T(0)};
-    kerr_rhs_vjp(y4, c, gk4, gy4, gc);
+    kerr_rhs_vjp_sc(y4, c, st4, ct4, gk4, gy4, gc);
     gy4.ph = T(0);  // f does not depend on phi; y4.ph feeds nothing
     gy = ky_axpy(gy, gy4, T(1));
     gk3 = ky_axpy(gk3, gy4, h);
     gh += ky_dot(gy4, k3);
     // k3 = f(y3), y3 = y + hh k2
     KY<T> gy3 = {T(0), T(0), T(0), T(0), T(0)};
-    kerr_rhs_vjp(y3, c, gk3, gy3, gc);
+    kerr_rhs_vjp_sc(y3, c, st3, ct3, gk3, gy3, gc);
     gy = ky_axpy(gy, gy3, T(1));
     gk2 = ky_axpy(gk2, gy3, hh);
     gh += T(0.5) * ky_dot(gy3, k2);
     // k2 = f(y2), y2 = y + hh k1
     KY<T> gy2 = {T(0), T(0), T(0), T(0), T(0)};
-    kerr_rhs_vjp(y2, c, gk2, gy2, gc);
+    kerr_rhs_vjp_sc(y2, c, st2, ct2, gk2, gy2, gc);
     gy = ky_axpy(gy, gy2, T(1));
     gk1 = ky_axpy(gk1, gy2, hh);
     gh += T(0.5) * ky_dot(gy2, k1);
     // k1 = f(y)
-    kerr_rhs_vjp(y, c, gk1, gy, gc);
+    kerr_rhs_vjp_sc(y, c, st1, ct1, gk1, gy, gc);
     // h = pole_step(theta)
     gy.th += gh * dh;
     return gy;

@@ -439,13 +439,14 @@ template <class T> SING_HD T pole_step(T th, T hs, T* dh_dth) {
     return pole_step_sc(st, ct, hs, dh_dth);
 }
 
-// sin/cos of a stage's theta th0 + d: FP32 rotates (s0, c0) (hsincos_shift);
-// FP64 evaluates in full (the rotation's registers push the FP64 backward
-// kernel, which recomputes steps inside the VJP sweep, into more spills).
+// sin/cos of a stage's theta th0 + d: (s0, c0) rotated by d (hsincos_shift),
+// in both precisions.
 SING_HD void stage_sincos(float th0, float d, float s0, float c0, float* s, float* c) {
     hsincos_shift(th0, d, s0, c0, s, c);
 }
-SING_HD void stage_sincos(double th0, double d, double, double, double* s, double* c) { hsincos(th0 + d, s, c); }
+SING_HD void stage_sincos(double th0, double d, double s0, double c0, double* s, double* c) {
+    hsincos_shift(th0, d, s0, c0, s, c);
+}
 
 // The damped RK4 step.  sin/cos of theta are evaluated once, shared by the
 // pole damping and the first stage.

@@ -97,6 +97,9 @@ __global__ __launch_bounds__(256) void diff_classify_kernel(const DiffSet* __res
 // pixels; 3: FP64 over a list (b.list / b.count).
 template <class R, int MODE>
 __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict__ ds, DiffBuffers b) {
+    // the mixed form's FP64 lists (MODE 3) hold its longest rays, which set the
+    // evaluation's latency while FP32 waves share their SIMDs: issue first
+    if (MODE == 3) __builtin_amdgcn_s_setprio(2);
     const std::uint32_t idx = blockIdx.x * 256u + threadIdx.x;
     std::uint32_t pix;
     if (MODE == 3) {
@@ -224,6 +227,7 @@ __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict
 // b.list; 2: the pixels of b.list, then those of b.list2.
 template <class R, int MODE>
 __global__ __launch_bounds__(256) void diff_bwd_kernel(const DiffSet* __restrict__ ds, DiffBuffers b) {
+    if (MODE == 2) __builtin_amdgcn_s_setprio(2);  // as the forward's FP64 lists
     const std::uint32_t idx = blockIdx.x * 256u + threadIdx.x;
     std::uint32_t pix;
     if (MODE == 1) {

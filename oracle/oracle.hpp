@@ -23,7 +23,11 @@
 // headers, the shading against the reference's host twins
 // (core/include/physics/{disc,redshift}.hpp) and hand-derived known answers,
 // the bloom passes by their closed forms (uniform blocks, impulses, the knee
-// at the threshold, the weight sum).
+// at the threshold, the weight sum); the options beyond the reference by
+// convergence and closed forms (tests/test_oracle_adaptive.py: error-controlled
+// steps; tests/test_oracle_refine.py: the refined crossing against straight
+// lines in flat space, the null-constraint re-projection against the exact
+// second-order residual of its Newton step).
 
 #pragma once
 

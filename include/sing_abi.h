@@ -183,6 +183,18 @@ int sing_capture_class(SingContext* ctx, uint8_t* dst, size_t bytes);
 int sing_capture_steps(SingContext* ctx, uint32_t* dst, size_t bytes);
 int sing_capture_hdr16(SingContext* ctx, uint16_t* dst, size_t bytes);
 
+/* Re-runs the bloom post pass (extract, 7-tap blurs, composite with exposure,
+ * ACES, sRGB; render/vulkan/shaders/bloom.hlsl:8-77, blit.hlsl:43-58) over the
+ * last frame's half-float HDR (a frame rendered with SING_OUT_HDR16 or bloom)
+ * with the exposure, bloom_threshold and bloom_strength of *u (nothing else of
+ * *u is read), into the RGBA8 image: a new exposure or bloom setting without
+ * re-tracing.  The pass is launched `repeat` times back to back (1..1000;
+ * every launch writes the same image); stats (optional) gets post_ms = the
+ * mean time per launch between HIP events around all of them and kernel_ms =
+ * their total, other fields zero.  No counterpart in the reference, whose post
+ * passes run inside each frame. */
+int sing_post_bloom(SingContext* ctx, const SingUniforms* u, uint32_t repeat, SingRenderStats* stats);
+
 /* --- render without device allocation ---------------------------------------
  * No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, serving loops, harnesses that forbid

@@ -140,6 +140,7 @@ _SIGS = {
     "sing_capture_class": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "sing_capture_steps": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "sing_capture_hdr16": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "sing_post_bloom": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "sing_scene_defaults": (None, [C.POINTER(Scene)]),
     "sing_scene_config_defaults": (None, [C.POINTER(SceneConfig)]),
     "sing_parse_scene_config": (
@@ -299,6 +300,13 @@ class HipBackend:
         st = self.render(u, opts)
         self.frame_time_sec += np.float32(1.0) / np.float32(30.0)
         self.frame_time_sec = float(np.float32(self.frame_time_sec))
+        return st
+
+    def post_bloom(self, u: Uniforms, repeat: int = 1) -> RenderStats:
+        """sing_post_bloom: the bloom pass again over the last frame's half-float HDR with
+        u's exposure and bloom settings (no re-trace); post_ms = mean per launch."""
+        st = RenderStats()
+        self._check(lib().sing_post_bloom(self._ctx, C.byref(u), int(repeat), C.byref(st)))
         return st
 
     def capture(self, outputs: int) -> Frame:

@@ -442,4 +442,34 @@ int sing_capture_hdr16(SingContext* c, uint16_t* dst, size_t bytes) {
              : SING_ERR_ARG;
 }
 
+int sing_post_bloom(SingContext* c, const SingUniforms* u, uint32_t repeat, SingRenderStats* stats) {
+    if (!c) return SING_ERR_ARG;
+    if (!u) return c->fail(SING_ERR_ARG, "sing_post_bloom: null uniforms");
+    if (repeat < 1u || repeat > 1000u) return c->fail(SING_ERR_ARG, "sing_post_bloom: repeat must be in [1, 1000]");
+    if (c->width < 2 || c->height < 2) return c->fail(SING_ERR_ARG, "sing_post_bloom: bloom needs at least 2x2 pixels");
+    if (!(c->outputs_valid & SING_OUT_HDR16))
+        return c->fail(SING_ERR_STATE, "sing_post_bloom: the last frame has no half-float HDR (SING_OUT_HDR16 or bloom)");
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return c->hip_fail(e, "hipSetDevice");
+    if (const int rc = c->ensure_events(2); rc != SING_OK) return rc;
+    hipEvent_t p0 = c->events[0], p1 = c->events[1];
+    hipEventRecord(p0, c->stream);
+    for (std::uint32_t i = 0; i < repeat; ++i) {
+        e = sing::launch_bloom(c->d_hdr16, c->width, c->height, u->bloom_threshold, u->bloom_strength, u->exposure,
+                               c->d_rgba, c->stream);
+        if (e != hipSuccess) return c->hip_fail(e, "bloom launch");
+    }
+    hipEventRecord(p1, c->stream);
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return c->hip_fail(e, "hipStreamSynchronize");
+    c->outputs_valid |= SING_OUT_RGBA8;
+    if (stats) {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, p0, p1);
+        std::memset(stats, 0, sizeof(*stats));
+        stats->kernel_ms = ms;
+        stats->post_ms = double(ms) / repeat;
+    }
+    return SING_OK;
+}
+
 }  // extern "C"

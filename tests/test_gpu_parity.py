@@ -191,6 +191,31 @@ def test_bloom_post_pass_matches_oracle(backends, orc, size):
     assert f.stats.post_ms > 0.0
 
 
+def test_post_bloom_again_equals_a_frame_with_those_settings(backends):
+    """sing_post_bloom over the last frame's half-float HDR with a new exposure and
+    bloom setting equals a full frame rendered with them (the trace does not
+    depend on them); repeated launches write the same image; without a half-float
+    HDR in the last frame, and with repeat 0, it is an error."""
+    u = config2(160, 90, 2)
+    be = backends(160, 90)
+    f = render(backends, u, outputs=sa.OUT_HDR16 | sa.OUT_RGBA8, post=sa.POST_BLOOM)
+    st = be.post_bloom(u, repeat=3)
+    assert st.post_ms > 0.0 and st.kernel_ms >= st.post_ms
+    np.testing.assert_array_equal(be.capture(sa.OUT_RGBA8).rgba, f.rgba)
+    u2 = config2(160, 90, 2)
+    u2.exposure, u2.bloom_threshold, u2.bloom_strength = 0.7, 0.4, 0.9
+    be.post_bloom(u2)
+    again = be.capture(sa.OUT_RGBA8).rgba
+    full = render(backends, u2, outputs=sa.OUT_HDR16 | sa.OUT_RGBA8, post=sa.POST_BLOOM)
+    np.testing.assert_array_equal(again, full.rgba)
+    assert not np.array_equal(again, f.rgba)
+    with pytest.raises(sa.SingError):
+        be.post_bloom(u2, repeat=0)
+    render(backends, u2, outputs=sa.OUT_RGBA8)
+    with pytest.raises(sa.SingError):
+        be.post_bloom(u2)
+
+
 def test_resize_and_capture_errors(backends):
     be = sa.HipBackend(16, 16)
     u = scene_uniforms(width=16, height=16)

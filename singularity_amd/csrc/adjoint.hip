@@ -224,23 +224,15 @@ __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict
 }
 
 // MODE 0: every pixel whose forward ran in R's precision; 1: the pixels of
-// b.list; 2: the pixels of b.list, then those of b.list2.
+// b.list (the mixed form's FP64 lists).
 template <class R, int MODE>
 __global__ __launch_bounds__(256) void diff_bwd_kernel(const DiffSet* __restrict__ ds, DiffBuffers b) {
-    if (MODE == 2) __builtin_amdgcn_s_setprio(2);  // as the forward's FP64 lists
+    if (MODE == 1) __builtin_amdgcn_s_setprio(2);  // the FP64 lists, as in the forward
     const std::uint32_t idx = blockIdx.x * 256u + threadIdx.x;
     std::uint32_t pix;
     if (MODE == 1) {
         if (idx >= *b.count) return;
         pix = b.list[idx];
-    } else if (MODE == 2) {  // the classified list, then the late list, in one launch
-        const std::uint32_t n1 = *b.count;
-        if (idx < n1) {
-            pix = b.list[idx];
-        } else {
-            if (idx - n1 >= *b.count2) return;
-            pix = b.list2[idx - n1];
-        }
     } else {
         if (idx >= b.P) return;
         pix = idx;
@@ -454,26 +446,27 @@ hipError_t launch_diff(const DiffSet* ds_dev, const DiffBuffers& b, std::uint32_
         DiffBuffers late = b;
         late.list = b.list2;
         late.count = b.count2;
+        // A pixel's backward needs only its own forward (the loss gradient
+        // dL/dI of the pixel is written at the end of its forward), so the
+        // classified FP64 rays — the evaluation's longest — go straight on to
+        // their backward on stream2 while stream runs the FP32 forward, the
+        // late FP64 forward and both remaining backwards.
         hipLaunchKernelGGL(diff_classify_kernel, grid, block, 0, stream, ds_dev, b);
         (void)hipEventRecord(join[0], stream);
         (void)hipStreamWaitEvent(stream2, join[0], 0);
         hipLaunchKernelGGL((diff_fwd_kernel<double, 3>), grid, block, 0, stream2, ds_dev, b);
         (void)hipEventRecord(join[1], stream2);
+        if (backward) hipLaunchKernelGGL((diff_bwd_kernel<double, 1>), grid, block, 0, stream2, ds_dev, b);
+        (void)hipEventRecord(join[2], stream2);
         hipLaunchKernelGGL((diff_fwd_kernel<float, 1>), grid, block, 0, stream, ds_dev, b);
         hipLaunchKernelGGL((diff_fwd_kernel<double, 3>), grid, block, 0, stream, ds_dev, late);
         (void)hipStreamWaitEvent(stream, join[1], 0);
         (void)hipEventRecord(fwd_end, stream);
         if (backward) {
-            // every FP64 ray (classified and late) in one launch: each launch
-            // lasts as long as its longest ray, so one launch instead of two
-            // in a row halves the FP64 backward's latency floor
-            (void)hipEventRecord(join[2], stream);
-            (void)hipStreamWaitEvent(stream2, join[2], 0);
-            hipLaunchKernelGGL((diff_bwd_kernel<double, 2>), grid, block, 0, stream2, ds_dev, b);
-            (void)hipEventRecord(join[3], stream2);
             hipLaunchKernelGGL((diff_bwd_kernel<float, 0>), grid, block, 0, stream, ds_dev, b);
-            (void)hipStreamWaitEvent(stream, join[3], 0);
+            hipLaunchKernelGGL((diff_bwd_kernel<double, 1>), grid, block, 0, stream, ds_dev, late);
         }
+        (void)hipStreamWaitEvent(stream, join[2], 0);
     }
     hipLaunchKernelGGL(diff_reduce_partial_kernel, dim3(kReduceBlocks), dim3(256), 0, stream, b.loss, b.grad,
                        b.steps, b.P, b.partial);

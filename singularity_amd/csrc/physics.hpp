@@ -50,30 +50,22 @@ __device__ __forceinline__ void dsincos(double x, double* s, double* c) {
     const double k = rint(x * 0.63661977236758134);
     double r = fma(-k, 1.5707963267948966, x);
     r = fma(-k, 6.123233995736766e-17, r);
-    // the polynomials as sums of coefficient x power (highest first): every
-    // FMA accumulates into the running sum, so a coefficient is only ever a
-    // multiplicand.  Horner's form makes it the addend, which the compiler
-    // turned into a 64-bit register copy plus a two-address FMA per term in
-    // the register-bound FP64 kernels.  Same accuracy (<= 1.4 ulp on |x| <= 7).
-    const double r2 = r * r, r4 = r2 * r2, r6 = r4 * r2, r8 = r4 * r4, r10 = r8 * r2, r12 = r8 * r4,
-                 r14 = r8 * r6;
-    double ps = 2.8114572543455206e-15 * r14;
-    ps = fma(-7.647163731819816e-13, r12, ps);
-    ps = fma(1.6059043836821613e-10, r10, ps);
-    ps = fma(-2.505210838544172e-08, r8, ps);
-    ps = fma(2.7557319223985893e-06, r6, ps);
-    ps = fma(-0.0001984126984126984, r4, ps);
-    ps = fma(0.008333333333333333, r2, ps);
-    ps = ps - 0.16666666666666666;
+    const double r2 = r * r;
+    double ps = fma(r2, 2.8114572543455206e-15, -7.647163731819816e-13);
+    ps = fma(r2, ps, 1.6059043836821613e-10);
+    ps = fma(r2, ps, -2.505210838544172e-08);
+    ps = fma(r2, ps, 2.7557319223985893e-06);
+    ps = fma(r2, ps, -0.0001984126984126984);
+    ps = fma(r2, ps, 0.008333333333333333);
+    ps = fma(r2, ps, -0.16666666666666666);
     const double sr = fma(r * r2, ps, r);
-    double pc = 4.779477332387385e-14 * r14;
-    pc = fma(-1.1470745597729725e-11, r12, pc);
-    pc = fma(2.08767569878681e-09, r10, pc);
-    pc = fma(-2.755731922398589e-07, r8, pc);
-    pc = fma(2.48015873015873e-05, r6, pc);
-    pc = fma(-0.001388888888888889, r4, pc);
-    pc = fma(0.041666666666666664, r2, pc);
-    pc = pc - 0.5;
+    double pc = fma(r2, 4.779477332387385e-14, -1.1470745597729725e-11);
+    pc = fma(r2, pc, 2.08767569878681e-09);
+    pc = fma(r2, pc, -2.755731922398589e-07);
+    pc = fma(r2, pc, 2.48015873015873e-05);
+    pc = fma(r2, pc, -0.001388888888888889);
+    pc = fma(r2, pc, 0.041666666666666664);
+    pc = fma(r2, pc, -0.5);
     const double cr = fma(r2, pc, 1.0);
     const int q = int(k);
     double ss = (q & 1) ? cr : sr;
@@ -102,7 +94,15 @@ __device__ __forceinline__ void sincos_shift(float th0, float d, float s0, float
 }
 __device__ __forceinline__ void sincos_shift(double th0, double d, double s0, double c0, double* s, double* c) {
     if (__builtin_expect(fabs(d) <= 0.125, 1)) {
-        // sums of coefficient x power, as in dsincos (double)
+        // sums of coefficient x power, highest first: every FMA accumulates
+        // into the running sum, so a coefficient is only ever a multiplicand.
+        // In Horner's form it is the addend, which the compiler turned into a
+        // 64-bit register copy plus a two-address FMA per term in the
+        // register-bound fixed-step FP64 passes (this rotation runs three
+        // times per step there).  The full evaluation above keeps Horner: it
+        // runs once per 8 steps in those passes, and in the error-controlled
+        // FP64 pass, which evaluates it every stage, the power form was
+        // slower.  Same accuracy either way (<= 0.51 ulp here).
         const double d2 = d * d, d4 = d2 * d2, d6 = d4 * d2, d8 = d4 * d4;
         double ps = 2.505210838544172e-08 * d8;
         ps = fma(-2.755731922398589e-06, d6, ps);

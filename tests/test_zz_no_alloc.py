@@ -138,6 +138,14 @@ def test_noalloc_equals_sing_render_on_two_calls():
     h3, s3 = _noalloc(u3, o3)
     f3, r3 = _reference(u3, o3)
     _compare(h3, f3, s3, r3, outs3)
+    # call 4: mixed with both fixed-step refinements (their own trace kernels)
+    outs4 = sa.OUT_RGBA8 | sa.OUT_HDR | sa.OUT_CLASS
+    u4 = scene_uniforms(metric=sa.METRIC_KERR, spin=0.9, r_in=2.3, width=48, height=32, spp=2)
+    o4 = sa.RenderOpts(precision=sa.PREC_MIXED, outputs=outs4, refine=sa.REFINE_CROSS64 | sa.REFINE_REPROJECT)
+    print("noalloc call 4: Kerr a=0.9 48x32 2 spp mixed, refine cross64+reproject", flush=True)
+    h4, s4 = _noalloc(u4, o4)
+    f4, r4 = _reference(u4, o4)
+    _compare(h4, f4, s4, r4, outs4)
 
 
 def test_noalloc_rejects_frames_beyond_its_scratch():

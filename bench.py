@@ -312,7 +312,9 @@ def adjoint_leg(evals: int = 50, W: int = 512) -> dict:
     at the optimisation's start point against a target rendered at the goal.
     Also the longest ray's RK4 steps per precision: each launch lasts as long as
     its longest ray, so forward_ms / longest steps is the per-step time on the
-    critical path."""
+    critical path.  batched_8: eight independent evaluations (the central-
+    difference stencil of the four free parameters) per sing_diff_loss_grad_batch
+    call."""
     be = sa.HipBackend(W, W)
     base = sa.diff_base_uniforms(W, W)
     cam = sa.DiffCamera(50.0, 0.0, 50.0)
@@ -339,12 +341,25 @@ def adjoint_leg(evals: int = 50, W: int = 512) -> dict:
         wall = time.perf_counter() - t0
         _, meta = sa.diff_capture_pixels(be)
         steps, on64 = meta[..., 0], meta[..., 3] == 1
+        # independent evaluations in one call (sing_diff_loss_grad_batch): the
+        # central-difference stencil of the four free parameters, 8 points
+        x0 = start.as_array()
+        pts = [x0 + sgn * h * np.eye(5)[j] for j, h in ((0, 1e-3), (1, 1e-3), (3, 1e-2), (4, 1e-3))
+               for sgn in (-1.0, 1.0)]
+        sa.diff_loss_grad_batch(be, base, pts, cam, tgt, None, opts(prec))
+        tb = time.perf_counter()
+        for _ in range(max(2, evals // 8)):
+            _, _, _, bst = sa.diff_loss_grad_batch(be, base, pts, cam, tgt, None, opts(prec))
+        batched = len(pts) * max(2, evals // 8) / (time.perf_counter() - tb)
         res[name] = {"grads_per_s": round(evals / wall, 1), "forward_ms": round(float(np.mean(fwd)), 3),
                      "adjoint_ms": round(float(np.mean(bwd)), 3), "fp64_rays": int(st.fp64_rays),
                      "forward_steps": int(st.steps),
                      "forward_steps_per_s": round(int(st.steps) / (float(np.mean(fwd)) * 1e-3), 1),
                      "longest_ray_steps_fp32": int(steps[~on64].max()) if (~on64).any() else 0,
-                     "longest_ray_steps_fp64": int(steps[on64].max()) if on64.any() else 0}
+                     "longest_ray_steps_fp64": int(steps[on64].max()) if on64.any() else 0,
+                     "batched_8_grads_per_s": round(batched, 1),
+                     "batched_8_ms_per_call": round(bst.forward_ms + bst.backward_ms, 3),
+                     "batched_8_forward_steps_per_s": round(int(bst.steps) / (bst.forward_ms * 1e-3), 1)}
         log(f"adjoint {name}: {res[name]}")
     be.shutdown()
     return res

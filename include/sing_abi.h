@@ -340,7 +340,22 @@ int sing_diff_loss_grad(SingContext* ctx, const SingUniforms* base, const SingDi
                         const SingDiffCamera* cam, const SingDiffOpts* opts, const float* target,
                         const float* weights, double* loss, double* grad, float* hdr_out, SingDiffStats* stats);
 
-/* Per-pixel results of the last sing_diff_* call on ctx (either may be NULL):
+/* n parameter points (1..SING_DIFF_MAX_BATCH) of the same image in one set of
+ * launches: the losses (loss[n]) and gradients (grad[n*5]) that n calls of
+ * sing_diff_loss_grad would return, bit for bit.  cams: one camera for all
+ * (n_cams = 1) or one per point (n_cams = n); hdr_out (optional) n*W*H*3
+ * floats, image after image.  One 512x512 evaluation leaves most lanes idle
+ * (its time is its longest ray's), so independent evaluations — finite-
+ * difference stencils, multi-start or population fits, several cameras —
+ * share the launches.  No counterpart in the reference. */
+#define SING_DIFF_MAX_BATCH 32u
+int sing_diff_loss_grad_batch(SingContext* ctx, const SingUniforms* base, uint32_t n, const SingDiffParams* params,
+                              const SingDiffCamera* cams, uint32_t n_cams, const SingDiffOpts* opts,
+                              const float* target, const float* weights, double* loss, double* grad, float* hdr_out,
+                              SingDiffStats* stats);
+
+/* Per-pixel results of the last sing_diff_* call on ctx (either may be NULL);
+ * after a batch call, those of its first parameter point:
  * grad_out W*H*5 doubles, the pixel's loss gradient wrt the 5 parameters (r_in
  * partial; zero after sing_diff_forward); meta_out W*H*4 u32: RK4 steps,
  * accepted disc crossings, termination (1 horizon, 2 escape, 3 disc, 4 budget,

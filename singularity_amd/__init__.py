@@ -378,7 +378,11 @@ _SIGS.update({
     "sing_diff_loss_grad": (C.c_int, [C.c_void_p, C.POINTER(Uniforms), C.POINTER(DiffParams),
                                       C.POINTER(DiffCamera), C.POINTER(DiffOpts), C.c_void_p, C.c_void_p,
                                       C.POINTER(C.c_double), C.c_void_p, C.c_void_p, C.POINTER(DiffStats)]),
+    "sing_diff_loss_grad_batch": (C.c_int, [C.c_void_p, C.POINTER(Uniforms), C.c_uint32, C.c_void_p, C.c_void_p,
+                                            C.c_uint32, C.POINTER(DiffOpts), C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.POINTER(DiffStats)]),
 })
+DIFF_MAX_BATCH = 32
 
 
 _SIGS["sing_diff_capture_pixels"] = (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p])
@@ -440,6 +444,37 @@ def diff_loss_grad(be: "HipBackend", base: Uniforms, params: DiffParams, cam: Di
     st = _diff_call(be, "sing_diff_loss_grad", base, params, cam, opts, t.ctypes.data,
                     None if w is None else w.ctypes.data, C.byref(loss), grad.ctypes.data, hdr.ctypes.data)
     return loss.value, grad, hdr, st
+
+
+def diff_loss_grad_batch(be: "HipBackend", base: Uniforms, params, cams, target: np.ndarray,
+                         weights: np.ndarray | None = None, opts: DiffOpts | None = None, want_hdr: bool = False):
+    """sing_diff_loss_grad_batch: n parameter points (a sequence of DiffParams or an (n, 5)
+    array) in one set of launches; cams is one DiffCamera or one per point.  Returns
+    (losses (n,), gradients (n, 5), radiance (n, H, W, 3) or None, stats)."""
+    plist = [p if isinstance(p, DiffParams) else DiffParams.from_array(p) for p in params]
+    n = len(plist)
+    clist = [cams] if isinstance(cams, DiffCamera) else list(cams)
+    if n < 1 or n > DIFF_MAX_BATCH or len(clist) not in (1, n):
+        raise ValueError(f"{n} parameter points (1..{DIFF_MAX_BATCH}) and {len(clist)} cameras (1 or n)")
+    t = np.ascontiguousarray(target, dtype=np.float32)
+    if t.shape != (be.height, be.width, 3):
+        raise ValueError(f"target shape {t.shape} != {(be.height, be.width, 3)}")
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    if w is not None and w.shape != (be.height, be.width):
+        raise ValueError(f"weights shape {w.shape} != {(be.height, be.width)}")
+    parr = (DiffParams * n)(*plist)
+    carr = (DiffCamera * len(clist))(*clist)
+    loss = np.zeros(n, np.float64)
+    grad = np.zeros((n, 5), np.float64)
+    hdr = np.empty((n, be.height, be.width, 3), np.float32) if want_hdr else None
+    o = opts if opts is not None else DiffOpts()
+    st = DiffStats()
+    be._check(lib().sing_diff_loss_grad_batch(be._ctx, C.byref(base), n, C.addressof(parr), C.addressof(carr),
+                                              len(clist), C.byref(o), t.ctypes.data,
+                                              None if w is None else w.ctypes.data, loss.ctypes.data,
+                                              grad.ctypes.data, None if hdr is None else hdr.ctypes.data,
+                                              C.byref(st)))
+    return loss, grad, hdr, st
 
 
 def adam_fit(be: "HipBackend", base: Uniforms, start, target: np.ndarray, cam: DiffCamera, steps: int = 200,

@@ -23,6 +23,7 @@
 #include <cstdint>
 #include <cstring>
 #include <type_traits>
+#include <vector>
 
 #include "../../include/sing_abi.h"
 #include "adjoint.hpp"
@@ -74,9 +75,10 @@ template <class R> __device__ __forceinline__ KY<R> load_state(const double* p, 
 // meta[3] = FP64) before any march, so the FP32 and FP64 forwards can run
 // concurrently.
 __global__ __launch_bounds__(256) void diff_classify_kernel(const DiffSet* __restrict__ ds, DiffBuffers b) {
-    const std::uint32_t pix = blockIdx.x * 256u + threadIdx.x;
-    if (pix >= b.P) return;
-    const DiffScene<float>& sc = ds->sf;
+    const std::uint32_t ev = blockIdx.y, pix = blockIdx.x * 256u + threadIdx.x;
+    if (pix >= b.Pimg) return;
+    const std::uint32_t q = ev * b.Pimg + pix;
+    const DiffScene<float>& sc = ds[ev].sf;
     KY<float> y;
     float E, L;
     const bool ok = diff_init(sc, pix % sc.width, pix / sc.width, y, E, L);
@@ -88,34 +90,36 @@ __global__ __launch_bounds__(256) void diff_classify_kernel(const DiffSet* __res
         const float rp = sc.M + sqrtf(fmaxf(sc.M * sc.M - sc.a * sc.a, 0.0f));
         hard = hard_ray<float>(L / E, Q / (E * E), sc.a, sc.M, rp, y.pr < 0.0f, kBandDiff);
     }
-    b.meta[4u * pix + 3u] = hard ? kMetaPrec64 : 0u;
-    if (hard) b.list[atomicAdd(b.count, 1u)] = pix;
+    b.meta[4u * q + 3u] = hard ? kMetaPrec64 : 0u;
+    if (hard) b.list[ev * b.Pimg + atomicAdd(b.count + ev, 1u)] = pix;
 }
 
 // MODE 0: FP32 over all pixels; 1: FP32 over the pixels the classification
 // left in FP32, handing grazing rays to the late list (list2); 2: FP64 over all
-// pixels; 3: FP64 over a list (b.list / b.count).
+// pixels; 3: FP64 over a list (b.list / b.count).  Grid: (pixels of one image,
+// evaluations); blockIdx.y is the evaluation, so its scene stays wave-uniform.
 template <class R, int MODE>
 __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict__ ds, DiffBuffers b) {
     // the mixed form's FP64 lists (MODE 3) hold its longest rays, which set the
     // evaluation's latency while FP32 waves share their SIMDs: issue first
     if (MODE == 3) __builtin_amdgcn_s_setprio(2);
-    const std::uint32_t idx = blockIdx.x * 256u + threadIdx.x;
+    const std::uint32_t ev = blockIdx.y, idx = blockIdx.x * 256u + threadIdx.x;
     std::uint32_t pix;
     if (MODE == 3) {
-        if (idx >= *b.count) return;
-        pix = b.list[idx];
+        if (idx >= b.count[ev]) return;
+        pix = b.list[ev * b.Pimg + idx];
     } else {
-        if (idx >= b.P) return;
+        if (idx >= b.Pimg) return;
         pix = idx;
     }
-    const DiffScene<R>& sc = scene_of<R>(ds);
+    const std::uint32_t q = ev * b.Pimg + pix;  // index into the per-pixel arrays
+    const DiffScene<R>& sc = scene_of<R>(ds + ev);
     const std::uint32_t gx = pix % sc.width, gy = pix / sc.width;
     const std::size_t S = b.P;
-    std::uint32_t* meta = b.meta + 4u * pix;
+    std::uint32_t* meta = b.meta + 4u * q;
     if (MODE == 1 && meta[3] == kMetaPrec64) return;  // classified hard: the FP64 forward owns it
-    double* cp = b.cp + pix;
-    double* cr = b.cross + pix;
+    double* cp = b.cp + q;
+    double* cr = b.cross + q;
     KY<R> y;
     R E, L;
     const bool ok = diff_init(sc, gx, gy, y, E, L);
@@ -127,8 +131,8 @@ __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict
     R near_eq = R(1e30), rmin = R(1e30);
     if (ok) {
         const KC<R> c{E, L, sc.a, sc.M};
-        b.EL[pix] = double(E);
-        b.EL[S + pix] = double(L);
+        b.EL[q] = double(E);
+        b.EL[S + q] = double(L);
         store_state(cp, S, y);
         R prev_r = y.r, prev_th = y.th, prev_ph = y.ph;
         const R mid = R(1.5707963267948966192313216916397514);
@@ -191,7 +195,7 @@ __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict
         const R rp = sc.M + hsqrt(hmax(sc.M * sc.M - sc.a * sc.a, R(0)));
         if (MODE == 1 && (float(near_eq) < kGrazeThetaDiff || (ncross > 0u && rmin < R(2) * rp))) {
             meta[3] = kMetaPrec64;
-            b.list2[atomicAdd(b.count2, 1u)] = pix;
+            b.list2[ev * b.Pimg + atomicAdd(b.count2 + ev, 1u)] = pix;
             return;
         }
         if (term == kTermEscape && (sc.flags & SING_FLAG_STARFIELD_ON)) {
@@ -208,19 +212,19 @@ __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict
     meta[0] = step;
     meta[1] = ncross;
     meta[2] = term;
-    double* skp = b.sky + 3u * pix;
+    double* skp = b.sky + 3u * q;
     const float* tgt = b.target + 3u * pix;
     const double w = b.weight ? double(b.weight[pix]) : 1.0;
     double l = 0.0;
     for (int k = 0; k < 3; ++k) {
         skp[k] = sky[k];
-        b.hdr[3u * pix + k] = float(acc[k]);
+        b.hdr[3u * q + k] = float(acc[k]);
         const double diff = double(acc[k]) - double(tgt[k]);
-        b.gI[3u * pix + k] = 2.0 * w * diff;
+        b.gI[3u * q + k] = 2.0 * w * diff;
         l += w * diff * diff;
     }
-    b.loss[pix] = l;
-    b.steps[pix] = step;
+    b.loss[q] = l;
+    b.steps[q] = step;
 }
 
 // MODE 0: every pixel whose forward ran in R's precision; 1: the pixels of
@@ -228,17 +232,19 @@ __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict
 template <class R, int MODE>
 __global__ __launch_bounds__(256) void diff_bwd_kernel(const DiffSet* __restrict__ ds, DiffBuffers b) {
     if (MODE == 1) __builtin_amdgcn_s_setprio(2);  // the FP64 lists, as in the forward
-    const std::uint32_t idx = blockIdx.x * 256u + threadIdx.x;
+    const std::uint32_t ev = blockIdx.y, idx = blockIdx.x * 256u + threadIdx.x;
     std::uint32_t pix;
     if (MODE == 1) {
-        if (idx >= *b.count) return;
-        pix = b.list[idx];
+        if (idx >= b.count[ev]) return;
+        pix = b.list[ev * b.Pimg + idx];
     } else {
-        if (idx >= b.P) return;
+        if (idx >= b.Pimg) return;
         pix = idx;
     }
-    const std::uint32_t* meta = b.meta + 4u * pix;
-    double* g = b.grad + 5u * pix;
+    const std::uint32_t q = ev * b.Pimg + pix;  // index into the per-pixel arrays
+    ds += ev;
+    const std::uint32_t* meta = b.meta + 4u * q;
+    double* g = b.grad + 5u * q;
     const bool want64 = std::is_same<R, double>::value;
     if ((meta[3] == kMetaPrec64) != want64) return;
     for (int k = 0; k < 5; ++k) g[k] = 0.0;
@@ -252,9 +258,9 @@ __global__ __launch_bounds__(256) void diff_bwd_kernel(const DiffSet* __restrict
     if (J == 0u) return;  // no crossing: the radiance has no parameter dependence
     const std::size_t S = b.P;
     // the forward's own constants, so the recomputed segments repeat it exactly
-    const KC<R> c{R(b.EL[pix]), R(b.EL[S + pix]), sc.a, sc.M};
-    const double* gI = b.gI + 3u * pix;
-    const double* cr = b.cross + pix;
+    const KC<R> c{R(b.EL[q]), R(b.EL[S + q]), sc.a, sc.M};
+    const double* gI = b.gI + 3u * q;
+    const double* cr = b.cross + q;
 
     // ---- crossing gradients (front-to-back composite)
     R tau[kMaxCross];
@@ -266,7 +272,7 @@ __global__ __launch_bounds__(256) void diff_bwd_kernel(const DiffSet* __restrict
     const bool sky_behind = (term == kTermEscape);
     R behind[3] = {R(0), R(0), R(0)};
     if (sky_behind)
-        for (int k = 0; k < 3; ++k) behind[k] = R(b.sky[3u * pix + k]);
+        for (int k = 0; k < 3; ++k) behind[k] = R(b.sky[3u * q + k]);
     R rc_bar[kMaxCross], ph_bar[kMaxCross];
     R gL = R(0), g_a = R(0), g_rin = R(0), g_rout = R(0), g_ts = R(0);
     const R Ts = R(ds->tscale), T0 = R(ds->T0);
@@ -309,8 +315,8 @@ __global__ __launch_bounds__(256) void diff_bwd_kernel(const DiffSet* __restrict
     KY<R> lam = {R(0), R(0), R(0), R(0), R(0)};
     KG<R> gc = {R(0), R(0), R(0)};
     gc.L = gL;
-    const double* cp = b.cp + pix;
-    double* scr = b.scratch + pix;
+    const double* cp = b.cp + q;
+    double* scr = b.scratch + q;
     int jn = int(J) - 1;  // next crossing (from the end) whose seeds are pending
     // The state adjoint is zero after the last crossing, so the sweep starts
     // there.  (Sweeping the remaining steps would only add 0 * J^T terms, and
@@ -377,15 +383,17 @@ __global__ __launch_bounds__(256) void diff_bwd_kernel(const DiffSet* __restrict
 // pixels (a block-stride loop, then a tree in LDS); stage 2, one block sums
 // the per-block partials in block order.
 // The step total is a sum of integers below 2^53, exact in double.
+// Block (x, e) sums slice x of evaluation e's pixels (Pimg per evaluation).
 __global__ __launch_bounds__(256) void diff_reduce_partial_kernel(const double* __restrict__ loss,
                                                                   const double* __restrict__ grad,
                                                                   const std::uint32_t* __restrict__ steps,
-                                                                  std::uint32_t P, double* __restrict__ partial) {
+                                                                  std::uint32_t Pimg, double* __restrict__ partial) {
     constexpr int V = int(kReduceVals);
     __shared__ double sh[V][256];
     const std::uint32_t t = threadIdx.x;
-    const std::uint32_t per = (P + kReduceBlocks - 1u) / kReduceBlocks;
-    const std::uint32_t lo = blockIdx.x * per, hi = min(P, lo + per);
+    const std::uint32_t per = (Pimg + kReduceBlocks - 1u) / kReduceBlocks;
+    const std::uint32_t base = blockIdx.y * Pimg;
+    const std::uint32_t lo = base + blockIdx.x * per, hi = base + min(Pimg, blockIdx.x * per + per);
     double s[V] = {0, 0, 0, 0, 0, 0, 0};
     for (std::uint32_t i = lo + t; i < hi; i += 256u) {
         s[0] += loss[i];
@@ -399,17 +407,20 @@ __global__ __launch_bounds__(256) void diff_reduce_partial_kernel(const double* 
             for (int k = 0; k < V; ++k) sh[k][t] += sh[k][t + w];
         __syncthreads();
     }
-    if (t < kReduceVals) partial[kReduceVals * blockIdx.x + t] = sh[t][0];
+    if (t < kReduceVals) partial[kReduceVals * (blockIdx.y * kReduceBlocks + blockIdx.x) + t] = sh[t][0];
 }
 
-// out[0..kReduceVals): the sums; out[kReduceVals]: the FP64 list's length
-// (0 without a list), so one copy brings back every per-evaluation result.
+// Block e: out[e * kDiffOut + 0..kReduceVals): evaluation e's sums;
+// out[e * kDiffOut + kReduceVals]: the length of its FP64 list (0 without a
+// list), so one copy brings back every per-evaluation result.
 __global__ __launch_bounds__(256) void diff_reduce_final_kernel(const double* __restrict__ partial,
                                                                 const std::uint32_t* __restrict__ count,
                                                                 double* __restrict__ out) {
     constexpr int V = int(kReduceVals);
     __shared__ double sh[V][kReduceBlocks];
     const std::uint32_t t = threadIdx.x;
+    partial += std::size_t(blockIdx.x) * kReduceBlocks * kReduceVals;
+    out += std::size_t(blockIdx.x) * kDiffOut;
     for (int k = 0; k < V; ++k) sh[k][t] = partial[kReduceVals * t + k];
     __syncthreads();
     for (std::uint32_t w = kReduceBlocks / 2u; w > 0; w >>= 1) {
@@ -418,7 +429,7 @@ __global__ __launch_bounds__(256) void diff_reduce_final_kernel(const double* __
         __syncthreads();
     }
     if (t < kReduceVals) out[t] = sh[t][0];
-    if (t == kReduceVals) out[t] = count ? double(*count) : 0.0;
+    if (t == kReduceVals) out[t] = count ? double(count[blockIdx.x]) : 0.0;
 }
 
 }  // namespace
@@ -428,7 +439,7 @@ hipError_t launch_diff(const DiffSet* ds_dev, const DiffBuffers& b, std::uint32_
                        const hipEvent_t* join) {
     // fwd_end: timing event at the end of the forward (on `stream`); join[0..3]:
     // cross-stream joins, created without timing (they only order work)
-    const dim3 block(256), grid((b.P + 255u) / 256u);
+    const dim3 block(256), grid((b.Pimg + 255u) / 256u, b.n_eval);
     if (precision == SING_PREC_FP32) {
         hipLaunchKernelGGL((diff_fwd_kernel<float, 0>), grid, block, 0, stream, ds_dev, b);
         (void)hipEventRecord(fwd_end, stream);
@@ -468,9 +479,9 @@ hipError_t launch_diff(const DiffSet* ds_dev, const DiffBuffers& b, std::uint32_
         }
         (void)hipStreamWaitEvent(stream, join[2], 0);
     }
-    hipLaunchKernelGGL(diff_reduce_partial_kernel, dim3(kReduceBlocks), dim3(256), 0, stream, b.loss, b.grad,
-                       b.steps, b.P, b.partial);
-    hipLaunchKernelGGL(diff_reduce_final_kernel, dim3(1), dim3(kReduceBlocks), 0, stream, b.partial,
+    hipLaunchKernelGGL(diff_reduce_partial_kernel, dim3(kReduceBlocks, b.n_eval), dim3(256), 0, stream, b.loss,
+                       b.grad, b.steps, b.Pimg, b.partial);
+    hipLaunchKernelGGL(diff_reduce_final_kernel, dim3(b.n_eval), dim3(kReduceBlocks), 0, stream, b.partial,
                        precision == SING_PREC_MIXED ? b.count : nullptr, out);
     return hipGetLastError();
 }
@@ -505,17 +516,24 @@ void SingContext::free_diff() {
     d_target = d_weight = nullptr;
     d_out = nullptr;
     h_out = nullptr;
-    diff_P = diff_ncp = diff_C = 0;
+    diff_P = diff_n = diff_ncp = diff_C = 0;
 }
 
-int SingContext::ensure_diff(std::size_t P, std::size_t ncp, std::size_t C) {
-    if (P == diff_P && ncp <= diff_ncp && C == diff_C && d_dset) return SING_OK;
+int SingContext::ensure_diff(std::size_t Pimg, std::size_t n_eval, std::size_t ncp, std::size_t C) {
+    // buffers sized for n evaluations serve any smaller call (the per-pixel
+    // arrays are indexed with the call's own P = n_eval * Pimg)
+    if (Pimg == diff_P && n_eval <= diff_n && ncp <= diff_ncp && C == diff_C && d_dset) {
+        diff.P = std::uint32_t(Pimg * n_eval);
+        diff.n_eval = std::uint32_t(n_eval);
+        return SING_OK;
+    }
     free_diff();
+    const std::size_t P = Pimg * n_eval;
     hipError_t e = hipSuccess;
     auto A = [&](void** p, std::size_t bytes) {
         if (e == hipSuccess) e = hipMalloc(p, bytes);
     };
-    A((void**)&d_dset, sizeof(sing::DiffSet));
+    A((void**)&d_dset, n_eval * sizeof(sing::DiffSet));
     A((void**)&diff.cp, P * ncp * 5 * sizeof(double));
     A((void**)&diff.cross, P * sing::kMaxCross * 7 * sizeof(double));
     A((void**)&diff.meta, P * 4 * sizeof(std::uint32_t));
@@ -527,24 +545,28 @@ int SingContext::ensure_diff(std::size_t P, std::size_t ncp, std::size_t C) {
     A((void**)&diff.grad, P * 5 * sizeof(double));
     A((void**)&diff.scratch, P * (C + 1) * 5 * sizeof(double));
     A((void**)&diff.list, P * sizeof(std::uint32_t));
-    A((void**)&diff.count, sizeof(std::uint32_t));
+    A((void**)&diff.count, n_eval * sizeof(std::uint32_t));
     A((void**)&diff.list2, P * sizeof(std::uint32_t));
-    A((void**)&diff.count2, sizeof(std::uint32_t));
+    A((void**)&diff.count2, n_eval * sizeof(std::uint32_t));
     A((void**)&diff.steps, P * sizeof(std::uint32_t));
-    A((void**)&diff.partial, sing::kReduceBlocks * sing::kReduceVals * sizeof(double));
-    A((void**)&d_target, P * 3 * sizeof(float));
-    A((void**)&d_weight, P * sizeof(float));
-    A((void**)&d_out, sing::kDiffOut * sizeof(double));
-    // pinned, so the per-evaluation result copy is an ordinary stream operation
-    if (e == hipSuccess) e = hipHostMalloc((void**)&h_out, sing::kDiffOut * sizeof(double), hipHostMallocDefault);
+    A((void**)&diff.partial, n_eval * sing::kReduceBlocks * sing::kReduceVals * sizeof(double));
+    A((void**)&d_target, Pimg * 3 * sizeof(float));
+    A((void**)&d_weight, Pimg * sizeof(float));
+    A((void**)&d_out, n_eval * sing::kDiffOut * sizeof(double));
+    // pinned, so the per-call result copy is an ordinary stream operation
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&h_out, n_eval * sing::kDiffOut * sizeof(double), hipHostMallocDefault);
     if (e != hipSuccess) {
         free_diff();
         return fail(SING_ERR_NOMEM, "hipMalloc of differentiable-render buffers failed: %s", hipGetErrorString(e));
     }
     diff.P = std::uint32_t(P);
+    diff.Pimg = std::uint32_t(Pimg);
+    diff.n_eval = std::uint32_t(n_eval);
     diff.ncp = std::uint32_t(ncp);
     diff.C = std::uint32_t(C);
-    diff_P = P;
+    diff_P = Pimg;
+    diff_n = n_eval;
     diff_ncp = ncp;
     diff_C = C;
     return SING_OK;
@@ -552,52 +574,19 @@ int SingContext::ensure_diff(std::size_t P, std::size_t ncp, std::size_t C) {
 
 namespace {
 
-int diff_run(SingContext* c, const SingUniforms* base, const SingDiffParams* p, const SingDiffCamera* cam,
-             const SingDiffOpts* opts_in, const float* target, const float* weights, bool backward, double* loss,
-             double* grad, float* hdr_out, SingDiffStats* stats) {
-    if (!c) return SING_ERR_ARG;
-    if (!base || !p || !cam) return c->fail(SING_ERR_ARG, "sing_diff: null argument");
-    if (base->width != c->width || base->height != c->height)
-        return c->fail(SING_ERR_ARG, "sing_diff: base size %ux%u != context %ux%u", base->width, base->height,
-                       c->width, c->height);
-    SingDiffOpts o{};
-    if (opts_in) o = *opts_in;
-    if (o.precision > SING_PREC_MIXED) return c->fail(SING_ERR_ARG, "sing_diff: unknown precision %u", o.precision);
-    const double w = o.soft_edge > 0.0 ? o.soft_edge : 1.0;
-    const std::size_t C = o.checkpoint_every ? o.checkpoint_every : 64u;
-    if (C > 4096) return c->fail(SING_ERR_ARG, "sing_diff: checkpoint_every too large");
-    if (!(p->spin >= 0.0 && p->spin < 1.0)) return c->fail(SING_ERR_ARG, "sing_diff: spin must be in [0, 1)");
-    if (!(p->t_scale > 0.0) || !(cam->distance > 0.0))
-        return c->fail(SING_ERR_ARG, "sing_diff: need t_scale > 0, distance > 0");
-    if (o.r_in_isco > 1u) return c->fail(SING_ERR_ARG, "sing_diff: r_in_isco must be 0 or 1");
-    if (backward && !target) return c->fail(SING_ERR_ARG, "sing_diff: null target");
-    const std::size_t P = std::size_t(c->width) * c->height;
-    const std::size_t ncp = base->max_steps / C + 1;
-    hipError_t e = hipSetDevice(c->device);
-    if (e != hipSuccess) return c->hip_fail(e, "hipSetDevice");
-    int rc = c->ensure_diff(P, ncp, C);
-    if (rc != SING_OK) return rc;
-    if ((rc = c->ensure_events(6)) != SING_OK) return rc;
-    if ((rc = c->ensure_join_events(4)) != SING_OK) return rc;
-
-    // r_in: given, or the prograde ISCO of the spin with its derivative
-    double r_in = p->r_in, drin_da = 0.0;
-    if (o.r_in_isco) {
-        const sing::Dual<1, double> ri = sing::isco_prograde(sing::Dual<1, double>::var(p->spin, 0));
-        r_in = ri.v;
-        drin_da = ri.d[0];
-    }
-    if (!(p->r_out > r_in)) return c->fail(SING_ERR_ARG, "sing_diff: need r_out > r_in (r_in = %g)", r_in);
-
-    // scene in all precisions (host, double duals for the tangents)
+// The DiffSet of one parameter point (host): the scene in double, its
+// (a/M, inclination) tangents as Dual<2>, and both rounded to float.
+// r_in: given, or the prograde ISCO of the spin (drin_da its derivative).
+sing::DiffSet make_diff_set(const SingUniforms& base, const SingDiffParams& p, const SingDiffCamera& cam, double w,
+                            double r_in) {
     sing::DiffSet ds{};
     using D2 = sing::Dual<2, double>;
-    const sing::DiffScene<D2> td = sing::make_diff_scene<D2>(*base, D2::var(p->spin, 0), D2::var(p->incl, 1),
-                                                             D2(r_in), D2(p->r_out), D2(p->t_scale),
-                                                             cam->distance, cam->azimuth, cam->fov_y_deg, w);
+    const sing::DiffScene<D2> td = sing::make_diff_scene<D2>(base, D2::var(p.spin, 0), D2::var(p.incl, 1), D2(r_in),
+                                                             D2(p.r_out), D2(p.t_scale), cam.distance, cam.azimuth,
+                                                             cam.fov_y_deg, w);
     ds.td = td;
-    ds.sd = sing::make_diff_scene<double>(*base, p->spin, p->incl, r_in, p->r_out, p->t_scale, cam->distance,
-                                          cam->azimuth, cam->fov_y_deg, w);
+    ds.sd = sing::make_diff_scene<double>(base, p.spin, p.incl, r_in, p.r_out, p.t_scale, cam.distance, cam.azimuth,
+                                          cam.fov_y_deg, w);
     // float copies are the double values rounded (same discrete model)
     auto to_f = [](const sing::DiffScene<double>& d) {
         sing::DiffScene<float> f{};
@@ -628,24 +617,78 @@ int diff_run(SingContext* c, const SingUniforms* base, const SingDiffParams* p, 
         }
         ds.tf.width = td.width; ds.tf.height = td.height; ds.tf.flags = td.flags; ds.tf.max_steps = td.max_steps;
     }
-    ds.tscale = p->t_scale;
-    ds.T0 = base->disc_peak_T;
+    ds.tscale = p.t_scale;
+    ds.T0 = base.disc_peak_T;
+    return ds;
+}
 
-    if ((e = hipMemcpyAsync(c->d_dset, &ds, sizeof(ds), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+// n parameter points (cameras: one per point, or n_cams == 1 for all) of one
+// image size, evaluated in one set of launches.
+int diff_run(SingContext* c, const SingUniforms* base, std::uint32_t n, const SingDiffParams* params,
+             const SingDiffCamera* cams, std::uint32_t n_cams, const SingDiffOpts* opts_in, const float* target,
+             const float* weights, bool backward, double* loss, double* grad, float* hdr_out, SingDiffStats* stats,
+             const char* who) {
+    if (!c) return SING_ERR_ARG;
+    if (!base || !params || !cams) return c->fail(SING_ERR_ARG, "%s: null argument", who);
+    if (n < 1 || n > SING_DIFF_MAX_BATCH)
+        return c->fail(SING_ERR_ARG, "%s: evaluations must be in [1, %u]", who, unsigned(SING_DIFF_MAX_BATCH));
+    if (n_cams != 1 && n_cams != n) return c->fail(SING_ERR_ARG, "%s: give one camera or one per evaluation", who);
+    if (base->width != c->width || base->height != c->height)
+        return c->fail(SING_ERR_ARG, "%s: base size %ux%u != context %ux%u", who, base->width, base->height,
+                       c->width, c->height);
+    SingDiffOpts o{};
+    if (opts_in) o = *opts_in;
+    if (o.precision > SING_PREC_MIXED) return c->fail(SING_ERR_ARG, "%s: unknown precision %u", who, o.precision);
+    const double w = o.soft_edge > 0.0 ? o.soft_edge : 1.0;
+    const std::size_t C = o.checkpoint_every ? o.checkpoint_every : 64u;
+    if (C > 4096) return c->fail(SING_ERR_ARG, "%s: checkpoint_every too large", who);
+    if (o.r_in_isco > 1u) return c->fail(SING_ERR_ARG, "%s: r_in_isco must be 0 or 1", who);
+    if (backward && !target) return c->fail(SING_ERR_ARG, "%s: null target", who);
+    const std::size_t Pimg = std::size_t(c->width) * c->height;
+    if (Pimg * n >= (std::size_t(1) << 31)) return c->fail(SING_ERR_ARG, "%s: too many pixels in one call", who);
+    std::vector<sing::DiffSet> sets(n);
+    std::vector<double> drin_da(n, 0.0);
+    for (std::uint32_t k = 0; k < n; ++k) {
+        const SingDiffParams& p = params[k];
+        const SingDiffCamera& cam = cams[n_cams == 1 ? 0 : k];
+        if (!(p.spin >= 0.0 && p.spin < 1.0)) return c->fail(SING_ERR_ARG, "%s: spin must be in [0, 1)", who);
+        if (!(p.t_scale > 0.0) || !(cam.distance > 0.0))
+            return c->fail(SING_ERR_ARG, "%s: need t_scale > 0, distance > 0", who);
+        double r_in = p.r_in;
+        if (o.r_in_isco) {
+            const sing::Dual<1, double> ri = sing::isco_prograde(sing::Dual<1, double>::var(p.spin, 0));
+            r_in = ri.v;
+            drin_da[k] = ri.d[0];
+        }
+        if (!(p.r_out > r_in)) return c->fail(SING_ERR_ARG, "%s: need r_out > r_in (r_in = %g)", who, r_in);
+        sets[k] = make_diff_set(*base, p, cam, w, r_in);
+    }
+    const std::size_t ncp = base->max_steps / C + 1;
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return c->hip_fail(e, "hipSetDevice");
+    int rc = c->ensure_diff(Pimg, n, ncp, C);
+    if (rc != SING_OK) return rc;
+    if ((rc = c->ensure_events(6)) != SING_OK) return rc;
+    if ((rc = c->ensure_join_events(4)) != SING_OK) return rc;
+    const std::size_t P = Pimg * n;
+
+    if ((e = hipMemcpyAsync(c->d_dset, sets.data(), n * sizeof(sing::DiffSet), hipMemcpyHostToDevice, c->stream)) !=
+        hipSuccess)
         return c->hip_fail(e, "hipMemcpyAsync(scene)");
     if (target) {
-        if ((e = hipMemcpyAsync(c->d_target, target, P * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream)) !=
+        if ((e = hipMemcpyAsync(c->d_target, target, Pimg * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream)) !=
             hipSuccess)
             return c->hip_fail(e, "hipMemcpyAsync(target)");
-    } else if ((e = hipMemsetAsync(c->d_target, 0, P * 3 * sizeof(float), c->stream)) != hipSuccess) {
+    } else if ((e = hipMemsetAsync(c->d_target, 0, Pimg * 3 * sizeof(float), c->stream)) != hipSuccess) {
         return c->hip_fail(e, "hipMemsetAsync(target)");
     }
     if (weights &&
-        (e = hipMemcpyAsync(c->d_weight, weights, P * sizeof(float), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+        (e = hipMemcpyAsync(c->d_weight, weights, Pimg * sizeof(float), hipMemcpyHostToDevice, c->stream)) !=
+            hipSuccess)
         return c->hip_fail(e, "hipMemcpyAsync(weights)");
-    if ((e = hipMemsetAsync(c->diff.count2, 0, sizeof(std::uint32_t), c->stream)) != hipSuccess)
+    if ((e = hipMemsetAsync(c->diff.count2, 0, n * sizeof(std::uint32_t), c->stream)) != hipSuccess)
         return c->hip_fail(e, "hipMemsetAsync(count2)");
-    if ((e = hipMemsetAsync(c->diff.count, 0, sizeof(std::uint32_t), c->stream)) != hipSuccess)
+    if ((e = hipMemsetAsync(c->diff.count, 0, n * sizeof(std::uint32_t), c->stream)) != hipSuccess)
         return c->hip_fail(e, "hipMemsetAsync(count)");
     if (!backward && (e = hipMemsetAsync(c->diff.grad, 0, P * 5 * sizeof(double), c->stream)) != hipSuccess)
         return c->hip_fail(e, "hipMemsetAsync(grad)");
@@ -658,9 +701,9 @@ int diff_run(SingContext* c, const SingUniforms* base, const SingDiffParams* p, 
         hipSuccess)
         return c->hip_fail(e, "differentiable render launch");
     (void)hipEventRecord(c->events[5], c->stream);
-    // loss, gradient, step total and FP64 list length in one copy to pinned
-    // memory (the step total is reduced on the device, not copied per pixel)
-    if ((e = hipMemcpyAsync(c->h_out, c->d_out, sing::kDiffOut * sizeof(double), hipMemcpyDeviceToHost,
+    // losses, gradients, step totals and the FP64 list length in one copy to
+    // pinned memory (the step totals are reduced on the device)
+    if ((e = hipMemcpyAsync(c->h_out, c->d_out, n * sing::kDiffOut * sizeof(double), hipMemcpyDeviceToHost,
                             c->stream)) != hipSuccess)
         return c->hip_fail(e, "hipMemcpyAsync(result)");
     if (hdr_out &&
@@ -668,11 +711,17 @@ int diff_run(SingContext* c, const SingUniforms* base, const SingDiffParams* p, 
             hipSuccess)
         return c->hip_fail(e, "hipMemcpyAsync(hdr)");
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return c->hip_fail(e, "hipStreamSynchronize");
-    const double* out = c->h_out;
-    if (loss) *loss = out[0];
-    if (grad) {
-        for (int k = 0; k < 5; ++k) grad[k] = out[1 + k];
-        grad[0] += grad[2] * drin_da;
+    std::uint64_t steps = 0, listed64 = 0;
+    for (std::uint32_t k = 0; k < n; ++k) {
+        const double* out = c->h_out + std::size_t(k) * sing::kDiffOut;
+        listed64 += std::uint64_t(out[sing::kReduceVals]);
+        if (loss) loss[k] = out[0];
+        if (grad) {
+            double* g = grad + 5u * k;
+            for (int j = 0; j < 5; ++j) g[j] = out[1 + j];
+            g[0] += g[2] * drin_da[k];
+        }
+        steps += std::uint64_t(out[6]);
     }
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
@@ -682,8 +731,8 @@ int diff_run(SingContext* c, const SingUniforms* base, const SingDiffParams* p, 
         stats->forward_ms = a;
         stats->backward_ms = bms;
         stats->rays = P;
-        stats->steps = std::uint64_t(out[6]);
-        stats->fp64_rays = (o.precision == SING_PREC_FP64) ? P : std::uint64_t(out[sing::kReduceVals]);
+        stats->steps = steps;
+        stats->fp64_rays = (o.precision == SING_PREC_FP64) ? P : listed64;
     }
     return SING_OK;
 }
@@ -694,20 +743,30 @@ extern "C" {
 
 int sing_diff_forward(SingContext* ctx, const SingUniforms* base, const SingDiffParams* p, const SingDiffCamera* cam,
                       const SingDiffOpts* opts, float* hdr_out, SingDiffStats* stats) {
-    return diff_run(ctx, base, p, cam, opts, nullptr, nullptr, false, nullptr, nullptr, hdr_out, stats);
+    return diff_run(ctx, base, 1, p, cam, 1, opts, nullptr, nullptr, false, nullptr, nullptr, hdr_out, stats,
+                    "sing_diff_forward");
 }
 
 int sing_diff_loss_grad(SingContext* ctx, const SingUniforms* base, const SingDiffParams* p,
                         const SingDiffCamera* cam, const SingDiffOpts* opts, const float* target,
                         const float* weights, double* loss, double* grad, float* hdr_out, SingDiffStats* stats) {
-    return diff_run(ctx, base, p, cam, opts, target, weights, true, loss, grad, hdr_out, stats);
+    return diff_run(ctx, base, 1, p, cam, 1, opts, target, weights, true, loss, grad, hdr_out, stats,
+                    "sing_diff_loss_grad");
+}
+
+int sing_diff_loss_grad_batch(SingContext* ctx, const SingUniforms* base, uint32_t n, const SingDiffParams* params,
+                              const SingDiffCamera* cams, uint32_t n_cams, const SingDiffOpts* opts,
+                              const float* target, const float* weights, double* loss, double* grad, float* hdr_out,
+                              SingDiffStats* stats) {
+    return diff_run(ctx, base, n, params, cams, n_cams, opts, target, weights, true, loss, grad, hdr_out, stats,
+                    "sing_diff_loss_grad_batch");
 }
 
 int sing_diff_capture_pixels(SingContext* c, double* grad_out, uint32_t* meta_out) {
     if (!c) return SING_ERR_ARG;
     if (!c->diff_P || !c->d_dset) return c->fail(SING_ERR_STATE, "sing_diff_capture_pixels: no differentiable frame");
     hipError_t e = hipSetDevice(c->device);
-    const std::size_t P = c->diff_P;
+    const std::size_t P = c->diff_P;  // one image: the first evaluation of the last call
     if (e == hipSuccess && grad_out) e = hipMemcpy(grad_out, c->diff.grad, P * 5 * sizeof(double), hipMemcpyDeviceToHost);
     if (e == hipSuccess && meta_out) e = hipMemcpy(meta_out, c->diff.meta, P * 4 * sizeof(std::uint32_t), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return c->hip_fail(e, "sing_diff_capture_pixels");

@@ -19,9 +19,13 @@ struct DiffSet {
     double tscale;                  // peak-temperature scale parameter
     double T0;                      // unscaled peak temperature (K)
 };
+// (device memory holds one DiffSet per evaluation of a call)
 
-// Per-pixel device buffers of one differentiable frame.
-// Blocks of the first stage of the loss/gradient reduction.
+// Per-pixel device buffers of the differentiable frames of one call.  A call
+// evaluates n_eval parameter points of one image size; their pixels are laid
+// out one image after another (q = evaluation * Pimg + pixel), and every
+// per-pixel array below is indexed by q with P = n_eval * Pimg entries.
+// Blocks of the first stage of the loss/gradient reduction (per evaluation).
 constexpr std::uint32_t kReduceBlocks = 256;
 // reduced per evaluation: loss, the 5 gradient entries, the RK4 step total
 constexpr std::uint32_t kReduceVals = 7;
@@ -39,17 +43,21 @@ struct DiffBuffers {
     double* loss;           // [P]
     double* grad;           // [P][5]
     double* scratch;        // [C+1][5][P] recomputed segment states
-    const float* target;    // [P][3]
-    const float* weight;    // [P] or null
-    std::uint32_t* list;    // pixels of the FP64 pass (classified hard before the march)
-    std::uint32_t* count;   // length of list
-    std::uint32_t* list2;   // pixels handed to FP64 after the FP32 march (grazing, near-hole emission)
-    std::uint32_t* count2;  // length of list2
+    const float* target;    // [Pimg][3], shared by the evaluations
+    const float* weight;    // [Pimg] or null
+    std::uint32_t* list;    // [n_eval][Pimg] pixels of the FP64 pass (classified hard before the march)
+    std::uint32_t* count;   // [n_eval] lengths of list
+    std::uint32_t* list2;   // [n_eval][Pimg] pixels handed to FP64 after the FP32 march (grazing, near-hole emission)
+    std::uint32_t* count2;  // [n_eval] lengths of list2
     std::uint32_t* steps;   // [P] RK4 steps of the forward
-    double* partial;        // [kReduceBlocks][kReduceVals] per-block sums of loss, gradient, steps
+    double* partial;        // [n_eval][kReduceBlocks][kReduceVals] per-block sums of loss, gradient, steps
     std::uint32_t P, ncp, C;
+    std::uint32_t Pimg;     // pixels of one image (P = n_eval * Pimg)
+    std::uint32_t n_eval;
 };
 
+// out: n_eval x kDiffOut doubles (per evaluation: loss, gradient, step total,
+// then the length of its FP64 list).
 hipError_t launch_diff(const DiffSet* ds_dev, const DiffBuffers& b, std::uint32_t precision, bool backward,
                        double* out, hipStream_t stream, hipStream_t stream2, hipEvent_t fwd_end,
                        const hipEvent_t* join);

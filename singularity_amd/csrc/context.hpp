@@ -54,13 +54,13 @@ struct SingContext {
     std::uint32_t outputs_valid = 0;  // SING_OUT_* of the last frame
     std::string err;
     // differentiable render
-    sing::DiffSet* d_dset = nullptr;
+    sing::DiffSet* d_dset = nullptr;  // one per evaluation of a call
     sing::DiffBuffers diff{};
     float* d_target = nullptr;
     float* d_weight = nullptr;
-    double* d_out = nullptr;  // the adjoint's reduced results (sing::kDiffOut doubles)
+    double* d_out = nullptr;  // the adjoint's reduced results (sing::kDiffOut doubles per evaluation)
     double* h_out = nullptr;  // pinned host copy of d_out
-    std::size_t diff_P = 0, diff_ncp = 0, diff_C = 0;
+    std::size_t diff_P = 0, diff_n = 0, diff_ncp = 0, diff_C = 0;  // diff_P: pixels of one image
     // batched render
     sing::BatchScene* d_bscenes = nullptr;
     std::uint64_t* d_brec_bases = nullptr;
@@ -89,7 +89,7 @@ struct SingContext {
     int ensure_join_events(std::size_t n);
     std::uint32_t chunk_samples(std::uint32_t spp) const;
     sing::LaunchShape launch_shape(const SingRenderOpts& o) const;
-    int ensure_diff(std::size_t P, std::size_t ncp, std::size_t C);
+    int ensure_diff(std::size_t Pimg, std::size_t n_eval, std::size_t ncp, std::size_t C);
     void free_diff();
     void free_batch();
     void drop_batch_graph();

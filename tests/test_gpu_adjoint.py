@@ -240,6 +240,52 @@ def test_argument_errors(be, base, target):
         sa.diff_forward(be, base, sa.DiffParams.from_array([0.5, 1.0, 20.0, 10.0, 1.0]), cam(), opts())
 
 
+@pytest.mark.parametrize("prec", [sa.PREC_FP32, sa.PREC_FP64, sa.PREC_MIXED])
+def test_batch_equals_separate_evaluations(be, base, target, prec):
+    """sing_diff_loss_grad_batch over several parameter points (one camera, then one
+    camera per point) returns, bit for bit, the losses, gradients and images of
+    separate sing_diff_loss_grad calls; afterwards the per-pixel capture holds the
+    first point's."""
+    xs = [START, TARGET, 0.5 * (START + TARGET), START + np.array([0.1, 0.05, 0.0, 1.0, 0.05])]
+    cams = [cam(), sa.DiffCamera(45.0, 0.3, 50.0), sa.DiffCamera(55.0, -0.2, 45.0), cam()]
+    o = opts(prec)
+    print(f"batch of {len(xs)} at {W}x{H} prec={prec}", flush=True)
+    for per_point in (False, True):
+        cs = cams if per_point else cam()
+        loss, grad, hdr, st = sa.diff_loss_grad_batch(be, base, xs, cs, target, None, o, want_hdr=True)
+        assert st.rays == len(xs) * W * H
+        for k, x in enumerate(xs):
+            l1, g1, h1, _ = sa.diff_loss_grad(be, base, sa.DiffParams.from_array(x), cams[k] if per_point else cam(),
+                                              target, None, o)
+            assert loss[k] == l1 and np.array_equal(grad[k], g1), (k, per_point, loss[k], l1, grad[k], g1)
+            assert np.array_equal(hdr[k], h1)
+    sa.diff_loss_grad(be, base, sa.DiffParams.from_array(xs[0]), cam(), target, None, o)
+    g_single, m_single = sa.diff_capture_pixels(be)
+    sa.diff_loss_grad_batch(be, base, xs, cam(), target, None, o)
+    g_batch, m_batch = sa.diff_capture_pixels(be)
+    assert np.array_equal(g_batch, g_single) and np.array_equal(m_batch, m_single)
+
+
+def test_batch_argument_errors(be, base, target):
+    with pytest.raises(ValueError):
+        sa.diff_loss_grad_batch(be, base, [], cam(), target)
+    with pytest.raises(ValueError):
+        sa.diff_loss_grad_batch(be, base, [START] * 33, cam(), target)
+    with pytest.raises(ValueError):
+        sa.diff_loss_grad_batch(be, base, [START] * 3, [cam(), cam()], target)
+    L = sa.lib()
+    arr = (sa.DiffParams * 1)(sa.DiffParams.from_array(START))
+    carr = (sa.DiffCamera * 1)(cam())
+    t = np.ascontiguousarray(target, np.float32)
+    for n, nc in ((0, 1), (33, 1), (1, 2)):
+        rc = L.sing_diff_loss_grad_batch(be._ctx, C.byref(base), n, C.addressof(arr), C.addressof(carr), nc,
+                                         C.byref(opts()), t.ctypes.data, None, None, None, None, None)
+        assert rc == 1, (n, nc, rc)
+    bad = [START, np.array([1.2, 1.0, 6.0, 20.0, 1.0])]
+    with pytest.raises(sa.SingError):
+        sa.diff_loss_grad_batch(be, base, bad, cam(), target)
+
+
 # ---- the 512 x 512 inverse-render workload ---------------------------------
 WL = 512
 
@@ -314,6 +360,19 @@ def test_mixed_gradient_matches_central_differences_at_512(be512, workload):
         fd[i] = (lp - lm) / (2 * h)
     print("adjoint", g[free], "\nFD     ", fd[free], "\nrel", np.abs(g[free] - fd[free]) / np.abs(fd[free]), flush=True)
     assert close_entries(g[free], fd[free]), (g[free], fd[free])
+
+
+def test_batch_at_512_equals_separate_evaluations(be512, workload):
+    """The central-difference stencil of a/M and r_out at the start point as one batch
+    (mixed precision, r_in = ISCO(a)): the same losses and gradients as separate calls."""
+    base, tgt = workload
+    o = opts(sa.PREC_MIXED, isco=1)
+    xs = [START + s * np.eye(5)[j] * h for j, h in ((0, 1e-3), (3, 1e-2)) for s in (-1.0, 1.0)]
+    print(f"batch of {len(xs)} at {WL}x{WL} mixed", flush=True)
+    loss, grad, _, st = sa.diff_loss_grad_batch(be512, base, xs, cam(), tgt, None, o)
+    for k, x in enumerate(xs):
+        l1, g1, _, _ = sa.diff_loss_grad(be512, base, sa.DiffParams.from_array(x), cam(), tgt, None, o)
+        assert loss[k] == l1 and np.array_equal(grad[k], g1), (k, loss[k], l1)
 
 
 def test_adam_recovers_parameters(be512, workload):

@@ -131,7 +131,8 @@ typedef struct SingRenderOpts {
     float adapt_tol;      /* > 0: error-controlled RK4 (step doubling) with this per-step tolerance */
     float band_width;     /* mixed precision: photon-orbit margin below which a ray is traced in FP64 (0 = 0.03) */
     float peak_T_scale;   /* multiplies disc_peak_T; 0 is read as 1 */
-    uint32_t step_batch;  /* RK4 steps between lane-refill checks (0 = tuned default) */
+    uint32_t step_batch;  /* RK4 steps between lane-refill checks (0 = tuned default, 32); the mixed
+                             form's FP64 list pass uses a quarter of it (at least 8) */
     uint32_t waves_per_cu;/* persistent grid size in waves per CU (0 = tuned default) */
     uint32_t sampler;     /* SING_SAMPLER_* */
     uint32_t refine;      /* SING_REFINE_* mask; Kerr with the fixed step (adapt_tol = 0) only */

@@ -365,28 +365,29 @@ hipError_t launch_batch_trace(const BatchLaunch& L, const BatchGroup& g, std::ui
                               bool starfield_any, hipStream_t stream) {
     const dim3 block(256);
     const std::uint32_t sb = shape.step_batch ? shape.step_batch : 32u;
+    const std::uint32_t sb_list = std::max(8u, (sb / 4u) & ~7u);  // the FP64 list pass, as in trace.hip
     const std::uint32_t n_kerr = g.count - g.n_schw, k0 = g.first + g.n_schw;
-#define SING_BTRACE(P, G)                                                                                       \
+#define SING_BTRACE(P, G, SB)                                                                                   \
     do {                                                                                                        \
         if (g.n_schw)                                                                                           \
             hipLaunchKernelGGL((trace_batch_kernel<0, P>), dim3(G), block, 0, stream, L.scenes, L, g.first,       \
-                               g.n_schw, sb, sample_div, claim);                                                \
+                               g.n_schw, SB, sample_div, claim);                                                \
         if (n_kerr && adaptive)                                                                                 \
-            hipLaunchKernelGGL((trace_batch_kernel<2, P>), dim3(G), block, 0, stream, L.scenes, L, k0, n_kerr, sb, \
+            hipLaunchKernelGGL((trace_batch_kernel<2, P>), dim3(G), block, 0, stream, L.scenes, L, k0, n_kerr, SB, \
                                sample_div, claim);                                                              \
         else if (n_kerr)                                                                                        \
-            hipLaunchKernelGGL((trace_batch_kernel<1, P>), dim3(G), block, 0, stream, L.scenes, L, k0, n_kerr, sb, \
+            hipLaunchKernelGGL((trace_batch_kernel<1, P>), dim3(G), block, 0, stream, L.scenes, L, k0, n_kerr, SB, \
                                sample_div, claim);                                                              \
     } while (0)
     if (mode == SING_PREC_FP32) {
-        SING_BTRACE(0, shape.blocks);
+        SING_BTRACE(0, shape.blocks, sb);
     } else if (mode == SING_PREC_FP64) {
-        SING_BTRACE(2, shape.blocks64);
+        SING_BTRACE(2, shape.blocks64, sb);
     } else {
-        SING_BTRACE(1, shape.blocks);
+        SING_BTRACE(1, shape.blocks, sb);
         if (starfield_any && sample_div <= 1u)
             hipLaunchKernelGGL(audit_batch_kernel, dim3(shape.blocks), block, 0, stream, L, g.first, g.count, g.rays);
-        SING_BTRACE(3, shape.blocks64);
+        SING_BTRACE(3, shape.blocks64, sb_list);
     }
 #undef SING_BTRACE
     return hipGetLastError();

@@ -340,18 +340,23 @@ hipError_t launch_trace_chunk(const SceneConst<float>& sf, const SceneConst<doub
     };
     const dim3 block(256);
     const std::uint32_t sb = shape.step_batch ? shape.step_batch : 32u;
+    // The mixed form's FP64 list mixes short captured rays (~70 steps) with
+    // long near-polar ones (~600): refilling every quarter of the step batch
+    // (at least 8, a multiple of 8) wastes fewer idle lane-steps there (the
+    // list's pass 3-7 % faster; the full-image FP64 pass is best at 32).
+    const std::uint32_t sb_list = std::max(8u, (sb / 4u) & ~7u);
     const bool kerr = sf.metric == SING_METRIC_KERR;
     const bool adapt = kerr && sf.tol > 0.0f;
-#define SING_TRACE(P, G)                                                                                  \
+#define SING_TRACE(P, G, SB)                                                                              \
     do {                                                                                                  \
         if (adapt)                                                                                        \
-            hipLaunchKernelGGL((trace_kernel<2, P>), dim3(G), block, 0, stream, sf, sd, halton, ch, ctl, sb); \
+            hipLaunchKernelGGL((trace_kernel<2, P>), dim3(G), block, 0, stream, sf, sd, halton, ch, ctl, SB); \
         else if (kerr && sf.refine)                                                                       \
-            hipLaunchKernelGGL((trace_kernel<3, P>), dim3(G), block, 0, stream, sf, sd, halton, ch, ctl, sb); \
+            hipLaunchKernelGGL((trace_kernel<3, P>), dim3(G), block, 0, stream, sf, sd, halton, ch, ctl, SB); \
         else if (kerr)                                                                                    \
-            hipLaunchKernelGGL((trace_kernel<1, P>), dim3(G), block, 0, stream, sf, sd, halton, ch, ctl, sb); \
+            hipLaunchKernelGGL((trace_kernel<1, P>), dim3(G), block, 0, stream, sf, sd, halton, ch, ctl, SB); \
         else                                                                                              \
-            hipLaunchKernelGGL((trace_kernel<0, P>), dim3(G), block, 0, stream, sf, sd, halton, ch, ctl, sb); \
+            hipLaunchKernelGGL((trace_kernel<0, P>), dim3(G), block, 0, stream, sf, sd, halton, ch, ctl, SB); \
     } while (0)
     const std::uint64_t n_items = std::uint64_t(ch.S) * ch.n_tiles * 64u;
     const dim3 setup_grid(std::uint32_t(std::min<std::uint64_t>((n_items + 255u) / 256u, shape.blocks * 8ull)));
@@ -365,7 +370,7 @@ hipError_t launch_trace_chunk(const SceneConst<float>& sf, const SceneConst<doub
     if (mode == SING_PREC_FP32) {
         SING_SETUP(0);
         mark(0);
-        SING_TRACE(0, shape.blocks);
+        SING_TRACE(0, shape.blocks, sb);
         mark(1);
         mark(2);
         mark(3);
@@ -373,17 +378,17 @@ hipError_t launch_trace_chunk(const SceneConst<float>& sf, const SceneConst<doub
         mark(0);
         mark(1);
         mark(2);
-        SING_TRACE(2, shape.blocks64);
+        SING_TRACE(2, shape.blocks64, sb);
         mark(3);
     } else {
         SING_SETUP(1);
         mark(0);
-        SING_TRACE(1, shape.blocks);
+        SING_TRACE(1, shape.blocks, sb);
         mark(1);
         if (sf.flags & SING_FLAG_STARFIELD_ON)
             hipLaunchKernelGGL(audit_kernel, dim3(shape.blocks), block, 0, stream, sf, ch, ctl);
         mark(2);
-        SING_TRACE(3, shape.blocks64);
+        SING_TRACE(3, shape.blocks64, sb_list);
         mark(3);
     }
 #undef SING_TRACE

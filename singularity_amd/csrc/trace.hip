@@ -36,9 +36,16 @@ using namespace core;
 // 2: FP64 trace of every ray; 3: FP64 trace of the deferred list.
 // The fixed-step Kerr FP64 passes are held to 168 VGPRs, 3 waves per SIMD (the
 // compiler's own choice, 170, leaves room for 2), without spills; METRIC 3
-// keeps the compiler's choice (held to 168 it spills).
+// keeps the compiler's choice (held to 168 it spills).  The error-controlled
+// FP32 passes are held to 128 VGPRs, 4 waves per SIMD (left alone the mixed
+// form's pass takes 133, 3 waves; held, it spills one 16-byte value, outside
+// the step loop): that pass 6.6 % faster.  The error-controlled FP64 passes
+// (220 VGPRs, 2 waves) were slower held to 168 (3 waves; 176 bytes spilled,
+// reloaded in the step loop).
 template <int METRIC, int PASS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((METRIC == 1 && PASS >= 2) ? 3 : 1)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((METRIC == 1 && PASS >= 2) ? 3
+                                                                     : (METRIC == 2 && PASS < 2) ? 4
+                                                                                                 : 1)))
 void trace_kernel(SceneConst<float> sf, SceneConst<double> sd,
                                                     const float2* __restrict__ halton, Chunk ch,
                                                     QueueCtl* __restrict__ ctl, std::uint32_t step_batch) {

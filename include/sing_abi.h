@@ -131,8 +131,9 @@ typedef struct SingRenderOpts {
     float adapt_tol;      /* > 0: error-controlled RK4 (step doubling) with this per-step tolerance */
     float band_width;     /* mixed precision: photon-orbit margin below which a ray is traced in FP64 (0 = 0.03) */
     float peak_T_scale;   /* multiplies disc_peak_T; 0 is read as 1 */
-    uint32_t step_batch;  /* RK4 steps between lane-refill checks (0 = tuned default, 32); the mixed
-                             form's FP64 list pass uses a quarter of it (at least 8) */
+    uint32_t step_batch;  /* RK4 steps between lane-refill checks (0 = tuned default, 32; rounded up
+                             to a multiple of 8); the mixed form's FP64 list pass uses a quarter of it
+                             (at least 8, a multiple of 8) */
     uint32_t waves_per_cu;/* persistent grid size in waves per CU (0 = tuned default) */
     uint32_t sampler;     /* SING_SAMPLER_* */
     uint32_t refine;      /* SING_REFINE_* mask; Kerr with the fixed step (adapt_tol = 0) only */
@@ -385,7 +386,8 @@ typedef struct SingBatchOpts {
     uint32_t autotune;      /* 1: time candidates for waves per CU, step batch, rays per refill and queue
                                chunk on a 1/16 sample of this batch and keep the fastest for this (metric,
                                precision, stepping) in the context */
-    uint32_t step_batch;    /* RK4 steps between lane refills (0 = tuned or 32) */
+    uint32_t step_batch;    /* RK4 steps between lane refills (0 = tuned or 32; rounded up to a
+                               multiple of 8) */
     uint32_t waves_per_cu;  /* persistent waves per CU (0 = tuned or 24) */
     float adapt_tol;        /* > 0: error-controlled stepping (applies to the Kerr scenes) */
     uint32_t turb_seed;     /* seeded disc-band phases, as SingRenderOpts */

@@ -669,8 +669,8 @@ extern "C" int sing_render_batch(SingContext* c, const SingBatchItem* items, uin
     if (!tu.waves_per_cu) tu.waves_per_cu = 24u;
     if (!tu.rays_per_refill) tu.rays_per_refill = 1u;
     if (!tu.queue_chunk) tu.queue_chunk = 1u;
-    const std::uint32_t sb = tu.step_batch, wpc = tu.waves_per_cu;
-    const sing::LaunchShape shape = shape_of(c, sb, wpc);
+    const sing::LaunchShape shape = shape_of(c, tu.step_batch, tu.waves_per_cu);
+    const std::uint32_t sb = shape.step_batch, wpc = tu.waves_per_cu;
     const sing::BatchClaim claim{tu.rays_per_refill, tu.queue_chunk};
 
     // graph key: everything the recorded launches depend on

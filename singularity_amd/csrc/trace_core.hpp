@@ -90,10 +90,12 @@ __device__ __forceinline__ void refined_crossing(const RayState<S>& y, const Sce
 // (budget, horizon, escape, then a crossing inside the annulus) is sorted out
 // only on the rare exit path.  XTRA (fixed-step Kerr): the step keeps the
 // state before it for the crossing refinement, and sc.refine selects the FP64
-// crossing root and the null-constraint re-projection.
+// crossing root and the null-constraint re-projection.  phase = y.steps mod 8
+// (fixed-step Kerr): run_steps passes its loop counter, the same for every
+// live lane of the wave (see there), so the branch on it is uniform.
 template <class R, class S, bool KERR, bool TRACK = false, bool ADAPT = false, bool XTRA = false>
 __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneConst<R>& sc, const KerrK<R>& k,
-                                                 std::uint32_t* rejected = nullptr) {
+                                                 std::uint32_t* rejected, std::uint32_t phase) {
     R r = R(y.r), th = R(y.th);
     const R mid = R(1.5707963267948966192313216916397514);
     const R dp = R(y.prev_th) - mid, dc = th - mid;
@@ -172,7 +174,7 @@ __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneCons
             // exact difference of the stored angles (sincos_shift); the
             // rotation error stays below a few ulp between full evaluations.
             R st, ct;
-            if ((y.steps & 7u) == 0u) {
+            if (phase == 0u) {
                 dsincos(th, &st, &ct);
             } else {
                 st = R(y.sth);
@@ -192,7 +194,7 @@ __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneCons
                 if (sc.refine & SING_REFINE_REPROJECT) kerr_reproject(r, pr, pt, st, sc, k);
                 y.sth = S(st);
                 y.cth = S(ct);
-            } else if (((y.steps + 1u) & 7u) != 0u) {
+            } else if (phase != 7u) {
                 sincos_shift(th0, th - th0, st, ct, &st, &ct);
                 y.sth = S(st);
                 y.cth = S(ct);
@@ -218,7 +220,10 @@ __device__ __forceinline__ std::uint32_t advance(RayState<S>& y, const SceneCons
 }
 
 // Up to n steps of a live ray (advance); returns its termination code, or
-// kTermLive if it is still marching, and clears live when it ends.
+// kTermLive if it is still marching, and clears live when it ends.  n is a
+// multiple of 8 and every ray starts (steps = 0) at a call's first step, so at
+// step i of the call each live lane has taken i mod 8 steps modulo 8 (a
+// fixed-step ray advances one step per iteration until it ends).
 template <class R, class S, bool KERR, bool TRACK, bool ADAPT, bool XTRA = false>
 __device__ __forceinline__ std::uint32_t run_steps(RayState<S>& y, const SceneConst<R>& sc, bool& live,
                                                    std::uint32_t n, std::uint32_t* rejected) {
@@ -226,7 +231,7 @@ __device__ __forceinline__ std::uint32_t run_steps(RayState<S>& y, const SceneCo
     const KerrK<R> k = ray_k<R, S, KERR>(y, sc);
     for (std::uint32_t i = 0; i < n; ++i) {
         if (live) {
-            const std::uint32_t t = advance<R, S, KERR, TRACK, ADAPT, XTRA>(y, sc, k, rejected);
+            const std::uint32_t t = advance<R, S, KERR, TRACK, ADAPT, XTRA>(y, sc, k, rejected, i & 7u);
             if (t != kTermLive) {
                 live = false;
                 term = t;

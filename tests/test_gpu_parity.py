@@ -113,7 +113,9 @@ def test_frames_are_bit_reproducible(backends):
 def test_frames_do_not_depend_on_launch_shape(backends, prec):
     """A ray's arithmetic depends only on its own state: the full sin/cos
     evaluation of the fixed step comes every 8th step of the ray, not at lane
-    refills, so step batch and grid size leave the frame bit-identical."""
+    refills (step batches are rounded up to a multiple of 8, here 5 -> 8, so
+    the live lanes of a wave share that phase), and step batch and grid size
+    leave the frame bit-identical."""
     u = config3(192, 108, 3)
     ref = render(backends, u, prec, step_batch=32, waves_per_cu=24)
     for sb, wpc in ((8, 16), (5, 12), (64, 32)):

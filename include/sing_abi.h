@@ -137,12 +137,18 @@ typedef struct SingRenderOpts {
     uint32_t waves_per_cu;/* persistent grid size in waves per CU (0 = tuned default) */
     uint32_t sampler;     /* SING_SAMPLER_* */
     uint32_t refine;      /* SING_REFINE_* mask; Kerr with the fixed step (adapt_tol = 0) only */
-    uint32_t reserved[5];
+    uint32_t serial_chunks; /* 1: a mixed-precision frame of several sample chunks runs them one after
+                               another on one stream; 0 (default): a chunk's FP64 list pass and shade
+                               run on a second stream beside the next chunk's set-up and FP32 pass
+                               (same frame bits).  FP32 and FP64 frames always run serially. */
+    uint32_t reserved[4];
 } SingRenderOpts;
 
 typedef struct SingRenderStats {
-    double kernel_ms;     /* all kernels of the frame, HIP events on the context stream */
-    double trace_ms;      /* trace kernels (FP32 pass and FP64 pass) */
+    double kernel_ms;     /* the frame on the GPU, first kernel start to last kernel end (HIP events) */
+    double trace_ms;      /* trace kernels (FP32 pass and FP64 pass), summed over the chunks; with
+                             overlapped chunks a chunk's FP64 pass and shade share the time of the
+                             next chunk's FP32 pass, so the parts can add up to more than kernel_ms */
     double shade_ms;      /* shade/accumulate kernels */
     double post_ms;       /* post-pass kernels (0 when no post pass) */
     uint64_t rays;        /* pixel samples traced */

@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256) void shade_kernel(SceneConst<float> sc, const 
 
 hipError_t launch_trace_chunk(const SceneConst<float>& sf, const SceneConst<double>& sd, const float2* halton,
                               const Chunk& ch, QueueCtl* ctl, std::uint32_t mode, const LaunchShape& shape,
-                              hipStream_t stream, const hipEvent_t* ev) {
+                              hipStream_t stream, const hipEvent_t* ev, hipStream_t tail, hipEvent_t join) {
     auto mark = [&](int i) {
         if (ev) (void)hipEventRecord(ev[i], stream);
     };
@@ -394,6 +394,12 @@ hipError_t launch_trace_chunk(const SceneConst<float>& sf, const SceneConst<doub
         mark(1);
         if (sf.flags & SING_FLAG_STARFIELD_ON)
             hipLaunchKernelGGL(audit_kernel, dim3(shape.blocks), block, 0, stream, sf, ch, ctl);
+        if (tail && tail != stream) {
+            hipError_t e = hipEventRecord(join, stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(tail, join, 0);
+            if (e != hipSuccess) return e;
+            stream = tail;
+        }
         mark(2);
         SING_TRACE(3, shape.blocks64, sb_list);
         mark(3);

@@ -88,12 +88,17 @@ struct LaunchShape {
 
 // mode: SING_PREC_FP32 -> FP32 trace; SING_PREC_FP64 -> FP64 trace of every
 // ray; SING_PREC_MIXED -> FP32 trace with band deferral, then FP64 over the list.
-// ev (optional, 4 events): recorded on `stream` before and after the FP32 trace
-// kernel (ev[0], ev[1]) and before and after the FP64 trace kernel (ev[2],
-// ev[3]); a pass the mode does not run gets its two events back to back.
+// ev (optional, 4 events): recorded before and after the FP32 trace kernel
+// (ev[0], ev[1]) and before and after the FP64 trace kernel (ev[2], ev[3]), on
+// the stream that runs it; a pass the mode does not run gets its two events
+// back to back.  tail (optional, mixed only): the stream of the FP64 list
+// pass; `join` is recorded on `stream` after the FP32 pass and its audit, and
+// `tail` waits on it, so the list pass can run beside whatever `stream` is
+// given next.
 hipError_t launch_trace_chunk(const SceneConst<float>& sf, const SceneConst<double>& sd, const float2* halton,
                               const Chunk& ch, QueueCtl* ctl, std::uint32_t mode, const LaunchShape& shape,
-                              hipStream_t stream, const hipEvent_t* ev = nullptr);
+                              hipStream_t stream, const hipEvent_t* ev = nullptr, hipStream_t tail = nullptr,
+                              hipEvent_t join = nullptr);
 hipError_t launch_shade_chunk(const SceneConst<float>& sf, const Chunk& ch, PixAcc* acc, bool first, bool last,
                               const TraceOutputs& out, hipStream_t stream);
 

@@ -1,0 +1,61 @@
+"""Frames of several sample chunks: the mixed form's overlapped schedule (a
+chunk's FP64 list pass and shade on a second stream beside the next chunk's
+set-up and FP32 pass, chunk buffers used by parity) against the serial one
+(SingRenderOpts.serial_chunks = 1), bit for bit; FP32 and FP64 frames (always
+serial) give the same bits either way.
+
+A chunk holds at most 2 GiB of 16-byte ray records, so 1024x768 at 400 spp is
+three chunks (170, 170 and 60 samples): the third reuses the first one's
+buffers and has to wait for its shade."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import singularity_amd as sa
+from helpers import scene_uniforms
+
+pytestmark = pytest.mark.gpu
+
+OUTS = sa.OUT_RGBA8 | sa.OUT_HDR | sa.OUT_CLASS | sa.OUT_STEPS | sa.OUT_HDR16
+
+
+@pytest.fixture(scope="module")
+def be():
+    b = sa.HipBackend(1024, 768)
+    yield b
+    b.shutdown()
+
+
+CASES = {
+    "fp32": dict(precision=sa.PREC_FP32),
+    "fp64": dict(precision=sa.PREC_FP64),
+    "mixed": dict(precision=sa.PREC_MIXED),
+    "mixed_adaptive_bloom": dict(precision=sa.PREC_MIXED, adapt_tol=1e-6, post=sa.POST_BLOOM, turb_seed=7),
+    "mixed_cross64": dict(precision=sa.PREC_MIXED, refine=sa.REFINE_CROSS64),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_overlapped_chunks_equal_serial_chunks(be, name):
+    u = scene_uniforms(metric=sa.METRIC_KERR, spin=0.94, r_in=1.7, r_out=30.0, width=1024, height=768, spp=400)
+    kw = dict(CASES[name])
+    prec = kw.pop("precision")
+    print(f"chunks {name}: Kerr a=0.94 1024x768 400 spp, overlapped", flush=True)
+    a = be.render_outputs(u, OUTS, prec, **kw)
+    print(f"chunks {name}: serial", flush=True)
+    b = be.render_outputs(u, OUTS, prec, serial_chunks=1, **kw)
+    assert a.stats.chunks == 3 and b.stats.chunks == 3
+    n = 1024 * 768 * 400
+    assert a.stats.rays == b.stats.rays == n
+    assert (a.stats.steps, a.stats.fp64_rays, a.stats.steps32, a.stats.steps64, a.stats.rejected) == (
+        b.stats.steps, b.stats.fp64_rays, b.stats.steps32, b.stats.steps64, b.stats.rejected)
+    assert np.array_equal(a.rgba, b.rgba)
+    assert np.array_equal(a.hdr.view(np.uint32), b.hdr.view(np.uint32))
+    assert np.array_equal(a.hdr16.view(np.uint16), b.hdr16.view(np.uint16))
+    assert np.array_equal(a.cls, b.cls)
+    assert np.array_equal(a.steps, b.steps)
+    # the per-pixel step totals add up to the frame's
+    assert int(a.steps.astype(np.uint64).sum()) == a.stats.steps
+    assert a.stats.kernel_ms > 0.0 and a.stats.trace32_ms + a.stats.trace64_ms > 0.0

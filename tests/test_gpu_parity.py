@@ -58,10 +58,13 @@ def test_fp32_matches_fp32_oracle(backends, orc, name):
     o = orc.render(u, 32)
     n = u.width * u.height
     assert f.stats.rays == n * max(1, u.supersample)
-    assert (f.cls.ravel() == o["cls"]).mean() >= 0.999
     d = np.abs(f.rgba.reshape(-1, 4).astype(int) - o["rgba"].astype(int)).max(-1)
+    rel = rel_err(f.hdr, o["hdr"])
+    print(f"{name} fp32 vs fp32 oracle: class {(f.cls.ravel() == o['cls']).mean():.6f}, rgba<=1 {(d <= 1).mean():.6f}, "
+          f"hdr<=1e-3 {(rel <= 1e-3).mean():.6f}", flush=True)
+    assert (f.cls.ravel() == o["cls"]).mean() >= 0.999
     assert (d <= 1).mean() >= 0.998
-    assert (rel_err(f.hdr, o["hdr"]) <= 1e-3).mean() >= 0.997
+    assert (rel <= 1e-3).mean() >= 0.997
     assert abs(int(f.steps.sum()) - int(o["steps"].sum())) <= 1e-3 * o["steps"].sum()
     assert int(f.steps.sum()) == f.stats.steps
 
@@ -72,9 +75,12 @@ def test_fp64_matches_fp64_oracle(backends, orc, name):
     f = render(backends, u, sa.PREC_FP64)
     o = orc.render(u, 64)
     assert f.stats.fp64_rays == f.stats.rays
+    rel = rel_err(f.hdr, o["hdr"])
+    print(f"{name} fp64 vs fp64 oracle: class {(f.cls.ravel() == o['cls']).mean():.6f}, "
+          f"hdr<=1e-5 {(rel <= 1e-5).mean():.6f}, hdr<=1e-3 {(rel <= 1e-3).mean():.6f}", flush=True)
     assert (f.cls.ravel() == o["cls"]).mean() >= 0.9999
     # FP64 rays are shaded from float termination records: radiance rounding ~1e-6
-    assert (rel_err(f.hdr, o["hdr"]) <= 1e-5).mean() >= 0.999
+    assert (rel <= 1e-5).mean() >= 0.999
 
 
 @pytest.mark.parametrize("name", ["kerr_a05", "kerr_a094", "config1", "config3_small"])
@@ -85,8 +91,15 @@ def test_mixed_matches_fp64_oracle_outside_band(backends, orc, name):
     band = orc.boundary_band(u)
     keep = ~band
     assert 0 < f.stats.fp64_rays < f.stats.rays
+    rel = rel_err(f.hdr, o["hdr"])
+    d = o["drift"]
+    print(f"{name} mixed vs fp64 oracle: band {band.mean():.6f}, class outside band "
+          f"{(f.cls.ravel()[keep] == o['cls'][keep]).mean():.6f}, hdr<=1e-3 outside band {(rel[keep] <= 1e-3).mean():.6f}, "
+          f"all pixels {(rel <= 1e-3).mean():.6f}; fp64 rays {f.stats.fp64_rays / f.stats.rays:.4f}; oracle drift "
+          f"(Schwarzschild: E, L, null residual; Kerr: 2H, Q) median {np.median(d, 0)[:3]}, max {d.max(0)[:3]}",
+          flush=True)
     assert (f.cls.ravel()[keep] == o["cls"][keep]).mean() >= 0.999
-    assert (rel_err(f.hdr, o["hdr"])[keep] <= 1e-3).mean() >= 0.999
+    assert (rel[keep] <= 1e-3).mean() >= 0.999
 
 
 def test_golden_structure_from_gpu(backends):

@@ -356,6 +356,14 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
     ch.sd = c->d_sd;
     hipEvent_t* ev = c->events.data();
     hipEvent_t* join = overlap ? c->join_events.data() : nullptr;  // per chunk: front done, shade done
+    // a failure after the first launch returns only after both streams have
+    // drained, so no queued pass of this frame still uses the chunk buffers or
+    // the queue counters when the next call resets them
+    auto bail = [&](hipError_t err, const char* where) {
+        if (overlap) (void)hipStreamSynchronize(sb);
+        (void)hipStreamSynchronize(sa);
+        return c->hip_fail(err, where);
+    };
     for (std::uint32_t k = 0; k < n_chunks; ++k) {
         const std::uint32_t b = k % nbuf;
         ch.s0 = k * S;
@@ -366,36 +374,36 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
         ch.item_rec = o.precision != SING_PREC_FP64 ? c->d_item_rec + b * items : nullptr;
         sing::QueueCtl* ctl = c->d_ctl + b;
         if (overlap && k >= 2 && (e = hipStreamWaitEvent(sa, join[2 * (k - 2) + 1], 0)) != hipSuccess)
-            return c->hip_fail(e, "hipStreamWaitEvent(chunk buffers)");
+            return bail(e, "hipStreamWaitEvent(chunk buffers)");
         // the work queues restart per chunk; the frame counters keep accumulating
         if (k >= nbuf && (e = hipMemsetAsync(ctl, 0, offsetof(sing::QueueCtl, steps), sa)) != hipSuccess)
-            return c->hip_fail(e, "hipMemsetAsync(queue)");
+            return bail(e, "hipMemsetAsync(queue)");
         hipEventRecord(ev[8 * k + 0], sa);
         const bool mixed = o.precision == SING_PREC_MIXED;
         if ((e = sing::launch_trace_chunk(sf, sd, offsets, ch, ctl, o.precision, shape, sa, ev + 8 * k + 4,
                                           mixed ? sb : nullptr, overlap ? join[2 * k] : nullptr)) != hipSuccess)
-            return c->hip_fail(e, "trace kernel launch");
+            return bail(e, "trace kernel launch");
         hipEventRecord(ev[8 * k + 1], sa);
         if (overlap && !mixed) {
             if ((e = hipEventRecord(join[2 * k], sa)) != hipSuccess ||
                 (e = hipStreamWaitEvent(sb, join[2 * k], 0)) != hipSuccess)
-                return c->hip_fail(e, "cross-stream join");
+                return bail(e, "cross-stream join");
         }
         hipEventRecord(ev[8 * k + 3], sb);
         if ((e = sing::launch_shade_chunk(sf, ch, c->d_acc, k == 0, k + 1 == n_chunks, to, sb)) != hipSuccess)
-            return c->hip_fail(e, "shade kernel launch");
+            return bail(e, "shade kernel launch");
         hipEventRecord(ev[8 * k + 2], sb);
         if (overlap && (e = hipEventRecord(join[2 * k + 1], sb)) != hipSuccess)
-            return c->hip_fail(e, "hipEventRecord(shade done)");
+            return bail(e, "hipEventRecord(shade done)");
     }
     if (overlap && (e = hipStreamWaitEvent(sa, join[2 * (n_chunks - 1) + 1], 0)) != hipSuccess)
-        return c->hip_fail(e, "hipStreamWaitEvent(last shade)");
+        return bail(e, "hipStreamWaitEvent(last shade)");
     hipEvent_t post0 = ev[8 * n_chunks], post1 = ev[8 * n_chunks + 1], fin = ev[8 * n_chunks + 2];
     if (o.post == SING_POST_BLOOM) {
         hipEventRecord(post0, sa);
         e = sing::launch_bloom(c->d_hdr16, c->width, c->height, u->bloom_threshold, u->bloom_strength, u->exposure,
                                c->d_rgba, sa);
-        if (e != hipSuccess) return c->hip_fail(e, "bloom launch");
+        if (e != hipSuccess) return bail(e, "bloom launch");
         hipEventRecord(post1, sa);
     }
     hipEventRecord(fin, sa);

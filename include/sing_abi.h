@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SING_ABI_VERSION 2
+#define SING_ABI_VERSION 3
 
 /* --- the 160-byte uniform block, byte-for-byte shared_shader/uniforms.h:54-95 --- */
 typedef struct SingVec4 {
@@ -178,9 +178,23 @@ const char* sing_backend_name(void); /* "HIP-gfx950" (RenderBackend::name) */
 int sing_abi_version(void);
 
 /* --- render (RenderBackend::render_frame + the kernel launch) --------------
- * Synchronous: returns after the frame is complete.  `u` is copied at call
- * time.  u->width/height must equal the context size. */
+ * Replaces CudaBackend::render_frame (render/cuda/cuda_backend.cu:139-152: one
+ * kernel launch, then cudaDeviceSynchronize).  Synchronous: returns after the
+ * frame is complete.  `u` is copied at call time.  u->width/height must equal
+ * the context size.  Equivalent to sing_render_async followed by sing_wait. */
 int sing_render(SingContext* ctx, const SingUniforms* u, const SingRenderOpts* opts, SingRenderStats* stats);
+/* The same frame without the wait: validates the arguments (errors are
+ * returned here, as from sing_render), enqueues the frame on the context's
+ * streams and returns; `u` and `opts` may be freed on return.  The host is
+ * free until sing_wait.  Any other call on the context (capture, render,
+ * resize, post pass, batch, differentiable render, diagnostics, destroy)
+ * first completes a pending frame, so its outputs are never read half-done;
+ * the frame's stats stay available to sing_wait until the next submission. */
+int sing_render_async(SingContext* ctx, const SingUniforms* u, const SingRenderOpts* opts);
+/* Waits for the frame submitted by sing_render_async (if it has not completed
+ * yet) and returns its status and stats (optional), as sing_render would
+ * have.  SING_ERR_STATE when no frame was submitted since the last wait. */
+int sing_wait(SingContext* ctx, SingRenderStats* stats);
 
 /* --- capture (RenderBackend::capture_frame) ---------------------------------
  * Copies the last frame's outputs into caller buffers of at least the stated

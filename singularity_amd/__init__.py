@@ -133,6 +133,8 @@ _SIGS = {
     "sing_destroy": (None, [C.c_void_p]),
     "sing_resize": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "sing_render": (C.c_int, [C.c_void_p, C.POINTER(Uniforms), C.POINTER(RenderOpts), C.POINTER(RenderStats)]),
+    "sing_render_async": (C.c_int, [C.c_void_p, C.POINTER(Uniforms), C.POINTER(RenderOpts)]),
+    "sing_wait": (C.c_int, [C.c_void_p, C.POINTER(RenderStats)]),
     "sing_render_noalloc": (C.c_int, [C.c_int, C.POINTER(Uniforms), C.POINTER(RenderOpts), C.POINTER(DeviceOutputs),
                                       C.c_void_p, C.POINTER(RenderStats)]),
     "sing_capture_rgba8": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -292,6 +294,18 @@ class HipBackend:
         st = RenderStats()
         o = opts if opts is not None else RenderOpts()
         self._check(lib().sing_render(self._ctx, C.byref(u), C.byref(o), C.byref(st)))
+        return st
+
+    def render_async(self, u: Uniforms, opts: RenderOpts | None = None) -> None:
+        """sing_render_async: enqueue the frame and return; wait() completes it (as does
+        any other call on this backend)."""
+        o = opts if opts is not None else RenderOpts()
+        self._check(lib().sing_render_async(self._ctx, C.byref(u), C.byref(o)))
+
+    def wait(self) -> RenderStats:
+        """sing_wait: the stats of the frame submitted by render_async."""
+        st = RenderStats()
+        self._check(lib().sing_wait(self._ctx, C.byref(st)))
         return st
 
     def render_frame(self, scene: Scene, camera: Camera, opts: RenderOpts | None = None) -> RenderStats:

@@ -629,6 +629,7 @@ int diff_run(SingContext* c, const SingUniforms* base, std::uint32_t n, const Si
              const float* weights, bool backward, double* loss, double* grad, float* hdr_out, SingDiffStats* stats,
              const char* who) {
     if (!c) return SING_ERR_ARG;
+    c->complete_pending();
     if (!base || !params || !cams) return c->fail(SING_ERR_ARG, "%s: null argument", who);
     if (n < 1 || n > SING_DIFF_MAX_BATCH)
         return c->fail(SING_ERR_ARG, "%s: evaluations must be in [1, %u]", who, unsigned(SING_DIFF_MAX_BATCH));
@@ -764,6 +765,7 @@ int sing_diff_loss_grad_batch(SingContext* ctx, const SingUniforms* base, uint32
 
 int sing_diff_capture_pixels(SingContext* c, double* grad_out, uint32_t* meta_out) {
     if (!c) return SING_ERR_ARG;
+    c->complete_pending();
     if (!c->diff_P || !c->d_dset) return c->fail(SING_ERR_STATE, "sing_diff_capture_pixels: no differentiable frame");
     hipError_t e = hipSetDevice(c->device);
     const std::size_t P = c->diff_P;  // one image: the first evaluation of the last call

@@ -483,6 +483,7 @@ void SingContext::free_batch() {
 extern "C" int sing_render_batch(SingContext* c, const SingBatchItem* items, uint32_t n, const SingBatchOpts* opts_in,
                                  SingBatchStats* stats) {
     if (!c) return SING_ERR_ARG;
+    c->complete_pending();
     if (!items || n == 0) return c->fail(SING_ERR_ARG, "sing_render_batch: no scenes");
     if (n > 65536u) return c->fail(SING_ERR_ARG, "sing_render_batch: at most 65536 scenes per batch");
     SingBatchOpts o{};

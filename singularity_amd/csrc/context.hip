@@ -223,6 +223,7 @@ SingContext* sing_create(uint32_t width, uint32_t height, int device) {
     if (e == hipSuccess) e = hipMalloc(&c->d_ctl, 2 * sizeof(sing::QueueCtl));  // one per chunk parity
     if (e == hipSuccess) e = hipMalloc(&c->d_halton, (1024 + sing::kGridTableLen) * sizeof(float2));
     if (e == hipSuccess) e = hipMalloc(&c->d_sd, sizeof(sing::SceneConst<double>));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_ctl, 2 * sizeof(sing::QueueCtl), hipHostMallocDefault);
     if (e != hipSuccess) {
         sing_set_global_error(std::string("sing_create: HIP setup failed: ") + hipGetErrorString(e));
         sing_destroy(c);
@@ -251,6 +252,7 @@ SingContext* sing_create(uint32_t width, uint32_t height, int device) {
 void sing_destroy(SingContext* c) {
     if (!c) return;
     hipSetDevice(c->device);
+    c->complete_pending();
     if (c->stream) hipStreamSynchronize(c->stream);
     c->free_frame();
     hipFree(c->d_ctl);
@@ -260,6 +262,7 @@ void sing_destroy(SingContext* c) {
     hipFree(c->d_band);
     hipFree(c->d_init);
     hipFree(c->d_item_rec);
+    if (c->h_ctl) hipHostFree(c->h_ctl);
     c->free_diff();
     c->free_batch();
     for (auto& e : c->events) hipEventDestroy(e);
@@ -275,6 +278,7 @@ void sing_destroy(SingContext* c) {
 int sing_resize(SingContext* c, uint32_t w, uint32_t h) {
     if (!c) return SING_ERR_ARG;
     if (w < 1 || h < 1 || w > 32768 || h > 32768) return c->fail(SING_ERR_ARG, "sing_resize: bad size %ux%u", w, h);
+    c->complete_pending();
     if (w == c->width && h == c->height) return SING_OK;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
@@ -282,8 +286,10 @@ int sing_resize(SingContext* c, uint32_t w, uint32_t h) {
     return c->alloc_frame(w, h);
 }
 
-int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opts_in, SingRenderStats* stats) {
+int sing_render_async(SingContext* c, const SingUniforms* u, const SingRenderOpts* opts_in) {
     if (!c) return SING_ERR_ARG;
+    c->complete_pending();
+    c->stats_unclaimed = false;
     if (!u) return c->fail(SING_ERR_ARG, "sing_render: null uniforms");
     if (u->width != c->width || u->height != c->height)
         return c->fail(SING_ERR_ARG, "sing_render: uniforms size %ux%u != context %ux%u", u->width, u->height,
@@ -292,6 +298,7 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
     if (opts_in) o = *opts_in;
     char msg[160];
     if (!sing::render_args_ok(*u, o, "sing_render", msg, sizeof(msg))) return c->fail(SING_ERR_ARG, "%s", msg);
+    c->outputs_valid = 0;  // the outputs are overwritten from here on
     std::uint32_t outs = o.outputs ? o.outputs : SING_OUT_RGBA8;
     if (o.post == SING_POST_BLOOM) outs |= SING_OUT_HDR16 | SING_OUT_RGBA8;
     hipError_t e = hipSetDevice(c->device);
@@ -407,20 +414,34 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
         hipEventRecord(post1, sa);
     }
     hipEventRecord(fin, sa);
-    sing::QueueCtl cq[2]{};
-    if ((e = hipMemcpyAsync(cq, c->d_ctl, nbuf * sizeof(sing::QueueCtl), hipMemcpyDeviceToHost, sa)) != hipSuccess)
-        return c->hip_fail(e, "hipMemcpyAsync(stats)");
-    if ((e = hipStreamSynchronize(sa)) != hipSuccess) return c->hip_fail(e, "hipStreamSynchronize");
-    c->outputs_valid = outs;
-    sing::QueueCtl ctl = cq[0];
-    if (nbuf == 2u) {
-        ctl.steps += cq[1].steps;
-        ctl.rays += cq[1].rays;
-        ctl.rays64 += cq[1].rays64;
-        ctl.rejected += cq[1].rejected;
-        ctl.steps32 += cq[1].steps32;
-        ctl.steps64 += cq[1].steps64;
+    if ((e = hipMemcpyAsync(c->h_ctl, c->d_ctl, nbuf * sizeof(sing::QueueCtl), hipMemcpyDeviceToHost, sa)) !=
+        hipSuccess)
+        return bail(e, "hipMemcpyAsync(stats)");
+    c->pend = SingContext::PendingFrame{n_chunks, nbuf, outs, o.precision, o.post, overlap, std::uint64_t(P) * spp};
+    c->pending = true;
+    return SING_OK;
+}
+
+int SingContext::complete_pending() {
+    if (!pending) return SING_OK;
+    pending = false;
+    stats_unclaimed = true;
+    std::memset(&last_stats, 0, sizeof(last_stats));
+    hipSetDevice(device);
+    const hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return last_rc = hip_fail(e, "hipStreamSynchronize");
+    const PendingFrame& f = pend;
+    sing::QueueCtl ctl = h_ctl[0];
+    if (f.nbuf == 2u) {
+        ctl.steps += h_ctl[1].steps;
+        ctl.rays += h_ctl[1].rays;
+        ctl.rays64 += h_ctl[1].rays64;
+        ctl.rejected += h_ctl[1].rejected;
+        ctl.steps32 += h_ctl[1].steps32;
+        ctl.steps64 += h_ctl[1].steps64;
     }
+    hipEvent_t* ev = events.data();
+    const std::uint32_t n_chunks = f.n_chunks;
     double trace_ms = 0.0, shade_ms = 0.0, post_ms = 0.0, t32_ms = 0.0, t64_ms = 0.0;
     for (std::uint32_t k = 0; k < n_chunks; ++k) {
         float a = 0.0f, b = 0.0f, t32 = 0.0f, t64 = 0.0f;
@@ -430,44 +451,58 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
         hipEventElapsedTime(&t64, ev[8 * k + 6], ev[8 * k + 7]);
         // the mixed form's list pass sits inside [start, front done] unless it
         // ran on the second stream
-        trace_ms += a + ((overlap && o.precision == SING_PREC_MIXED) ? t64 : 0.0f);
+        trace_ms += a + ((f.overlap && f.precision == SING_PREC_MIXED) ? t64 : 0.0f);
         shade_ms += b;
         t32_ms += t32;
         t64_ms += t64;
     }
     float total = 0.0f;
-    hipEventElapsedTime(&total, ev[0], fin);
-    if (o.post == SING_POST_BLOOM) {
+    hipEventElapsedTime(&total, ev[0], ev[8 * n_chunks + 2]);
+    if (f.post == SING_POST_BLOOM) {
         float p = 0.0f;
-        hipEventElapsedTime(&p, post0, post1);
+        hipEventElapsedTime(&p, ev[8 * n_chunks], ev[8 * n_chunks + 1]);
         post_ms = p;
     }
-    if (stats) {
-        std::memset(stats, 0, sizeof(*stats));
-        stats->kernel_ms = total;
-        stats->trace_ms = trace_ms;
-        stats->shade_ms = shade_ms;
-        stats->post_ms = post_ms;
-        stats->rays = ctl.rays;
-        stats->steps = ctl.steps;
-        stats->fp64_rays = ctl.rays64;
-        stats->chunks = n_chunks;
-        stats->rejected = ctl.rejected;
-        stats->trace32_ms = t32_ms;
-        stats->trace64_ms = t64_ms;
-        stats->steps32 = ctl.steps32;
-        stats->steps64 = ctl.steps64;
-    }
-    const std::uint64_t expect = std::uint64_t(P) * spp;
-    if (ctl.rays != expect)
-        return c->fail(SING_ERR_STATE, "sing_render: traced %llu rays, expected %llu", (unsigned long long)ctl.rays,
-                       (unsigned long long)expect);
-    return SING_OK;
+    SingRenderStats& st = last_stats;
+    st.kernel_ms = total;
+    st.trace_ms = trace_ms;
+    st.shade_ms = shade_ms;
+    st.post_ms = post_ms;
+    st.rays = ctl.rays;
+    st.steps = ctl.steps;
+    st.fp64_rays = ctl.rays64;
+    st.chunks = n_chunks;
+    st.rejected = ctl.rejected;
+    st.trace32_ms = t32_ms;
+    st.trace64_ms = t64_ms;
+    st.steps32 = ctl.steps32;
+    st.steps64 = ctl.steps64;
+    if (ctl.rays != f.expect_rays)
+        return last_rc = fail(SING_ERR_STATE, "sing_render: traced %llu rays, expected %llu",
+                              (unsigned long long)ctl.rays, (unsigned long long)f.expect_rays);
+    outputs_valid = f.outs;
+    return last_rc = SING_OK;
+}
+
+int sing_wait(SingContext* c, SingRenderStats* stats) {
+    if (!c) return SING_ERR_ARG;
+    if (!c->pending && !c->stats_unclaimed) return c->fail(SING_ERR_STATE, "sing_wait: no frame was submitted");
+    const int rc = c->pending ? c->complete_pending() : c->last_rc;
+    c->stats_unclaimed = false;
+    if (stats) *stats = c->last_stats;
+    return rc;
+}
+
+int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opts, SingRenderStats* stats) {
+    const int rc = sing_render_async(c, u, opts);
+    if (rc != SING_OK) return rc;
+    return sing_wait(c, stats);
 }
 
 static int capture(SingContext* c, std::uint32_t bit, const void* src, void* dst, std::size_t need, std::size_t bytes,
                    const char* what) {
     if (!c) return SING_ERR_ARG;
+    c->complete_pending();
     if (!dst) return c->fail(SING_ERR_ARG, "capture %s: null destination", what);
     if (bytes < need) return c->fail(SING_ERR_ARG, "capture %s: buffer %zu bytes < %zu", what, bytes, need);
     if (!(c->outputs_valid & bit)) return c->fail(SING_ERR_STATE, "capture %s: not rendered in the last frame", what);
@@ -500,6 +535,7 @@ int sing_capture_hdr16(SingContext* c, uint16_t* dst, size_t bytes) {
 
 int sing_post_bloom(SingContext* c, const SingUniforms* u, uint32_t repeat, SingRenderStats* stats) {
     if (!c) return SING_ERR_ARG;
+    c->complete_pending();
     if (!u) return c->fail(SING_ERR_ARG, "sing_post_bloom: null uniforms");
     if (repeat < 1u || repeat > 1000u) return c->fail(SING_ERR_ARG, "sing_post_bloom: repeat must be in [1, 1000]");
     if (c->width < 2 || c->height < 2) return c->fail(SING_ERR_ARG, "sing_post_bloom: bloom needs at least 2x2 pixels");

@@ -53,6 +53,19 @@ struct SingContext {
     sing::SceneConst<double> h_sd{};           // its host source (lives until the copy has run)
     std::uint32_t outputs_valid = 0;  // SING_OUT_* of the last frame
     std::string err;
+    // a frame submitted by sing_render_async and not yet completed: what its
+    // completion needs (stats from the events and the queue counters)
+    struct PendingFrame {
+        std::uint32_t n_chunks, nbuf, outs, precision, post;
+        bool overlap;
+        std::uint64_t expect_rays;
+    };
+    bool pending = false;
+    PendingFrame pend{};
+    bool stats_unclaimed = false;      // a completed async frame whose stats sing_wait has not returned
+    SingRenderStats last_stats{};
+    int last_rc = 0;
+    sing::QueueCtl* h_ctl = nullptr;   // pinned host copy of the two queue-counter blocks
     // differentiable render
     sing::DiffSet* d_dset = nullptr;  // one per evaluation of a call
     sing::DiffBuffers diff{};
@@ -80,6 +93,11 @@ struct SingContext {
     std::vector<std::pair<std::uint64_t, sing::BatchTune>> b_tuned;  // kind of batch -> tuned launch
 
     int fail(int code, const char* fmt, ...);
+    // Completes a submitted frame (waits for it, fills last_stats, checks its
+    // ray total); every entry point on the context calls it first, so a
+    // pending frame is never overwritten or read half-done.  Returns the
+    // frame's status (SING_OK when nothing was pending).
+    int complete_pending();
     int hip_fail(hipError_t e, const char* where);
     void free_frame();
     int alloc_frame(std::uint32_t w, std::uint32_t h);

@@ -203,6 +203,7 @@ int sing_kerr_geometry(float M, float a, SingKerrGeometry* out) {
 int sing_photon_orbit(SingContext* c, float M, float a, float r_init, float orbits, float h, SingOrbitRow* rows,
                       std::uint32_t cap, std::uint32_t* n_rows, double* kernel_ms) {
     if (!c) return SING_ERR_ARG;
+    c->complete_pending();
     if (!n_rows) return c->fail(SING_ERR_ARG, "sing_photon_orbit: null n_rows");
     *n_rows = 0;
     // cli/main.cpp:1282-1305: argument checks in the driver's order
@@ -298,6 +299,7 @@ int sing_photon_orbit(SingContext* c, float M, float a, float r_init, float orbi
 
 int sing_benchmark_fan(SingContext* c, const SingFanOpts* o, std::uint32_t* steps_per_ray, SingFanStats* stats) {
     if (!c) return SING_ERR_ARG;
+    c->complete_pending();
     if (!o) return c->fail(SING_ERR_ARG, "sing_benchmark_fan: null options");
     const bool kerr = o->metric == SING_METRIC_KERR;
     if (!kerr && o->metric != SING_METRIC_SCHWARZSCHILD)

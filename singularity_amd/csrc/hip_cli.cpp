@@ -303,21 +303,37 @@ int main(int argc, char** argv) {
 
     constexpr float kTwoPi = 6.28318530717958647693f;
     float frame_time = 0.0f;
-    std::vector<std::uint8_t> rgba(std::size_t(width) * height * 4);
-    for (std::uint32_t f = 0; f < frames; ++f) {
+    // Frame f+1 is submitted (sing_render_async) as soon as frame f's outputs
+    // are captured, so its trace runs on the GPU while the host writes frame
+    // f's files; the files, their order and the printed lines are those of a
+    // render-then-write loop (cuda_cli/main.cpp:243-299).
+    auto submit = [&](std::uint32_t f) {
         const float az = camera_azimuth + (frames > 1 ? (kTwoPi * float(f) / float(frames)) : 0.0f);
         SingCamera cam;
         sing_orbital_camera(camera_distance, camera_elevation, az, fov_y_deg, &cam);
         SingUniforms u;
         sing_pack_uniforms(&scene, &cam, width, height, frame_time, &u);
-        SingRenderStats st{};
-        if (sing_render(ctx, &u, &opts, &st) != SING_OK ||
-            sing_capture_rgba8(ctx, rgba.data(), rgba.size()) != SING_OK) {
-            std::fprintf(stderr, "hip_cli: render failed (frame %u): %s\n", f, sing_last_error(ctx));
-            sing_destroy(ctx);
-            return 1;
-        }
         frame_time += 1.0f / 30.0f;
+        return sing_render_async(ctx, &u, &opts);
+    };
+    auto fail_frame = [&](std::uint32_t f) {
+        std::fprintf(stderr, "hip_cli: render failed (frame %u): %s\n", f, sing_last_error(ctx));
+        sing_destroy(ctx);
+        return 1;
+    };
+    std::vector<std::uint8_t> rgba(std::size_t(width) * height * 4);
+    std::vector<float> hdr(hdr_path.empty() ? 0 : std::size_t(width) * height * 3);
+    std::vector<std::uint8_t> cls(cls_path.empty() ? 0 : std::size_t(width) * height);
+    if (submit(0) != SING_OK) return fail_frame(0);
+    for (std::uint32_t f = 0; f < frames; ++f) {
+        SingRenderStats st{};
+        if (sing_wait(ctx, &st) != SING_OK || sing_capture_rgba8(ctx, rgba.data(), rgba.size()) != SING_OK)
+            return fail_frame(f);
+        const bool hdr_ok = hdr_path.empty() || sing_capture_hdr(ctx, hdr.data(), hdr.size() * 4) == SING_OK;
+        const bool cls_ok = cls_path.empty() || sing_capture_class(ctx, cls.data(), cls.size()) == SING_OK;
+        // the next frame renders while this one's files are written; a
+        // submission error is reported after them, where a serial loop would
+        const bool next_ok = f + 1 >= frames || submit(f + 1) == SING_OK;
         char path_buf[1024];
         const char* path;
         if (frames > 1) {
@@ -331,23 +347,15 @@ int main(int argc, char** argv) {
             sing_destroy(ctx);
             return 1;
         }
-        if (!hdr_path.empty()) {
-            std::vector<float> hdr(std::size_t(width) * height * 3);
-            if (sing_capture_hdr(ctx, hdr.data(), hdr.size() * 4) != SING_OK ||
-                !write_pfm(hdr_path.c_str(), hdr.data(), width, height)) {
-                std::fprintf(stderr, "hip_cli: failed to write '%s'\n", hdr_path.c_str());
-                sing_destroy(ctx);
-                return 1;
-            }
+        if (!hdr_path.empty() && (!hdr_ok || !write_pfm(hdr_path.c_str(), hdr.data(), width, height))) {
+            std::fprintf(stderr, "hip_cli: failed to write '%s'\n", hdr_path.c_str());
+            sing_destroy(ctx);
+            return 1;
         }
-        if (!cls_path.empty()) {
-            std::vector<std::uint8_t> cls(std::size_t(width) * height);
-            if (sing_capture_class(ctx, cls.data(), cls.size()) != SING_OK ||
-                !write_pgm(cls_path.c_str(), cls.data(), width, height)) {
-                std::fprintf(stderr, "hip_cli: failed to write '%s'\n", cls_path.c_str());
-                sing_destroy(ctx);
-                return 1;
-            }
+        if (!cls_path.empty() && (!cls_ok || !write_pgm(cls_path.c_str(), cls.data(), width, height))) {
+            std::fprintf(stderr, "hip_cli: failed to write '%s'\n", cls_path.c_str());
+            sing_destroy(ctx);
+            return 1;
         }
         if (frames > 1) {
             std::printf("wrote %s (%ux%u, frame %u/%u, HIP)\n", path, width, height, f + 1, frames);
@@ -359,6 +367,7 @@ int main(int argc, char** argv) {
             std::printf("stats: frame %u kernel %.2f ms trace %.2f ms rays %llu steps %llu fp64 %llu (%.1f Mrays/s)\n", f,
                         st.kernel_ms, st.trace_ms, (unsigned long long)st.rays, (unsigned long long)st.steps,
                         (unsigned long long)st.fp64_rays, double(st.rays) / st.kernel_ms / 1e3);
+        if (!next_ok) return fail_frame(f + 1);
     }
     sing_destroy(ctx);
 

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 20
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
-    assert L.sing_abi_version() == 2
+    assert L.sing_abi_version() == 3
     assert L.sing_backend_name() == b"HIP-gfx950"
 
 
@@ -43,6 +43,8 @@ def test_create_fails_cleanly_without_device_or_bad_size():
     assert b"width/height" in L.sing_last_error(None)
     # null-context calls return SING_ERR_ARG instead of crashing
     assert L.sing_render(None, None, None, None) == 1
+    assert L.sing_render_async(None, None, None) == 1
+    assert L.sing_wait(None, None) == 1
     assert L.sing_capture_rgba8(None, None, 0) == 1
     assert L.sing_resize(None, 4, 4) == 1
     assert L.sing_render_batch(None, None, 0, None, None) == 1

@@ -376,6 +376,19 @@ int sing_diff_loss_grad_batch(SingContext* ctx, const SingUniforms* base, uint32
                               const float* target, const float* weights, double* loss, double* grad, float* hdr_out,
                               SingDiffStats* stats);
 
+/* The backward pass for a caller's own loss: given dl_dhdr = dLoss/dI (W*H*3
+ * floats, one RGB cotangent per pixel of the HDR image I), returns grad[5] =
+ * sum_p dl_dhdr_p . dI_p/dparams (the vector-Jacobian product of the render,
+ * same parameters and model as sing_diff_loss_grad) and *pairing (optional) =
+ * sum_p dl_dhdr_p . I_p (I before its rounding to the float image), the
+ * scalar whose gradient that is.  With dl_dhdr =
+ * 2 w (I - target) it is sing_diff_loss_grad's gradient: the backward pass
+ * takes a target image there and a dLoss/dHDR image here.  No counterpart in
+ * the reference (it has no differentiable path). */
+int sing_diff_vjp(SingContext* ctx, const SingUniforms* base, const SingDiffParams* p, const SingDiffCamera* cam,
+                  const SingDiffOpts* opts, const float* dl_dhdr, double* pairing, double* grad, float* hdr_out,
+                  SingDiffStats* stats);
+
 /* Per-pixel results of the last sing_diff_* call on ctx (either may be NULL);
  * after a batch call, those of its first parameter point:
  * grad_out W*H*5 doubles, the pixel's loss gradient wrt the 5 parameters (r_in

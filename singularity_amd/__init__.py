@@ -395,6 +395,9 @@ _SIGS.update({
     "sing_diff_loss_grad_batch": (C.c_int, [C.c_void_p, C.POINTER(Uniforms), C.c_uint32, C.c_void_p, C.c_void_p,
                                             C.c_uint32, C.POINTER(DiffOpts), C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_void_p, C.c_void_p, C.POINTER(DiffStats)]),
+    "sing_diff_vjp": (C.c_int, [C.c_void_p, C.POINTER(Uniforms), C.POINTER(DiffParams), C.POINTER(DiffCamera),
+                                C.POINTER(DiffOpts), C.c_void_p, C.POINTER(C.c_double), C.c_void_p, C.c_void_p,
+                                C.POINTER(DiffStats)]),
 })
 DIFF_MAX_BATCH = 32
 
@@ -458,6 +461,22 @@ def diff_loss_grad(be: "HipBackend", base: Uniforms, params: DiffParams, cam: Di
     st = _diff_call(be, "sing_diff_loss_grad", base, params, cam, opts, t.ctypes.data,
                     None if w is None else w.ctypes.data, C.byref(loss), grad.ctypes.data, hdr.ctypes.data)
     return loss.value, grad, hdr, st
+
+
+def diff_vjp(be: "HipBackend", base: Uniforms, params: DiffParams, cam: DiffCamera, dl_dhdr: np.ndarray,
+             opts: DiffOpts | None = None):
+    """sing_diff_vjp: the backward pass for a caller's own loss.  dl_dhdr (H, W, 3) =
+    dLoss/dI; returns (sum dl_dhdr . dI/dparams (5,), the pairing sum dl_dhdr . I,
+    radiance I, stats)."""
+    g = np.ascontiguousarray(dl_dhdr, dtype=np.float32)
+    if g.shape != (be.height, be.width, 3):
+        raise ValueError(f"dl_dhdr shape {g.shape} != {(be.height, be.width, 3)}")
+    pairing = C.c_double(0.0)
+    grad = np.zeros(5, np.float64)
+    hdr = np.empty((be.height, be.width, 3), np.float32)
+    st = _diff_call(be, "sing_diff_vjp", base, params, cam, opts, g.ctypes.data, C.byref(pairing), grad.ctypes.data,
+                    hdr.ctypes.data)
+    return grad, pairing.value, hdr, st
 
 
 def diff_loss_grad_batch(be: "HipBackend", base: Uniforms, params, cams, target: np.ndarray,

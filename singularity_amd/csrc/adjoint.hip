@@ -213,15 +213,25 @@ __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict
     meta[1] = ncross;
     meta[2] = term;
     double* skp = b.sky + 3u * q;
-    const float* tgt = b.target + 3u * pix;
-    const double w = b.weight ? double(b.weight[pix]) : 1.0;
     double l = 0.0;
-    for (int k = 0; k < 3; ++k) {
-        skp[k] = sky[k];
-        b.hdr[3u * q + k] = float(acc[k]);
-        const double diff = double(acc[k]) - double(tgt[k]);
-        b.gI[3u * q + k] = 2.0 * w * diff;
-        l += w * diff * diff;
+    if (b.cot) {
+        const float* g = b.cot + 3u * pix;
+        for (int k = 0; k < 3; ++k) {
+            skp[k] = sky[k];
+            b.hdr[3u * q + k] = float(acc[k]);
+            b.gI[3u * q + k] = double(g[k]);
+            l += double(g[k]) * double(acc[k]);
+        }
+    } else {
+        const float* tgt = b.target + 3u * pix;
+        const double w = b.weight ? double(b.weight[pix]) : 1.0;
+        for (int k = 0; k < 3; ++k) {
+            skp[k] = sky[k];
+            b.hdr[3u * q + k] = float(acc[k]);
+            const double diff = double(acc[k]) - double(tgt[k]);
+            b.gI[3u * q + k] = 2.0 * w * diff;
+            l += w * diff * diff;
+        }
     }
     b.loss[q] = l;
     b.steps[q] = step;
@@ -627,7 +637,7 @@ sing::DiffSet make_diff_set(const SingUniforms& base, const SingDiffParams& p, c
 int diff_run(SingContext* c, const SingUniforms* base, std::uint32_t n, const SingDiffParams* params,
              const SingDiffCamera* cams, std::uint32_t n_cams, const SingDiffOpts* opts_in, const float* target,
              const float* weights, bool backward, double* loss, double* grad, float* hdr_out, SingDiffStats* stats,
-             const char* who) {
+             const char* who, const float* cot = nullptr) {
     if (!c) return SING_ERR_ARG;
     c->complete_pending();
     if (!base || !params || !cams) return c->fail(SING_ERR_ARG, "%s: null argument", who);
@@ -644,7 +654,7 @@ int diff_run(SingContext* c, const SingUniforms* base, std::uint32_t n, const Si
     const std::size_t C = o.checkpoint_every ? o.checkpoint_every : 64u;
     if (C > 4096) return c->fail(SING_ERR_ARG, "%s: checkpoint_every too large", who);
     if (o.r_in_isco > 1u) return c->fail(SING_ERR_ARG, "%s: r_in_isco must be 0 or 1", who);
-    if (backward && !target) return c->fail(SING_ERR_ARG, "%s: null target", who);
+    if (backward && !target && !cot) return c->fail(SING_ERR_ARG, "%s: null %s", who, "target");
     const std::size_t Pimg = std::size_t(c->width) * c->height;
     if (Pimg * n >= (std::size_t(1) << 31)) return c->fail(SING_ERR_ARG, "%s: too many pixels in one call", who);
     std::vector<sing::DiffSet> sets(n);
@@ -676,14 +686,14 @@ int diff_run(SingContext* c, const SingUniforms* base, std::uint32_t n, const Si
     if ((e = hipMemcpyAsync(c->d_dset, sets.data(), n * sizeof(sing::DiffSet), hipMemcpyHostToDevice, c->stream)) !=
         hipSuccess)
         return c->hip_fail(e, "hipMemcpyAsync(scene)");
-    if (target) {
-        if ((e = hipMemcpyAsync(c->d_target, target, Pimg * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream)) !=
-            hipSuccess)
+    if (target || cot) {  // the cotangent image travels in the target buffer
+        if ((e = hipMemcpyAsync(c->d_target, cot ? cot : target, Pimg * 3 * sizeof(float), hipMemcpyHostToDevice,
+                                c->stream)) != hipSuccess)
             return c->hip_fail(e, "hipMemcpyAsync(target)");
     } else if ((e = hipMemsetAsync(c->d_target, 0, Pimg * 3 * sizeof(float), c->stream)) != hipSuccess) {
         return c->hip_fail(e, "hipMemsetAsync(target)");
     }
-    if (weights &&
+    if (weights && !cot &&
         (e = hipMemcpyAsync(c->d_weight, weights, Pimg * sizeof(float), hipMemcpyHostToDevice, c->stream)) !=
             hipSuccess)
         return c->hip_fail(e, "hipMemcpyAsync(weights)");
@@ -695,7 +705,8 @@ int diff_run(SingContext* c, const SingUniforms* base, std::uint32_t n, const Si
         return c->hip_fail(e, "hipMemsetAsync(grad)");
     sing::DiffBuffers b = c->diff;
     b.target = c->d_target;
-    b.weight = weights ? c->d_weight : nullptr;
+    b.weight = (weights && !cot) ? c->d_weight : nullptr;
+    b.cot = cot ? c->d_target : nullptr;
     (void)hipEventRecord(c->events[0], c->stream);
     if ((e = sing::launch_diff(c->d_dset, b, o.precision, backward, c->d_out, c->stream, c->stream2,
                                c->events[1], c->join_events.data())) !=
@@ -761,6 +772,14 @@ int sing_diff_loss_grad_batch(SingContext* ctx, const SingUniforms* base, uint32
                               SingDiffStats* stats) {
     return diff_run(ctx, base, n, params, cams, n_cams, opts, target, weights, true, loss, grad, hdr_out, stats,
                     "sing_diff_loss_grad_batch");
+}
+
+int sing_diff_vjp(SingContext* ctx, const SingUniforms* base, const SingDiffParams* p, const SingDiffCamera* cam,
+                  const SingDiffOpts* opts, const float* dl_dhdr, double* pairing, double* grad, float* hdr_out,
+                  SingDiffStats* stats) {
+    if (ctx && !dl_dhdr) return ctx->fail(SING_ERR_ARG, "sing_diff_vjp: null dl_dhdr");
+    return diff_run(ctx, base, 1, p, cam, 1, opts, nullptr, nullptr, true, pairing, grad, hdr_out, stats,
+                    "sing_diff_vjp", dl_dhdr);
 }
 
 int sing_diff_capture_pixels(SingContext* c, double* grad_out, uint32_t* meta_out) {

@@ -45,6 +45,10 @@ struct DiffBuffers {
     double* scratch;        // [C+1][5][P] recomputed segment states
     const float* target;    // [Pimg][3], shared by the evaluations
     const float* weight;    // [Pimg] or null
+    // [Pimg][3] or null: a caller's dLoss/dRadiance (sing_diff_vjp); the loss
+    // is then the pairing sum_p cot_p . I_p, whose gradient is the
+    // vector-Jacobian product, and target/weight are not read
+    const float* cot;
     std::uint32_t* list;    // [n_eval][Pimg] pixels of the FP64 pass (classified hard before the march)
     std::uint32_t* count;   // [n_eval] lengths of list
     std::uint32_t* list2;   // [n_eval][Pimg] pixels handed to FP64 after the FP32 march (grazing, near-hole emission)

@@ -240,6 +240,54 @@ def test_argument_errors(be, base, target):
         sa.diff_forward(be, base, sa.DiffParams.from_array([0.5, 1.0, 20.0, 10.0, 1.0]), cam(), opts())
 
 
+@pytest.mark.parametrize("prec", [sa.PREC_FP64, sa.PREC_MIXED])
+def test_vjp_with_the_l2_cotangent_is_the_loss_gradient(be, base, target, prec):
+    """sing_diff_vjp fed dLoss/dI = 2 (I - target) gives sing_diff_loss_grad's gradient
+    (the cotangent arrives as float: agreement to float rounding of the cotangent)."""
+    x = sa.DiffParams.from_array(START)
+    loss, g_ref, hdr, _ = sa.diff_loss_grad(be, base, x, cam(), target, None, opts(prec))
+    cot = (2.0 * (hdr.astype(np.float64) - target.astype(np.float64))).astype(np.float32)
+    print(f"diff_vjp {W}x{H} prec={prec} with the L2 cotangent", flush=True)
+    g, pairing, hdr2, st = sa.diff_vjp(be, base, x, cam(), cot, opts(prec))
+    assert np.array_equal(hdr2, hdr)
+    assert st.rays == W * H
+    print("vjp", g, "loss grad", g_ref, flush=True)
+    assert np.all(np.abs(g - g_ref) <= 1e-6 * np.abs(g_ref).max())
+    # the device pairs the cotangent with the radiance before its rounding to
+    # the float image: agreement to float rounding of I
+    prod = cot.astype(np.float64) * hdr.astype(np.float64)
+    assert abs(pairing - float(prod.sum())) <= 1e-6 * float(np.abs(prod).sum())
+
+
+def test_vjp_is_linear_in_the_cotangent(be, base):
+    rng = np.random.default_rng(20261204)
+    g1 = rng.standard_normal((H, W, 3)).astype(np.float32)
+    g2 = rng.standard_normal((H, W, 3)).astype(np.float32)
+    x = sa.DiffParams.from_array(START)
+    print(f"diff_vjp {W}x{H} linearity", flush=True)
+    v1, p1, _, _ = sa.diff_vjp(be, base, x, cam(), g1, opts())
+    v2, p2, _, _ = sa.diff_vjp(be, base, x, cam(), g2, opts())
+    v12, p12, _, _ = sa.diff_vjp(be, base, x, cam(), (0.75 * g1 - 1.5 * g2).astype(np.float32), opts())
+    comb = 0.75 * v1 - 1.5 * v2
+    assert np.all(np.abs(v12 - comb) <= 1e-6 * np.abs(comb).max())
+    assert abs(p12 - (0.75 * p1 - 1.5 * p2)) <= 1e-6 * max(abs(p1), abs(p2))
+    # a zero cotangent has a zero product
+    v0, p0, _, _ = sa.diff_vjp(be, base, x, cam(), np.zeros((H, W, 3), np.float32), opts())
+    assert np.all(v0 == 0.0) and p0 == 0.0
+
+
+def test_vjp_argument_errors(be, base):
+    x = sa.DiffParams.from_array(START)
+    with pytest.raises(ValueError):
+        sa.diff_vjp(be, base, x, cam(), np.zeros((H, W), np.float32))
+    L = sa.lib()
+    o = opts()
+    st = sa.DiffStats()
+    rc = L.sing_diff_vjp(be._ctx, C.byref(base), C.byref(x), C.byref(cam()), C.byref(o), None, None, None, None,
+                         C.byref(st))
+    assert rc == 1 and b"dl_dhdr" in L.sing_last_error(be._ctx)
+
+
 @pytest.mark.parametrize("prec", [sa.PREC_FP32, sa.PREC_FP64, sa.PREC_MIXED])
 def test_batch_equals_separate_evaluations(be, base, target, prec):
     """sing_diff_loss_grad_batch over several parameter points (one camera, then one

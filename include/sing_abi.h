@@ -172,7 +172,10 @@ typedef struct SingContext SingContext;
 SingContext* sing_create(uint32_t width, uint32_t height, int device);
 void sing_destroy(SingContext* ctx);
 int sing_resize(SingContext* ctx, uint32_t width, uint32_t height);
-/* Last error message for ctx (or the process-wide last error for ctx == NULL). */
+/* Last error message for ctx, or for ctx == NULL the calling thread's last
+ * error (each host thread has its own, like errno).  Contexts are independent:
+ * several host threads may each drive their own context at the same time; one
+ * context is used by one thread at a time. */
 const char* sing_last_error(const SingContext* ctx);
 const char* sing_backend_name(void); /* "HIP-gfx950" (RenderBackend::name) */
 int sing_abi_version(void);

@@ -12,7 +12,6 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
-#include <mutex>
 #include <string>
 #include <vector>
 
@@ -24,18 +23,17 @@
 
 namespace {
 
-std::mutex g_err_mu;
-std::string g_last_error;
+// The last error of the calling thread (as errno): contexts may be driven
+// from several host threads at once, and a pointer handed out by
+// sing_last_error(NULL) must not change under its reader.
+thread_local std::string g_last_error;
 
 // Device memory budget for one chunk of termination records (16 B per ray).
 constexpr std::size_t kRecordBudget = std::size_t(2) << 30;
 
 }  // namespace
 
-void sing_set_global_error(const std::string& m) {
-    std::lock_guard<std::mutex> lk(g_err_mu);
-    g_last_error = m;
-}
+void sing_set_global_error(const std::string& m) { g_last_error = m; }
 
 int SingContext::fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -196,7 +194,6 @@ const char* sing_backend_name(void) { return "HIP-gfx950"; }
 
 const char* sing_last_error(const SingContext* ctx) {
     if (ctx) return ctx->err.c_str();
-    std::lock_guard<std::mutex> lk(g_err_mu);
     return g_last_error.c_str();
 }
 

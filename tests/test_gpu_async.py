@@ -122,3 +122,53 @@ def test_host_is_free_while_the_gpu_renders():
         assert st2.rays == W * H * 64
     finally:
         b.shutdown()
+
+
+def test_contexts_driven_from_two_host_threads_at_once():
+    """Two contexts, each rendered by its own host thread at the same time (ctypes
+    releases the GIL during the calls): every frame equals the same frame rendered
+    alone, and a bad call on one thread leaves its error on that thread only."""
+    import threading
+
+    scenes = {
+        "kerr": _u(spp=16, w=640, h=480),
+        "schw": scene_uniforms(metric=sa.METRIC_SCHWARZSCHILD, spin=0.0, r_in=6.0, r_out=20.0, width=480, height=320,
+                               spp=8),
+    }
+    precs = {"kerr": sa.PREC_MIXED, "schw": sa.PREC_FP32}
+    refs = {}
+    for k, u in scenes.items():
+        b = sa.HipBackend(u.width, u.height)
+        b.render(u, sa.RenderOpts(precision=precs[k], outputs=OUTS))
+        refs[k] = b.capture(OUTS)
+        b.shutdown()
+    results, errors = {}, {}
+
+    def worker(k):
+        try:
+            u = scenes[k]
+            b = sa.HipBackend(u.width, u.height)
+            frames = []
+            for _ in range(3):
+                b.render(u, sa.RenderOpts(precision=precs[k], outputs=OUTS))
+                frames.append(b.capture(OUTS))
+            # an error on this thread: the thread's own last error names it
+            assert sa.lib().sing_create(0, 1, 0) is None
+            errors[k] = sa.lib().sing_last_error(None)
+            b.shutdown()
+            results[k] = frames
+        except Exception as exc:  # reported by the main thread
+            results[k] = exc
+
+    print("two host threads, two contexts", flush=True)
+    threads = [threading.Thread(target=worker, args=(k,)) for k in scenes]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    for k in scenes:
+        assert not isinstance(results.get(k), Exception), results.get(k)
+        assert len(results[k]) == 3
+        for f in results[k]:
+            _frames_equal(f, refs[k])
+        assert b"width/height" in errors[k]

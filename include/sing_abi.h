@@ -141,7 +141,10 @@ typedef struct SingRenderOpts {
                                another on one stream; 0 (default): a chunk's FP64 list pass and shade
                                run on a second stream beside the next chunk's set-up and FP32 pass
                                (same frame bits).  FP32 and FP64 frames always run serially. */
-    uint32_t reserved[4];
+    uint32_t chunk_samples; /* samples per chunk (0 = as many as fit 2 GiB of 16-byte ray records);
+                               smaller chunks hold less device memory and give a mixed frame more
+                               chunks to overlap; the frame bits do not depend on it */
+    uint32_t reserved[3];
 } SingRenderOpts;
 
 typedef struct SingRenderStats {

@@ -312,7 +312,7 @@ int sing_render_async(SingContext* c, const SingUniforms* u, const SingRenderOpt
     to.steps = (outs & SING_OUT_STEPS) ? c->d_steps : nullptr;
 
     const std::uint32_t spp = sf.spp;
-    const std::uint32_t S = c->chunk_samples(spp);
+    const std::uint32_t S = o.chunk_samples ? std::min(c->chunk_samples(spp), o.chunk_samples) : c->chunk_samples(spp);
     const std::uint32_t n_chunks = (spp + S - 1u) / S;
     const std::size_t P = std::size_t(c->width) * c->height;
     // Overlapped chunks (mixed form, two or more): chunk k's tail — the FP64

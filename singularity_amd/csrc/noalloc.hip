@@ -171,6 +171,7 @@ extern "C" int sing_render_noalloc(int device, const SingUniforms* u, const Sing
     std::size_t S = (kScratchBytes - fixed - slack) / per_sample;
     S = std::min<std::size_t>(S, spp);
     S = std::min<std::size_t>(S, (std::size_t(1) << 31) / P);  // record indices stay in u32
+    if (o.chunk_samples) S = std::min<std::size_t>(S, o.chunk_samples);
     S = std::max<std::size_t>(S, 1);
 
     hipError_t e = hipSetDevice(device);

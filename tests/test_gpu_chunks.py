@@ -59,3 +59,23 @@ def test_overlapped_chunks_equal_serial_chunks(be, name):
     # the per-pixel step totals add up to the frame's
     assert int(a.steps.astype(np.uint64).sum()) == a.stats.steps
     assert a.stats.kernel_ms > 0.0 and a.stats.trace32_ms + a.stats.trace64_ms > 0.0
+
+
+@pytest.mark.parametrize("prec", [sa.PREC_FP32, sa.PREC_MIXED])
+def test_chunk_samples_option_does_not_change_the_frame(prec):
+    """SingRenderOpts.chunk_samples caps the samples per chunk (less record memory,
+    more chunks): 40 spp in chunks of 7 (six chunks, the last of 5) equals the
+    one-chunk frame bit for bit."""
+    u = scene_uniforms(metric=sa.METRIC_KERR, spin=0.94, r_in=1.7, r_out=30.0, width=256, height=192, spp=40)
+    b = sa.HipBackend(256, 192)
+    try:
+        print(f"chunk_samples 7 vs default, prec={prec}", flush=True)
+        a = b.render_outputs(u, OUTS, prec)
+        c = b.render_outputs(u, OUTS, prec, chunk_samples=7)
+        assert a.stats.chunks == 1 and c.stats.chunks == 6
+        assert (a.stats.rays, a.stats.steps, a.stats.fp64_rays) == (c.stats.rays, c.stats.steps, c.stats.fp64_rays)
+        assert np.array_equal(a.rgba, c.rgba)
+        assert np.array_equal(a.hdr.view(np.uint32), c.hdr.view(np.uint32))
+        assert np.array_equal(a.steps, c.steps)
+    finally:
+        b.shutdown()

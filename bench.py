@@ -438,6 +438,13 @@ def main() -> int:
     steps32, steps64 = int(stats[-1].steps32), int(stats[-1].steps64)
     achieved = FL.KERR_STEP_FLOPS * steps32 / (t32 * 1e-3) / 1e12
     achieved64 = FL.KERR_STEP_FLOPS * steps64 / (t64 * 1e-3) / 1e12 if t64 > 0 else 0.0
+    # VALU issue of each pass per launch: its steps x instructions per step (SQ
+    # counters) over its launch time (HIP events), in wave64 instructions per CU
+    # per clock
+    issue32 = FL.valu_issue_per_cu_clk(steps32 / chunks, t32 / chunks * 1e-3, FL.KERR_STEP_VALU_INSTS,
+                                       FL.KERR_STEP_LANES)
+    issue64 = (FL.valu_issue_per_cu_clk(steps64 / chunks, t64 / chunks * 1e-3, FL.KERR_STEP_VALU_INSTS_FP64,
+                                        FL.KERR_STEP_LANES_FP64) if t64 > 0 else 0.0)
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
@@ -460,7 +467,8 @@ def main() -> int:
             "sincos_per_step": FL.KERR_STEP_TRIG, "launches_per_frame": chunks,
             "steps_per_launch": round(steps32 / chunks), "launch_ms": round(t32 / chunks, 3),
             "valu_insts_per_step": FL.KERR_STEP_VALU_INSTS,
-            "valu_issue_frac": round(FL.VALU_ISSUE_PER_CU_CLK / FL.VALU_ISSUE_PEAK_PER_CU_CLK, 3),
+            "valu_issue_per_cu_clk": round(issue32, 3),
+            "valu_issue_frac": round(issue32 / FL.VALU_ISSUE_PEAK_PER_CU_CLK, 3),
         },
         "roofline_fp64": {
             "bound": "valu", "achieved": round(achieved64, 3), "peak": FL.PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
@@ -468,6 +476,9 @@ def main() -> int:
             "kernel": "trace_kernel<1, 3> (FP64 pass over the rays routed to it)",
             "flops_per_step": FL.KERR_STEP_FLOPS, "steps_per_launch": round(steps64 / chunks),
             "launch_ms": round(t64 / chunks, 3),
+            "valu_insts_per_step": FL.KERR_STEP_VALU_INSTS_FP64,
+            "valu_issue_per_cu_clk": round(issue64, 3),
+            "valu_issue_frac": round(issue64 / FL.VALU_ISSUE_PEAK_FP64_PER_CU_CLK, 3),
         },
         "kernel_ms_per_frame": round(kern_ms, 2), "trace_ms_per_frame": round(trace_ms, 2),
         "shade_ms_per_frame": round(float(np.mean([s.shade_ms for s in stats])), 2),

@@ -79,3 +79,20 @@ def test_chunk_samples_option_does_not_change_the_frame(prec):
         assert np.array_equal(a.steps, c.steps)
     finally:
         b.shutdown()
+
+
+def test_grid_sampler_across_chunks_equals_one_chunk():
+    """The grid sampler's tone map of the mean radiance happens in the last chunk's
+    shade: 16 grid samples in chunks of 5 (four chunks, mixed, overlapped) equal the
+    one-chunk frame."""
+    u = scene_uniforms(metric=sa.METRIC_KERR, spin=0.94, r_in=1.7, r_out=30.0, width=200, height=150, spp=16)
+    b = sa.HipBackend(200, 150)
+    try:
+        print("grid sampler, chunk_samples 5 vs default", flush=True)
+        a = b.render_outputs(u, OUTS, sa.PREC_MIXED, sampler=sa.SAMPLER_GRID)
+        c = b.render_outputs(u, OUTS, sa.PREC_MIXED, sampler=sa.SAMPLER_GRID, chunk_samples=5)
+        assert a.stats.chunks == 1 and c.stats.chunks == 4
+        assert np.array_equal(a.rgba, c.rgba)
+        assert np.array_equal(a.hdr.view(np.uint32), c.hdr.view(np.uint32))
+    finally:
+        b.shutdown()

@@ -137,10 +137,11 @@ typedef struct SingRenderOpts {
     uint32_t waves_per_cu;/* persistent grid size in waves per CU (0 = tuned default) */
     uint32_t sampler;     /* SING_SAMPLER_* */
     uint32_t refine;      /* SING_REFINE_* mask; Kerr with the fixed step (adapt_tol = 0) only */
-    uint32_t serial_chunks; /* 1: a mixed-precision frame of several sample chunks runs them one after
-                               another on one stream; 0 (default): a chunk's FP64 list pass and shade
-                               run on a second stream beside the next chunk's set-up and FP32 pass
-                               (same frame bits).  FP32 and FP64 frames always run serially. */
+    uint32_t overlap_chunks; /* 1: a mixed-precision frame of several sample chunks runs each chunk's
+                                FP64 list pass and shade on a second stream beside the next chunk's
+                                set-up and FP32 pass (same frame bits); 0 (default): chunk after chunk
+                                on one stream, which with the FP64 list's longest-first order is as
+                                fast or faster.  FP32 and FP64 frames always run serially. */
     uint32_t chunk_samples; /* samples per chunk (0 = as many as fit 2 GiB of 16-byte ray records);
                                smaller chunks hold less device memory and give a mixed frame more
                                chunks to overlap; the frame bits do not depend on it */
@@ -150,7 +151,7 @@ typedef struct SingRenderOpts {
 typedef struct SingRenderStats {
     double kernel_ms;     /* the frame on the GPU, first kernel start to last kernel end (HIP events) */
     double trace_ms;      /* trace kernels (FP32 pass and FP64 pass), summed over the chunks; with
-                             overlapped chunks a chunk's FP64 pass and shade share the time of the
+                             overlap_chunks a chunk's FP64 pass and shade share the time of the
                              next chunk's FP32 pass, so the parts can add up to more than kernel_ms */
     double shade_ms;      /* shade/accumulate kernels */
     double post_ms;       /* post-pass kernels (0 when no post pass) */

@@ -70,7 +70,7 @@ class RenderOpts(C.Structure):
         ("precision", C.c_uint32), ("outputs", C.c_uint32), ("post", C.c_uint32), ("turb_seed", C.c_uint32),
         ("adapt_tol", C.c_float), ("band_width", C.c_float), ("peak_T_scale", C.c_float),
         ("step_batch", C.c_uint32), ("waves_per_cu", C.c_uint32), ("sampler", C.c_uint32),
-        ("refine", C.c_uint32), ("serial_chunks", C.c_uint32), ("chunk_samples", C.c_uint32),
+        ("refine", C.c_uint32), ("overlap_chunks", C.c_uint32), ("chunk_samples", C.c_uint32),
         ("reserved", C.c_uint32 * 3),
     ]
 

@@ -315,7 +315,7 @@ int sing_render_async(SingContext* c, const SingUniforms* u, const SingRenderOpt
     const std::uint32_t S = o.chunk_samples ? std::min(c->chunk_samples(spp), o.chunk_samples) : c->chunk_samples(spp);
     const std::uint32_t n_chunks = (spp + S - 1u) / S;
     const std::size_t P = std::size_t(c->width) * c->height;
-    // Overlapped chunks (mixed form, two or more): chunk k's tail — the FP64
+    // Overlapped chunks (overlap_chunks, mixed form, two or more): chunk k's tail — the FP64
     // list pass, then its shade — runs on the second stream while the context
     // stream goes on with chunk k+1's set-up and FP32 pass, so the list pass's
     // long rays and the shade share the GPU with FP32 work instead of leaving
@@ -324,10 +324,13 @@ int sing_render_async(SingContext* c, const SingUniforms* u, const SingRenderOpt
     // chunk k's shade before it reuses them.  Shades stay in chunk order on
     // one stream and every ray's arithmetic is unchanged: the frame is
     // bit-identical to the serial order.
-    // Only the mixed form overlaps: a plain FP32 frame gains nothing (0.0 % on
-    // the 4K headline) and an all-FP64 frame loses 3.7 % when its shades run
-    // beside the next chunk's FP64 pass; the mixed headline gains 1.2 %.
-    const bool overlap = n_chunks >= 2 && !o.serial_chunks && o.precision == SING_PREC_MIXED;
+    // On request only, and for the mixed form only: a plain FP32 frame gains
+    // nothing (0.0 % on the 4K headline) and an all-FP64 frame loses 3.7 % when
+    // its shades run beside the next chunk's FP64 pass; the mixed form gained
+    // 1.2 % on the headline while its FP64 list ran in record order, and with
+    // the list's longest-first order the serial schedule is as fast there and
+    // 1.6 % faster at a = 0.998 (overlapped, -0 % and +2 %).
+    const bool overlap = n_chunks >= 2 && o.overlap_chunks && o.precision == SING_PREC_MIXED;
     const std::uint32_t nbuf = overlap ? 2u : 1u;
     hipStream_t sa = c->stream, sb = overlap ? c->stream2 : c->stream;
     int rc = c->ensure_records(std::size_t(nbuf) * S * P);

@@ -774,4 +774,24 @@ __device__ __forceinline__ bool hard_ray(R xi, R eta, R a, R M, R r_plus, bool i
     return R(1) - umax < R(8.9973e-4);  // sin^2(0.03)
 }
 
+// hard_ray's verdict with the kind of ray: 0 not hard; 1 close to a photon
+// orbit, vortical or reaching the polar axis (long rays: they wind round the
+// hole or are slowed by the pole damping, ~600 steps); 2 only ingoing and able
+// to reach 2 r+ (short rays: captured, or meeting the disc near the hole,
+// ~70 steps).  The FP64 pass takes the long ones first, so that its last rays
+// are short ones and its waves end together.
+template <class R>
+__device__ __forceinline__ std::uint32_t hard_ray_kind(R xi, R eta, R a, R M, R r_plus, bool ingoing, R band) {
+    const R m = photon_margin<R>(xi, eta, a, M, r_plus);
+    if (m >= R(0) && m < band) return 1u;
+    if (eta < R(0)) return 1u;
+    const R B = eta + xi * xi - a * a;
+    const R umax = eta > R(0) ? R(2) * eta / (B + dsqrt(B * B + R(4) * a * a * eta)) : R(0);
+    if (R(1) - umax < R(8.9973e-4)) return 1u;
+    const R C = (xi - a) * (xi - a) + eta;
+    const R r1 = R(2) * r_plus, r2 = r1 * r1;
+    const R Rv = r2 * r2 + (a * a - xi * xi - eta) * r2 + R(2) * M * C * r1 - a * a * eta;
+    return (ingoing && Rv >= R(0)) ? 2u : 0u;
+}
+
 }  // namespace sing

@@ -56,8 +56,11 @@ void trace_kernel(SceneConst<float> sf, SceneConst<double> sd,
     using S = typename std::conditional<F64, double, float>::type;
     const int lane = __lane_id();
     const std::uint32_t W = sf.width, H = sf.height, spp = sf.spp;
+    // the FP64 list: band_count entries at the front, band_back at the end of
+    // the chunk's S * P slots (filled backwards), taken front first
+    const std::uint64_t n_front = (PASS == 3) ? std::uint64_t(ctl->band_count) : 0u;
     const std::uint64_t n_items =
-        (PASS == 3) ? std::uint64_t(ctl->band_count) : std::uint64_t(ch.S) * ch.n_tiles * 64u;
+        (PASS == 3) ? n_front + ctl->band_back : std::uint64_t(ch.S) * ch.n_tiles * 64u;
     std::uint64_t* counter = (PASS == 3) ? &ctl->next_band : &ctl->next_item;
 
     bool retired = false, live = false, need = true;
@@ -110,7 +113,7 @@ void trace_kernel(SceneConst<float> sf, SceneConst<double> sd,
                         std::uint32_t gx, gy, s;
                         bool ok;
                         if (PASS == 3) {
-                            rec_idx = ch.band[item];
+                            rec_idx = ch.band[item < n_front ? item : std::uint64_t(ch.S) * ch.P - 1u - (item - n_front)];
                             const std::uint32_t sl = rec_idx / ch.P, pix = rec_idx - sl * ch.P;
                             gx = pix % W;
                             gy = pix / W;
@@ -209,7 +212,7 @@ __global__ __launch_bounds__(256) void setup_kernel(SceneConst<float> sf, const 
     const std::uint64_t first_lane = start - std::uint64_t(lane);
     for (std::uint64_t base = first_lane; base < n_items; base += stride) {
         const std::uint64_t item = base + std::uint64_t(lane);
-        std::uint32_t out = kItemSkip, rec_idx = 0, gx = 0, gy = 0, s = 0;
+        std::uint32_t out = kItemSkip, rec_idx = 0, gx = 0, gy = 0, s = 0, kind = 0;
         bool hard = false;
         if (item < n_items && item_to_ray(item, ch, sf.width, sf.height, rec_idx, gx, gy, s)) {
             float ox = 0.5f, oy = 0.5f;
@@ -219,7 +222,8 @@ __global__ __launch_bounds__(256) void setup_kernel(SceneConst<float> sf, const 
                 oy = hv.y;
             }
             RayState<float> y{};
-            const bool started = init_ray<float, float, KERR>(y, sf, gx, gy, ox, oy, PASS == 1 ? &hard : nullptr);
+            const bool started =
+                init_ray<float, float, KERR>(y, sf, gx, gy, ox, oy, PASS == 1 ? &hard : nullptr, &kind);
             if (!started) {
                 Rec rc;
                 rc.a = rc.b = rc.c = 0.0f;
@@ -233,17 +237,26 @@ __global__ __launch_bounds__(256) void setup_kernel(SceneConst<float> sf, const 
             }
         }
         if (PASS == 1) {
-            const std::uint64_t hw = __ballot(hard);
-            if (hw) {
-                const int leader = __ffsll((unsigned long long)hw) - 1;
+            // long hard rays at the front of the FP64 list, short ones from its
+            // end backwards (the FP64 pass takes the front first)
+            const bool front = hard && kind != 2u, back = hard && kind == 2u;
+            const std::uint64_t hf = __ballot(front), hb = __ballot(back);
+            const std::uint64_t below = (1ull << lane) - 1ull;
+            if (hf) {
+                const int leader = __ffsll((unsigned long long)hf) - 1;
                 std::uint32_t slot = 0;
-                if (lane == leader) slot = atomicAdd(&ctl->band_count, std::uint32_t(__popcll(hw)));
+                if (lane == leader) slot = atomicAdd(&ctl->band_count, std::uint32_t(__popcll(hf)));
                 slot = __shfl(slot, leader);
-                if (hard) {
-                    ch.band[slot + __popcll(hw & ((1ull << lane) - 1ull))] = rec_idx;
-                    ch.rec[rec_idx] = deferred_marker();
-                }
+                if (front) ch.band[slot + __popcll(hf & below)] = rec_idx;
             }
+            if (hb) {
+                const int leader = __ffsll((unsigned long long)hb) - 1;
+                std::uint32_t slot = 0;
+                if (lane == leader) slot = atomicAdd(&ctl->band_back, std::uint32_t(__popcll(hb)));
+                slot = __shfl(slot, leader);
+                if (back) ch.band[ch.S * ch.P - 1u - (slot + __popcll(hb & below))] = rec_idx;
+            }
+            if (hard) ch.rec[rec_idx] = deferred_marker();
         }
         if (item < n_items) ch.item_rec[item] = out;
     }

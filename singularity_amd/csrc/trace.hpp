@@ -48,8 +48,8 @@ struct PixAcc {
 struct QueueCtl {
     std::uint64_t next_item;   // trace queue
     std::uint64_t next_band;   // FP64 pass queue
-    std::uint32_t band_count;  // deferred records appended by the FP32 pass
-    std::uint32_t pad;
+    std::uint32_t band_count;  // deferred records at the front of the FP64 list (long hard rays, FP32-pass deferrals)
+    std::uint32_t band_back;   // short hard rays, at the back of the list (filled from its end)
     std::uint64_t steps;
     std::uint64_t rays;
     std::uint64_t rays64;

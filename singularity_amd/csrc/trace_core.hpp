@@ -255,7 +255,7 @@ __device__ __forceinline__ bool needs_fp64(const RayState<S>& y, std::uint32_t) 
 // accurate enough for it (hard_ray).
 template <class R, class S, bool KERR>
 __device__ __forceinline__ bool init_ray(RayState<S>& y, const SceneConst<R>& sc, std::uint32_t gx, std::uint32_t gy,
-                                         R ox, R oy, bool* hard_out) {
+                                         R ox, R oy, bool* hard_out, std::uint32_t* kind_out = nullptr) {
     R ur, uth, uph;
     camera_ray(sc, gx, gy, ox, oy, ur, uth, uph);
     y.r = S(sc.cam_r);
@@ -284,7 +284,12 @@ __device__ __forceinline__ bool init_ray(RayState<S>& y, const SceneConst<R>& sc
             const R Q = pt * pt + sc.q_cos2 * (L * L * sc.q_inv_s2 - sc.a2 * E * E);
             const R rp = sc.M + dsqrt(dmax(sc.M * sc.M - sc.a2, R(0)));
             const R iE = R(1) / E;
-            *hard_out = hard_ray<R>(L * iE, Q * iE * iE, sc.a, sc.M, rp, ur < R(0), sc.band);
+            if (kind_out) {
+                *kind_out = hard_ray_kind<R>(L * iE, Q * iE * iE, sc.a, sc.M, rp, ur < R(0), sc.band);
+                *hard_out = *kind_out != 0u;
+            } else {
+                *hard_out = hard_ray<R>(L * iE, Q * iE * iE, sc.a, sc.M, rp, ur < R(0), sc.band);
+            }
         }
     } else {
         const R cr = sc.cam_r;
@@ -301,7 +306,12 @@ __device__ __forceinline__ bool init_ray(RayState<S>& y, const SceneConst<R>& sc
             const R L2 = cr * cr * cr * cr * (uth * uth + sc.sin_th * sc.sin_th * uph * uph);
             const R Lz = cr * cr * sc.sin_th * sc.sin_th * uph;
             const R iE = R(1) / E;
-            *hard_out = hard_ray<R>(Lz * iE, (L2 - Lz * Lz) * iE * iE, R(0), sc.M, sc.twoM, ur < R(0), sc.band);
+            if (kind_out) {
+                *kind_out = hard_ray_kind<R>(Lz * iE, (L2 - Lz * Lz) * iE * iE, R(0), sc.M, sc.twoM, ur < R(0), sc.band);
+                *hard_out = *kind_out != 0u;
+            } else {
+                *hard_out = hard_ray<R>(Lz * iE, (L2 - Lz * Lz) * iE * iE, R(0), sc.M, sc.twoM, ur < R(0), sc.band);
+            }
         }
     }
     return true;

@@ -1,8 +1,8 @@
-"""Frames of several sample chunks: the mixed form's overlapped schedule (a
-chunk's FP64 list pass and shade on a second stream beside the next chunk's
-set-up and FP32 pass, chunk buffers used by parity) against the serial one
-(SingRenderOpts.serial_chunks = 1), bit for bit; FP32 and FP64 frames (always
-serial) give the same bits either way.
+"""Frames of several sample chunks: the mixed form's overlapped schedule
+(SingRenderOpts.overlap_chunks = 1: a chunk's FP64 list pass and shade on a
+second stream beside the next chunk's set-up and FP32 pass, chunk buffers used
+by parity) against the default serial one, bit for bit; FP32 and FP64 frames
+(always serial) give the same bits either way.
 
 A chunk holds at most 2 GiB of 16-byte ray records, so 1024x768 at 400 spp is
 three chunks (170, 170 and 60 samples): the third reuses the first one's
@@ -43,9 +43,9 @@ def test_overlapped_chunks_equal_serial_chunks(be, name):
     kw = dict(CASES[name])
     prec = kw.pop("precision")
     print(f"chunks {name}: Kerr a=0.94 1024x768 400 spp, overlapped", flush=True)
-    a = be.render_outputs(u, OUTS, prec, **kw)
+    a = be.render_outputs(u, OUTS, prec, overlap_chunks=1, **kw)
     print(f"chunks {name}: serial", flush=True)
-    b = be.render_outputs(u, OUTS, prec, serial_chunks=1, **kw)
+    b = be.render_outputs(u, OUTS, prec, **kw)
     assert a.stats.chunks == 3 and b.stats.chunks == 3
     n = 1024 * 768 * 400
     assert a.stats.rays == b.stats.rays == n
@@ -72,6 +72,8 @@ def test_chunk_samples_option_does_not_change_the_frame(prec):
         print(f"chunk_samples 7 vs default, prec={prec}", flush=True)
         a = b.render_outputs(u, OUTS, prec)
         c = b.render_outputs(u, OUTS, prec, chunk_samples=7)
+        d = b.render_outputs(u, OUTS, prec, chunk_samples=7, overlap_chunks=1)
+        assert np.array_equal(d.rgba, c.rgba) and np.array_equal(d.hdr.view(np.uint32), c.hdr.view(np.uint32))
         assert a.stats.chunks == 1 and c.stats.chunks == 6
         assert (a.stats.rays, a.stats.steps, a.stats.fp64_rays) == (c.stats.rays, c.stats.steps, c.stats.fp64_rays)
         assert np.array_equal(a.rgba, c.rgba)
@@ -90,7 +92,7 @@ def test_grid_sampler_across_chunks_equals_one_chunk():
     try:
         print("grid sampler, chunk_samples 5 vs default", flush=True)
         a = b.render_outputs(u, OUTS, sa.PREC_MIXED, sampler=sa.SAMPLER_GRID)
-        c = b.render_outputs(u, OUTS, sa.PREC_MIXED, sampler=sa.SAMPLER_GRID, chunk_samples=5)
+        c = b.render_outputs(u, OUTS, sa.PREC_MIXED, sampler=sa.SAMPLER_GRID, chunk_samples=5, overlap_chunks=1)
         assert a.stats.chunks == 1 and c.stats.chunks == 4
         assert np.array_equal(a.rgba, c.rgba)
         assert np.array_equal(a.hdr.view(np.uint32), c.hdr.view(np.uint32))

@@ -123,8 +123,9 @@ void trace_batch_kernel(const BatchScene* __restrict__ scenes, BatchLaunch L,
         if (!dry) {
             std::uint64_t want = __ballot(need);
             if (want && (std::uint32_t(__popcll(want)) >= claim.rays_per_refill || __ballot(live) == 0ull)) {
+                const std::uint64_t n_front = (PASS == 3) ? std::uint64_t(q.band_count) : 0u;
                 const std::uint64_t n_items =
-                    (PASS == 3) ? std::uint64_t(q.band_count) : item_limit(bs.n_items, sample_div);
+                    (PASS == 3) ? n_front + q.band_back : item_limit(bs.n_items, sample_div);
                 std::uint64_t* counter = (PASS == 3) ? &q.next_band : &q.next_item;
                 for (int round = 0; round < 2 && want != 0ull && !dry; ++round) {
                     const std::uint32_t k = std::uint32_t(__popcll(want));
@@ -161,7 +162,11 @@ void trace_batch_kernel(const BatchScene* __restrict__ scenes, BatchLaunch L,
                             std::uint32_t gx, gy, s;
                             bool ok;
                             if (PASS == 3) {
-                                rec_idx = L.band[bs.rec_base + item];
+                                // the scene's FP64 list: long rays at the front, short
+                                // ones at the end of its P * spp slots, front first
+                                rec_idx = L.band[bs.rec_base + (item < n_front ? item
+                                                                                 : std::uint64_t(bs.P) * bs.spp - 1u -
+                                                                                       (item - n_front))];
                                 const std::uint32_t sl = rec_idx / bs.P, pix = rec_idx - sl * bs.P;
                                 gx = pix % W;
                                 gy = pix / W;
@@ -184,19 +189,25 @@ void trace_batch_kernel(const BatchScene* __restrict__ scenes, BatchLaunch L,
                                     oy = hv.y;
                                 }
                                 bool started, deferred = false;
+                                std::uint32_t kind = 0;
                                 if (F64) {
                                     started = init_ray<double, S, KERR>(y, bs.sd, gx, gy, double(ox), double(oy),
                                                                         (bool*)nullptr);
                                 } else if (PASS == 1) {
                                     bool hard = false;
-                                    started = init_ray<float, S, KERR>(y, bs.sf, gx, gy, ox, oy, &hard);
+                                    started = init_ray<float, S, KERR>(y, bs.sf, gx, gy, ox, oy, &hard, &kind);
                                     deferred = started && hard;
                                 } else {
                                     started = init_ray<float, S, KERR>(y, bs.sf, gx, gy, ox, oy, (bool*)nullptr);
                                 }
                                 if (deferred) {
-                                    const std::uint32_t slot = atomicAdd(&q.band_count, 1u);
-                                    L.band[bs.rec_base + slot] = rec_idx;
+                                    if (kind == 2u) {  // short: from the end of the list
+                                        const std::uint32_t slot = atomicAdd(&q.band_back, 1u);
+                                        L.band[bs.rec_base + std::uint64_t(bs.P) * bs.spp - 1u - slot] = rec_idx;
+                                    } else {
+                                        const std::uint32_t slot = atomicAdd(&q.band_count, 1u);
+                                        L.band[bs.rec_base + slot] = rec_idx;
+                                    }
                                     L.rec[bs.rec_base + rec_idx] = deferred_marker();
                                 } else if (!started) {
                                     Rec rc;
@@ -222,8 +233,9 @@ void trace_batch_kernel(const BatchScene* __restrict__ scenes, BatchLaunch L,
             for (std::uint32_t j = 1; j <= count; ++j) {
                 const std::uint32_t s2 = first + (sid - first + j) % count;
                 const SceneQueue& q2 = L.queues[s2];
-                const std::uint64_t lim = (PASS == 3) ? std::uint64_t(load_relaxed(&q2.band_count))
-                                                      : item_limit(scenes[s2].n_items, sample_div);
+                const std::uint64_t lim =
+                    (PASS == 3) ? std::uint64_t(load_relaxed(&q2.band_count)) + load_relaxed(&q2.band_back)
+                                : item_limit(scenes[s2].n_items, sample_div);
                 const std::uint64_t cur = load_relaxed((PASS == 3) ? &q2.next_band : &q2.next_item);
                 if (__builtin_amdgcn_readfirstlane(cur < lim ? 1u : 0u)) {
                     sid = __builtin_amdgcn_readfirstlane(s2);
@@ -622,16 +634,19 @@ extern "C" int sing_render_batch(SingContext* c, const SingBatchItem* items, uin
             if (!explicit_qc) tu.queue_chunk = kv.second.queue_chunk;
         }
     if (o.autotune && (!o.step_batch || !o.waves_per_cu || !explicit_rpr || !explicit_qc)) {
-        // coordinate search on the first 1/16 of the first group's work items:
+        // coordinate search on the first 1/8 of the first group's work items:
         // waves per CU, step batch, rays per refill, queue chunk, each with the
-        // best of the knobs before it
+        // best of the knobs before it.  A candidate replaces the best only when
+        // it is at least 1 % faster: differences below that are the sample
+        // launches' own spread, and following them picked shapes up to 3 %
+        // slower on the full batch.
         auto time_shape = [&](const sing::BatchTune& t, float* ms) -> int {
             const sing::LaunchShape sh = shape_of(c, t.step_batch, t.waves_per_cu);
             const sing::BatchClaim cl{t.rays_per_refill, t.queue_chunk};
             hipError_t r = sing::launch_batch_reset(L, groups[0], true, c->stream);
             if (r == hipSuccess) r = hipEventRecord(c->events[0], c->stream);
             if (r == hipSuccess)
-                r = sing::launch_batch_trace(L, groups[0], o.precision, adaptive, sh, cl, 16u, false, c->stream);
+                r = sing::launch_batch_trace(L, groups[0], o.precision, adaptive, sh, cl, 8u, false, c->stream);
             if (r == hipSuccess) r = hipEventRecord(c->events[1], c->stream);
             if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
             if (r == hipSuccess) r = hipEventElapsedTime(ms, c->events[0], c->events[1]);
@@ -649,7 +664,7 @@ extern "C" int sing_render_batch(SingContext* c, const SingBatchItem* items, uin
                 if (t.*field == best_t.*field) continue;
                 const int r = time_shape(t, &ms);
                 if (r != SING_OK) return r;
-                if (ms < best) best = ms, best_t = t;
+                if (ms < 0.99f * best) best = ms, best_t = t;
             }
             return SING_OK;
         };

@@ -30,8 +30,8 @@ struct BatchScene {
 struct SceneQueue {
     std::uint64_t next_item;   // FP32 (or FP64) trace queue
     std::uint64_t next_band;   // FP64 pass queue
-    std::uint32_t band_count;  // deferred records of the scene
-    std::uint32_t pad;
+    std::uint32_t band_count;  // deferred records at the front of the scene's FP64 list
+    std::uint32_t band_back;   // short hard rays at its end (filled backwards; taken after the front)
 };
 
 struct BatchOutputs {

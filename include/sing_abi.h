@@ -347,7 +347,10 @@ typedef struct SingDiffOpts {
     double soft_edge;          /* annulus edge ramp width in M (0 = 1.0) */
     uint32_t r_in_isco;        /* 1: r_in = prograde ISCO of a/M (p->r_in ignored); grad[0] is then the
                                   total derivative through r_in(a), grad[2] the partial dL/d(r_in/M) */
-    uint32_t reserved[5];
+    uint32_t target_per_eval;  /* sing_diff_loss_grad_batch: 1 = target (and weights) hold one image per
+                                  parameter point, n of them one after another (with one camera per point:
+                                  several views of one scene); 0 = one image shared by all points */
+    uint32_t reserved[4];
 } SingDiffOpts;
 
 typedef struct SingDiffStats {
@@ -372,8 +375,12 @@ int sing_diff_loss_grad(SingContext* ctx, const SingUniforms* base, const SingDi
 /* n parameter points (1..SING_DIFF_MAX_BATCH) of the same image in one set of
  * launches: the losses (loss[n]) and gradients (grad[n*5]) that n calls of
  * sing_diff_loss_grad would return, bit for bit.  cams: one camera for all
- * (n_cams = 1) or one per point (n_cams = n); hdr_out (optional) n*W*H*3
- * floats, image after image.  One 512x512 evaluation leaves most lanes idle
+ * (n_cams = 1) or one per point (n_cams = n); target W*H*3 floats (weights
+ * W*H or NULL) shared by all points, or with opts->target_per_eval = 1 one
+ * image per point, n of them one after another (several views of one scene:
+ * one camera and one observed image per point, the fit's loss their sum);
+ * hdr_out (optional) n*W*H*3 floats, image after image.  One 512x512
+ * evaluation leaves most lanes idle
  * (its time is its longest ray's), so independent evaluations — finite-
  * difference stencils, multi-start or population fits, several cameras —
  * share the launches.  No counterpart in the reference. */

@@ -379,7 +379,7 @@ class DiffCamera(C.Structure):
 
 class DiffOpts(C.Structure):
     _fields_ = [("precision", C.c_uint32), ("checkpoint_every", C.c_uint32), ("soft_edge", C.c_double),
-                ("r_in_isco", C.c_uint32), ("reserved", C.c_uint32 * 5)]
+                ("r_in_isco", C.c_uint32), ("target_per_eval", C.c_uint32), ("reserved", C.c_uint32 * 4)]
 
 
 class DiffStats(C.Structure):
@@ -483,25 +483,30 @@ def diff_vjp(be: "HipBackend", base: Uniforms, params: DiffParams, cam: DiffCame
 def diff_loss_grad_batch(be: "HipBackend", base: Uniforms, params, cams, target: np.ndarray,
                          weights: np.ndarray | None = None, opts: DiffOpts | None = None, want_hdr: bool = False):
     """sing_diff_loss_grad_batch: n parameter points (a sequence of DiffParams or an (n, 5)
-    array) in one set of launches; cams is one DiffCamera or one per point.  Returns
-    (losses (n,), gradients (n, 5), radiance (n, H, W, 3) or None, stats)."""
+    array) in one set of launches; cams is one DiffCamera or one per point; target is one
+    (H, W, 3) image for all points or one per point, (n, H, W, 3) (weights likewise (H, W)
+    or (n, H, W)).  Returns (losses (n,), gradients (n, 5), radiance (n, H, W, 3) or None,
+    stats)."""
     plist = [p if isinstance(p, DiffParams) else DiffParams.from_array(p) for p in params]
     n = len(plist)
     clist = [cams] if isinstance(cams, DiffCamera) else list(cams)
     if n < 1 or n > DIFF_MAX_BATCH or len(clist) not in (1, n):
         raise ValueError(f"{n} parameter points (1..{DIFF_MAX_BATCH}) and {len(clist)} cameras (1 or n)")
     t = np.ascontiguousarray(target, dtype=np.float32)
-    if t.shape != (be.height, be.width, 3):
-        raise ValueError(f"target shape {t.shape} != {(be.height, be.width, 3)}")
+    per_eval = t.ndim == 4
+    img = (be.height, be.width)
+    if t.shape != ((n,) + img + (3,) if per_eval else img + (3,)):
+        raise ValueError(f"target shape {t.shape} != {img + (3,)} or {(n,) + img + (3,)}")
     w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
-    if w is not None and w.shape != (be.height, be.width):
-        raise ValueError(f"weights shape {w.shape} != {(be.height, be.width)}")
+    if w is not None and w.shape != ((n,) + img if per_eval else img):
+        raise ValueError(f"weights shape {w.shape} != {(n,) + img if per_eval else img}")
     parr = (DiffParams * n)(*plist)
     carr = (DiffCamera * len(clist))(*clist)
     loss = np.zeros(n, np.float64)
     grad = np.zeros((n, 5), np.float64)
     hdr = np.empty((n, be.height, be.width, 3), np.float32) if want_hdr else None
-    o = opts if opts is not None else DiffOpts()
+    o = DiffOpts.from_buffer_copy(bytes(opts)) if opts is not None else DiffOpts()
+    o.target_per_eval = 1 if per_eval else 0
     st = DiffStats()
     be._check(lib().sing_diff_loss_grad_batch(be._ctx, C.byref(base), n, C.addressof(parr), C.addressof(carr),
                                               len(clist), C.byref(o), t.ctypes.data,
@@ -522,7 +527,15 @@ def adam_fit(be: "HipBackend", base: Uniforms, start, target: np.ndarray, cam: D
     such spike would otherwise inflate Adam's second moment for hundreds of
     steps.  Returns (the parameters after the last update, history of (loss,
     params, grad) of every evaluated iterate).
+
+    Several views: target (n, H, W, 3) with cam a sequence of n cameras fits one
+    parameter point to all of them (loss and gradient summed over the views,
+    evaluated in one sing_diff_loss_grad_batch call per step).
     """
+    target = np.asarray(target)
+    views = target.ndim == 4
+    if views and (isinstance(cam, DiffCamera) or len(cam) != target.shape[0]):
+        raise ValueError("several target views need one camera per view")
     x = np.array(start, np.float64)
     lr = np.array([0.03, 0.015, 0.075, 0.3, 0.015]) if lr is None else np.asarray(lr, np.float64)
     tied = opts is not None and opts.r_in_isco
@@ -533,7 +546,11 @@ def adam_fit(be: "HipBackend", base: Uniforms, start, target: np.ndarray, cam: D
     v = np.zeros(5)
     history = []
     for k in range(1, steps + 1):
-        loss, g, _, _ = diff_loss_grad(be, base, DiffParams.from_array(x), cam, target, None, opts)
+        if views:
+            losses, grads, _, _ = diff_loss_grad_batch(be, base, [x] * target.shape[0], list(cam), target, None, opts)
+            loss, g = float(losses.sum()), grads.sum(axis=0)
+        else:
+            loss, g, _, _ = diff_loss_grad(be, base, DiffParams.from_array(x), cam, target, None, opts)
         if not (np.isfinite(loss) and np.all(np.isfinite(g))):
             raise FloatingPointError(f"non-finite loss/gradient at step {k}: {loss} {g} (params {x})")
         g = np.where(free, g, 0.0)

@@ -223,8 +223,9 @@ __global__ __launch_bounds__(256) void diff_fwd_kernel(const DiffSet* __restrict
             l += double(g[k]) * double(acc[k]);
         }
     } else {
-        const float* tgt = b.target + 3u * pix;
-        const double w = b.weight ? double(b.weight[pix]) : 1.0;
+        const std::size_t tp = std::size_t(ev) * b.tstride + pix;
+        const float* tgt = b.target + 3u * tp;
+        const double w = b.weight ? double(b.weight[tp]) : 1.0;
         for (int k = 0; k < 3; ++k) {
             skp[k] = sky[k];
             b.hdr[3u * q + k] = float(acc[k]);
@@ -560,8 +561,8 @@ int SingContext::ensure_diff(std::size_t Pimg, std::size_t n_eval, std::size_t n
     A((void**)&diff.count2, n_eval * sizeof(std::uint32_t));
     A((void**)&diff.steps, P * sizeof(std::uint32_t));
     A((void**)&diff.partial, n_eval * sing::kReduceBlocks * sing::kReduceVals * sizeof(double));
-    A((void**)&d_target, Pimg * 3 * sizeof(float));
-    A((void**)&d_weight, Pimg * sizeof(float));
+    A((void**)&d_target, P * 3 * sizeof(float));  // one image per evaluation at most
+    A((void**)&d_weight, P * sizeof(float));
     A((void**)&d_out, n_eval * sing::kDiffOut * sizeof(double));
     // pinned, so the per-call result copy is an ordinary stream operation
     if (e == hipSuccess)
@@ -655,6 +656,9 @@ int diff_run(SingContext* c, const SingUniforms* base, std::uint32_t n, const Si
     if (C > 4096) return c->fail(SING_ERR_ARG, "%s: checkpoint_every too large", who);
     if (o.r_in_isco > 1u) return c->fail(SING_ERR_ARG, "%s: r_in_isco must be 0 or 1", who);
     if (backward && !target && !cot) return c->fail(SING_ERR_ARG, "%s: null %s", who, "target");
+    if (o.target_per_eval > 1u) return c->fail(SING_ERR_ARG, "%s: target_per_eval must be 0 or 1", who);
+    if (o.target_per_eval && (cot || !target))
+        return c->fail(SING_ERR_ARG, "%s: target_per_eval needs the target images", who);
     const std::size_t Pimg = std::size_t(c->width) * c->height;
     if (Pimg * n >= (std::size_t(1) << 31)) return c->fail(SING_ERR_ARG, "%s: too many pixels in one call", who);
     std::vector<sing::DiffSet> sets(n);
@@ -686,15 +690,17 @@ int diff_run(SingContext* c, const SingUniforms* base, std::uint32_t n, const Si
     if ((e = hipMemcpyAsync(c->d_dset, sets.data(), n * sizeof(sing::DiffSet), hipMemcpyHostToDevice, c->stream)) !=
         hipSuccess)
         return c->hip_fail(e, "hipMemcpyAsync(scene)");
+    const std::size_t n_img = o.target_per_eval ? n : 1u;  // target (and weight) images
     if (target || cot) {  // the cotangent image travels in the target buffer
-        if ((e = hipMemcpyAsync(c->d_target, cot ? cot : target, Pimg * 3 * sizeof(float), hipMemcpyHostToDevice,
-                                c->stream)) != hipSuccess)
+        if ((e = hipMemcpyAsync(c->d_target, cot ? cot : target, n_img * Pimg * 3 * sizeof(float),
+                                hipMemcpyHostToDevice, c->stream)) != hipSuccess)
             return c->hip_fail(e, "hipMemcpyAsync(target)");
     } else if ((e = hipMemsetAsync(c->d_target, 0, Pimg * 3 * sizeof(float), c->stream)) != hipSuccess) {
         return c->hip_fail(e, "hipMemsetAsync(target)");
     }
     if (weights && !cot &&
-        (e = hipMemcpyAsync(c->d_weight, weights, Pimg * sizeof(float), hipMemcpyHostToDevice, c->stream)) !=
+        (e = hipMemcpyAsync(c->d_weight, weights, n_img * Pimg * sizeof(float), hipMemcpyHostToDevice,
+                            c->stream)) !=
             hipSuccess)
         return c->hip_fail(e, "hipMemcpyAsync(weights)");
     if ((e = hipMemsetAsync(c->diff.count2, 0, n * sizeof(std::uint32_t), c->stream)) != hipSuccess)
@@ -707,6 +713,7 @@ int diff_run(SingContext* c, const SingUniforms* base, std::uint32_t n, const Si
     b.target = c->d_target;
     b.weight = (weights && !cot) ? c->d_weight : nullptr;
     b.cot = cot ? c->d_target : nullptr;
+    b.tstride = o.target_per_eval ? std::uint32_t(Pimg) : 0u;
     (void)hipEventRecord(c->events[0], c->stream);
     if ((e = sing::launch_diff(c->d_dset, b, o.precision, backward, c->d_out, c->stream, c->stream2,
                                c->events[1], c->join_events.data())) !=

@@ -43,8 +43,9 @@ struct DiffBuffers {
     double* loss;           // [P]
     double* grad;           // [P][5]
     double* scratch;        // [C+1][5][P] recomputed segment states
-    const float* target;    // [Pimg][3], shared by the evaluations
-    const float* weight;    // [Pimg] or null
+    const float* target;    // [Pimg][3] shared by the evaluations, or [n_eval][Pimg][3] (tstride = Pimg)
+    const float* weight;    // [Pimg], [n_eval][Pimg] or null
+    std::uint32_t tstride;  // pixels between evaluations' targets and weights (0: one shared image)
     // [Pimg][3] or null: a caller's dLoss/dRadiance (sing_diff_vjp); the loss
     // is then the pairing sum_p cot_p . I_p, whose gradient is the
     // vector-Jacobian product, and target/weight are not read

@@ -314,6 +314,33 @@ def test_batch_equals_separate_evaluations(be, base, target, prec):
     assert np.array_equal(g_batch, g_single) and np.array_equal(m_batch, m_single)
 
 
+def test_batch_with_one_target_per_point_equals_separate_evaluations(be, base, orc):
+    """Several views of one scene: one camera and one observed image (and weight map)
+    per point (target_per_eval); each loss and gradient equals the lone call with that
+    view's camera, image and weights, bit for bit."""
+    cams = [cam(), sa.DiffCamera(45.0, 0.3, 50.0), sa.DiffCamera(55.0, -0.2, 45.0)]
+    targets = []
+    for c in cams:
+        t, _ = sa.diff_forward(be, base, sa.DiffParams.from_array(TARGET), c, opts())
+        targets.append(t)
+    targets = np.stack(targets)
+    rng = np.random.default_rng(7)
+    weights = rng.uniform(0.5, 1.5, (len(cams), H, W)).astype(np.float32)
+    xs = [START, START, START]
+    print(f"batch of {len(xs)} views with their own targets", flush=True)
+    for w in (None, weights):
+        loss, grad, hdr, st = sa.diff_loss_grad_batch(be, base, xs, cams, targets, w, opts(), want_hdr=True)
+        for k in range(len(cams)):
+            l1, g1, h1, _ = sa.diff_loss_grad(be, base, sa.DiffParams.from_array(xs[k]), cams[k], targets[k],
+                                              None if w is None else w[k], opts())
+            assert loss[k] == l1 and np.array_equal(grad[k], g1), (k, loss[k], l1)
+            assert np.array_equal(hdr[k], h1)
+    # the views' images differ, so do their losses
+    assert len(set(np.round(loss, 6))) == len(cams)
+    with pytest.raises(ValueError):
+        sa.diff_loss_grad_batch(be, base, xs, cams, targets[:2], None, opts())
+
+
 def test_batch_argument_errors(be, base, target):
     with pytest.raises(ValueError):
         sa.diff_loss_grad_batch(be, base, [], cam(), target)
@@ -442,3 +469,29 @@ def test_adam_recovers_parameters(be512, workload):
     assert abs(x[3] - 20.0) <= 0.25
     assert abs(x[4] - 1.0) <= 0.01
     assert loss <= 2e-3 * hist[0][0]
+
+
+def test_adam_fits_one_scene_to_three_views():
+    """Multi-view inverse render: three cameras (distance, azimuth, field of view
+    differ), each with its own image of the target scene, fitted jointly (loss and
+    gradient summed over the views, one batched call per step), 256x256, mixed,
+    r_in = ISCO(a), from the same start: the same parameter bars as the one-view fit."""
+    n = 256
+    b = sa.HipBackend(n, n)
+    try:
+        base = sa.diff_base_uniforms(n, n)
+        cams = [cam(), sa.DiffCamera(45.0, 0.3, 50.0), sa.DiffCamera(60.0, -0.5, 40.0)]
+        o = opts(sa.PREC_MIXED, isco=1)
+        tgts = np.stack([sa.diff_forward(b, base, sa.DiffParams.from_array(TARGET), c, opts(sa.PREC_FP64, isco=1))[0]
+                         for c in cams])
+        print(f"adam {n}x{n} mixed, 3 views, 200 steps", flush=True)
+        x, hist = sa.adam_fit(b, base, START, tgts, cams, steps=200, opts=o)
+        losses, _, _, _ = sa.diff_loss_grad_batch(b, base, [x] * 3, cams, tgts, None, o)
+        print("last", x, "loss", hist[0][0], "->", losses.sum(), "per view", losses, flush=True)
+        assert abs(x[0] - 0.7) <= 0.01
+        assert abs(np.degrees(x[1]) - 70.0) <= 0.5
+        assert abs(x[3] - 20.0) <= 0.25
+        assert abs(x[4] - 1.0) <= 0.01
+        assert losses.sum() <= 2e-3 * hist[0][0]
+    finally:
+        b.shutdown()

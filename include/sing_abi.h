@@ -196,7 +196,9 @@ int sing_render(SingContext* ctx, const SingUniforms* u, const SingRenderOpts* o
  * free until sing_wait.  Any other call on the context (capture, render,
  * resize, post pass, batch, differentiable render, diagnostics, destroy)
  * first completes a pending frame, so its outputs are never read half-done;
- * the frame's stats stay available to sing_wait until the next submission. */
+ * the frame's stats stay available to sing_wait until the next submission.
+ * A capture or post pass that completes a frame which failed returns that
+ * frame's error (sing_wait returns it too). */
 int sing_render_async(SingContext* ctx, const SingUniforms* u, const SingRenderOpts* opts);
 /* Waits for the frame submitted by sing_render_async (if it has not completed
  * yet) and returns its status and stats (optional), as sing_render would

@@ -502,7 +502,8 @@ int sing_render(SingContext* c, const SingUniforms* u, const SingRenderOpts* opt
 static int capture(SingContext* c, std::uint32_t bit, const void* src, void* dst, std::size_t need, std::size_t bytes,
                    const char* what) {
     if (!c) return SING_ERR_ARG;
-    c->complete_pending();
+    // a pending frame that fails reports its own error here (and to sing_wait)
+    if (const int rc = c->complete_pending(); rc != SING_OK) return rc;
     if (!dst) return c->fail(SING_ERR_ARG, "capture %s: null destination", what);
     if (bytes < need) return c->fail(SING_ERR_ARG, "capture %s: buffer %zu bytes < %zu", what, bytes, need);
     if (!(c->outputs_valid & bit)) return c->fail(SING_ERR_STATE, "capture %s: not rendered in the last frame", what);
@@ -535,7 +536,7 @@ int sing_capture_hdr16(SingContext* c, uint16_t* dst, size_t bytes) {
 
 int sing_post_bloom(SingContext* c, const SingUniforms* u, uint32_t repeat, SingRenderStats* stats) {
     if (!c) return SING_ERR_ARG;
-    c->complete_pending();
+    if (const int rc = c->complete_pending(); rc != SING_OK) return rc;
     if (!u) return c->fail(SING_ERR_ARG, "sing_post_bloom: null uniforms");
     if (repeat < 1u || repeat > 1000u) return c->fail(SING_ERR_ARG, "sing_post_bloom: repeat must be in [1, 1000]");
     if (c->width < 2 || c->height < 2) return c->fail(SING_ERR_ARG, "sing_post_bloom: bloom needs at least 2x2 pixels");

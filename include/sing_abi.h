@@ -131,9 +131,11 @@ typedef struct SingRenderOpts {
     float adapt_tol;      /* > 0: error-controlled RK4 (step doubling) with this per-step tolerance */
     float band_width;     /* mixed precision: photon-orbit margin below which a ray is traced in FP64 (0 = 0.03) */
     float peak_T_scale;   /* multiplies disc_peak_T; 0 is read as 1 */
-    uint32_t step_batch;  /* RK4 steps between lane-refill checks (0 = tuned default, 32; rounded up
-                             to a multiple of 8); the mixed form's FP64 list pass uses a quarter of it
-                             (at least 8, a multiple of 8) */
+    uint32_t step_batch;  /* RK4 steps between lane-refill checks, rounded up to a multiple of 8; the
+                             mixed form's FP64 list pass uses an eighth of it (at least 8, a multiple
+                             of 8).  0 = tuned defaults: 64 for the FP32 passes and the
+                             error-controlled FP64 pass, 32 for the fixed-step FP64 pass, 8 for the
+                             FP64 list */
     uint32_t waves_per_cu;/* persistent grid size in waves per CU (0 = tuned default) */
     uint32_t sampler;     /* SING_SAMPLER_* */
     uint32_t refine;      /* SING_REFINE_* mask; Kerr with the fixed step (adapt_tol = 0) only */

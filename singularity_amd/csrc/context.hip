@@ -150,13 +150,13 @@ sing::LaunchShape SingContext::launch_shape(const SingRenderOpts& o) const { ret
 
 sing::LaunchShape sing::launch_shape_for(int num_cu, const SingRenderOpts& o) {
     sing::LaunchShape sh{};
-    // defaults from a launch-shape sweep of the 4K Kerr frame: 32 steps
-    // between refills and 24 waves per CU (the trace kernels' occupancy
-    // limit) beat 8 steps and 16 waves by 12 %.  A step batch is a multiple of
-    // 8 (rounded up): the step loop's sin/cos phase relies on it.
+    // 24 waves per CU (the trace kernels' occupancy limit) from a launch-shape
+    // sweep of the 4K Kerr frame; step batch 0 = each pass's own default
+    // (launch_trace_chunk).  A step batch is a multiple of 8 (rounded up): the
+    // step loop's sin/cos phase relies on it.
     sh.step_batch = o.step_batch ? std::uint32_t(std::min<std::uint64_t>((std::uint64_t(o.step_batch) + 7u) & ~7ull,
                                                                          0xFFFFFFF8ull))
-                                 : 32u;
+                                 : 0u;
     const std::uint32_t wpc = o.waves_per_cu ? o.waves_per_cu : 24u;
     sh.blocks = std::uint32_t(num_cu) * ((wpc + 3u) / 4u);
     sh.blocks64 = std::uint32_t(num_cu) * ((std::max(wpc / 2u, 4u) + 3u) / 4u);

@@ -359,14 +359,19 @@ hipError_t launch_trace_chunk(const SceneConst<float>& sf, const SceneConst<doub
         if (ev) (void)hipEventRecord(ev[i], stream);
     };
     const dim3 block(256);
-    const std::uint32_t sb = shape.step_batch ? shape.step_batch : 32u;
-    // The mixed form's FP64 list mixes short captured rays (~70 steps) with
-    // long near-polar ones (~600): refilling every quarter of the step batch
-    // (at least 8, a multiple of 8) wastes fewer idle lane-steps there (the
-    // list's pass 3-7 % faster; the full-image FP64 pass is best at 32).
-    const std::uint32_t sb_list = std::max(8u, (sb / 4u) & ~7u);
     const bool kerr = sf.metric == SING_METRIC_KERR;
     const bool adapt = kerr && sf.tol > 0.0f;
+    // Step batches (RK4 steps, or error-controlled attempts, between lane
+    // refills) from launch-shape sweeps on MI355X: the FP32 passes run best
+    // at 64 (4K Kerr frame 1 % and the error-controlled 1080p frame 5 % faster
+    // than at 32), the error-controlled full FP64 pass at 64 (2 %), the
+    // fixed-step full FP64 pass at 32.  The mixed form's FP64 list mixes short
+    // captured rays (~70 steps) with long near-polar ones (~600): refilling
+    // every eighth of the step batch (at least 8, a multiple of 8) wastes
+    // fewer idle lane-steps there (8 beats 16 and 32 by 2-7 %).
+    const std::uint32_t sb = shape.step_batch ? shape.step_batch : 64u;
+    const std::uint32_t sb64 = shape.step_batch ? shape.step_batch : (adapt ? 64u : 32u);
+    const std::uint32_t sb_list = std::max(8u, (sb / 8u) & ~7u);
 #define SING_TRACE(P, G, SB)                                                                              \
     do {                                                                                                  \
         if (adapt)                                                                                        \
@@ -398,7 +403,7 @@ hipError_t launch_trace_chunk(const SceneConst<float>& sf, const SceneConst<doub
         mark(0);
         mark(1);
         mark(2);
-        SING_TRACE(2, shape.blocks64, sb);
+        SING_TRACE(2, shape.blocks64, sb64);
         mark(3);
     } else {
         SING_SETUP(1);

@@ -284,11 +284,17 @@ def c4_leg(seed: int = 20261203, reps: int = 2) -> dict:
         _, st = sa.render_batch(be, scenes, opts)
         walls.append(time.perf_counter() - t0)
         times.append(st.kernel_ms)
-    seq = 0.0
-    for u in scenes:
-        be.resize(u.width, u.height)
-        seq += be.render(u, sa.RenderOpts(precision=sa.PREC_MIXED, outputs=sa.OUT_RGBA8,
-                                          step_batch=st.step_batch, waves_per_cu=st.waves_per_cu)).kernel_ms
+    # the scenes one after another, at the single-scene default launch shape
+    # and at the batch's tuned one; the faster total is the comparison
+    seq_shapes = {"default": (0, 0), "batch_tuned": (st.step_batch, st.waves_per_cu)}
+    seq_ms = {}
+    for name, (sb, wpc) in seq_shapes.items():
+        seq_ms[name] = 0.0
+        for u in scenes:
+            be.resize(u.width, u.height)
+            seq_ms[name] += be.render(u, sa.RenderOpts(precision=sa.PREC_MIXED, outputs=sa.OUT_RGBA8,
+                                                       step_batch=sb, waves_per_cu=wpc)).kernel_ms
+    seq = min(seq_ms.values())
     be.shutdown()
     kern = float(np.mean(times))
     res = {
@@ -301,6 +307,7 @@ def c4_leg(seed: int = 20261203, reps: int = 2) -> dict:
         "waves_per_cu": int(st.waves_per_cu), "rays_per_refill": int(st.rays_per_refill),
         "queue_chunk": int(st.queue_chunk),
         "sequential_single_scene_ms": round(seq, 1), "batch_speedup_vs_sequential": round(seq / kern, 3),
+        "sequential_ms_by_shape": {k: round(v, 1) for k, v in seq_ms.items()},
     }
     log(f"c4: {res['mrays_s']} Mrays/s, batch {kern:.0f} ms vs sequential {seq:.0f} ms")
     return res

@@ -39,15 +39,16 @@ TRACE_HBM_BYTES_PER_RAY = (522196.0 * 1024 + 376424.0 * 1024 * 2.0) / 33177600.0
 # VALU instructions the Kerr trace kernels issue per RK4 step and the lanes a
 # wave64 instruction has active, from SQ counters of one 1080p x 16 spp mixed
 # frame (one chunk): SQ_INSTS_VALU x (SQ_THREAD_CYCLES_VALU / SQ_ACTIVE_INST_VALU)
-# / RK4 steps of the pass.  FP32 pass (trace_kernel<1, 1>): 34.13 G instructions,
-# 62.28 lanes, 7.704 G steps; FP64 list pass (trace_kernel<1, 3>, its list
-# taken longest first): 1.285 G, 55.78 lanes, 0.2054 G steps (47.65 lanes and
+# / RK4 steps of the pass.  FP32 pass (trace_kernel<1, 1>, 64 steps between
+# refills): 34.02 G instructions, 62.41 lanes, 7.704 G steps (34.13 G and 62.28
+# lanes at 32 steps); FP64 list pass (trace_kernel<1, 3>, its list taken
+# longest first): 1.285 G, 55.78 lanes, 0.2054 G steps (47.65 lanes and
 # 1.507 G instructions in record order).  A launch's issue rate is then steps x
 # instructions per step / lanes / (launch time x 256 CUs x 2.4 GHz), against
 # what the issue-rate calibration reaches: 1.9 wave64 FP32 instructions per CU
 # per clock with all-vector operands (dual issue), 0.98 FP64 FMAs.
 KERR_STEP_VALU_INSTS = 276
-KERR_STEP_LANES = 62.28
+KERR_STEP_LANES = 62.41
 KERR_STEP_VALU_INSTS_FP64 = 350
 KERR_STEP_LANES_FP64 = 55.78
 CU_COUNT = 256

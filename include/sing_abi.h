@@ -132,7 +132,7 @@ typedef struct SingRenderOpts {
     float band_width;     /* mixed precision: photon-orbit margin below which a ray is traced in FP64 (0 = 0.03) */
     float peak_T_scale;   /* multiplies disc_peak_T; 0 is read as 1 */
     uint32_t step_batch;  /* RK4 steps between lane-refill checks, rounded up to a multiple of 8; the
-                             mixed form's FP64 list pass uses an eighth of it (at least 8, a multiple
+                             mixed form's FP64 list pass uses a quarter of it (at least 8, a multiple
                              of 8).  0 = tuned defaults: 64 for the FP32 passes and the
                              error-controlled FP64 pass, 32 for the fixed-step FP64 pass, 8 for the
                              FP64 list */

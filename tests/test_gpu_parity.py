@@ -139,6 +139,22 @@ def test_frames_do_not_depend_on_launch_shape(backends, prec):
         assert f.stats.steps == ref.stats.steps
 
 
+@pytest.mark.parametrize("prec", [sa.PREC_FP32, sa.PREC_FP64, sa.PREC_MIXED])
+def test_error_controlled_frames_do_not_depend_on_launch_shape(backends, prec):
+    """Error-controlled stepping: a ray's attempts depend only on its own state,
+    so the step batch (attempts between refills; the default 0 = 64, the FP64
+    list 8) and the grid size leave the frame bit-identical in every precision."""
+    u = config2(192, 108, 2)
+    kw = dict(adapt_tol=1e-6, turb_seed=20261203)
+    ref = render(backends, u, prec, step_batch=32, waves_per_cu=24, **kw)
+    for sb, wpc in ((8, 16), (0, 0), (128, 32)):
+        print(f"error-controlled prec={prec} step_batch={sb} waves_per_cu={wpc}", flush=True)
+        f = render(backends, u, prec, step_batch=sb, waves_per_cu=wpc, **kw)
+        for k in ("rgba", "hdr", "cls", "steps"):
+            np.testing.assert_array_equal(getattr(f, k), getattr(ref, k), err_msg=f"{k} sb={sb} wpc={wpc}")
+        assert (f.stats.steps, f.stats.rejected) == (ref.stats.steps, ref.stats.rejected)
+
+
 def test_many_samples_match_oracle_small_image(backends, orc):
     """spp up to the 1024 cap: Halton jitter, per-sample ACES mean, chunked accumulation."""
     u = scene_uniforms(spin=0.94, r_in=1.7, r_out=30.0, width=8, height=6, spp=1024)

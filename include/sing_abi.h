@@ -134,8 +134,8 @@ typedef struct SingRenderOpts {
     uint32_t step_batch;  /* RK4 steps between lane-refill checks, rounded up to a multiple of 8; the
                              mixed form's FP64 list pass uses a quarter of it (at least 8, a multiple
                              of 8).  0 = tuned defaults: 64 for the FP32 passes and the
-                             error-controlled FP64 pass, 32 for the fixed-step FP64 pass, 8 for the
-                             FP64 list */
+                             error-controlled FP64 pass, 32 for the fixed-step FP64 pass; the FP64
+                             list 16 (8 with error control) */
     uint32_t waves_per_cu;/* persistent grid size in waves per CU (0 = tuned default) */
     uint32_t sampler;     /* SING_SAMPLER_* */
     uint32_t refine;      /* SING_REFINE_* mask; Kerr with the fixed step (adapt_tol = 0) only */
@@ -435,9 +435,9 @@ typedef struct SingBatchOpts {
     uint32_t use_graph;     /* 1: record the batch's launches once as a hipGraph and replay it for batches
                                of the same layout (sizes, sample counts, metrics, options) */
     uint32_t autotune;      /* 1: time candidates for waves per CU, step batch, rays per refill and queue
-                               chunk on a 1/16 sample of this batch and keep the fastest for this (metric,
+                               chunk on a 1/8 sample of this batch and keep the fastest for this (metric,
                                precision, stepping) in the context */
-    uint32_t step_batch;    /* RK4 steps between lane refills (0 = tuned or 32; rounded up to a
+    uint32_t step_batch;    /* RK4 steps between lane refills (0 = tuned or 64; rounded up to a
                                multiple of 8) */
     uint32_t waves_per_cu;  /* persistent waves per CU (0 = tuned or 24) */
     float adapt_tol;        /* > 0: error-controlled stepping (applies to the Kerr scenes) */
@@ -445,9 +445,10 @@ typedef struct SingBatchOpts {
     uint32_t record_budget_mib; /* device memory for one group's termination records (0 = 32 GiB); scenes are
                                    split into groups that fit, traced one group after another */
     uint32_t rays_per_refill;   /* a wave claims new rays once at least this many of its lanes are idle (or
-                                   none is live); 0 = tuned or 1, at most 64 */
+                                   none is live); 0 = tuned or 32, at most 64 */
     uint32_t queue_chunk;       /* work items a wave takes from its scene's queue per atomic and hands to its
-                                   lanes as they free up; 1 = one per idle lane; 0 = tuned or 1 */
+                                   lanes as they free up; 1 = one per idle lane; 0 = tuned, else 256
+                                   (64 below 1024 rays per wave of the grid, 1 below 256) */
     uint32_t reserved[2];
 } SingBatchOpts;
 

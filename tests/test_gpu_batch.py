@@ -119,6 +119,27 @@ def test_launch_shapes_and_groups_do_not_change_outputs(be):
             assert st.groups > 1
 
 
+def test_untuned_defaults_follow_the_work_per_wave():
+    """Without the tuner or explicit knobs: 64 steps between refills, 24 waves
+    per CU, a refill once 32 lanes are idle, and private runs of 256 items when
+    the grid's waves get at least 1024 rays each (64 from 256, else 1); the
+    frames equal those of an explicit item-by-item shape."""
+    b = sa.HipBackend(64, 64)
+    try:
+        big = [scene_uniforms(spin=0.94, r_in=1.7, r_out=30.0, width=512, height=512, spp=16),
+               scene_uniforms(spin=0.5, r_in=4.0, r_out=20.0, width=640, height=360, spp=16)]
+        small = [scene_uniforms(spin=0.94, r_in=1.7, r_out=30.0, width=128, height=128, spp=1)]
+        for name, scenes, qc in (("big", big, 256), ("small", small, 1)):
+            print(f"untuned defaults: {name}", flush=True)
+            frames, st = sa.render_batch(b, scenes, sa.BatchOpts(precision=sa.PREC_MIXED), hdr=True, cls=True)
+            assert (st.step_batch, st.waves_per_cu, st.rays_per_refill, st.queue_chunk) == (64, 24, 32, qc)
+            ref, _ = sa.render_batch(b, scenes, sa.BatchOpts(precision=sa.PREC_MIXED, step_batch=32, waves_per_cu=24,
+                                                             rays_per_refill=1, queue_chunk=1), hdr=True, cls=True)
+            check_equal(frames, ref)
+    finally:
+        b.shutdown()
+
+
 def test_graph_replay_matches_eager(be):
     """The recorded graph is replayed several times with eager work and host
     synchronisation in between; every replay must equal the eager batch, also

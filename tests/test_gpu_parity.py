@@ -128,7 +128,7 @@ def test_frames_do_not_depend_on_launch_shape(backends, prec):
     evaluation of the fixed step comes every 8th step of the ray, not at lane
     refills (step batches are rounded up to a multiple of 8, here 5 -> 8, so
     the live lanes of a wave share that phase), and step batch and grid size
-    leave the frame bit-identical (0, 0: the defaults, 64 steps and 8 for
+    leave the frame bit-identical (0, 0: the defaults, 64 steps and 16 for
     the FP64 list)."""
     u = config3(192, 108, 3)
     ref = render(backends, u, prec, step_batch=32, waves_per_cu=24)

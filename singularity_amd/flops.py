@@ -42,7 +42,8 @@ TRACE_HBM_BYTES_PER_RAY = (522196.0 * 1024 + 376424.0 * 1024 * 2.0) / 33177600.0
 # / RK4 steps of the pass.  FP32 pass (trace_kernel<1, 1>, 64 steps between
 # refills): 34.02 G instructions, 62.41 lanes, 7.704 G steps (34.13 G and 62.28
 # lanes at 32 steps); FP64 list pass (trace_kernel<1, 3>, its list taken
-# longest first): 1.285 G, 55.78 lanes, 0.2054 G steps (47.65 lanes and
+# longest first, a refill every 16 steps): 1.320 G, 54.14 lanes, 0.2054 G
+# steps (55.78 lanes and 1.285 G with a refill every 8; 47.65 lanes and
 # 1.507 G instructions in record order).  A launch's issue rate is then steps x
 # instructions per step / lanes / (launch time x 256 CUs x 2.4 GHz), against
 # what the issue-rate calibration reaches: 1.9 wave64 FP32 instructions per CU
@@ -50,7 +51,7 @@ TRACE_HBM_BYTES_PER_RAY = (522196.0 * 1024 + 376424.0 * 1024 * 2.0) / 33177600.0
 KERR_STEP_VALU_INSTS = 276
 KERR_STEP_LANES = 62.41
 KERR_STEP_VALU_INSTS_FP64 = 350
-KERR_STEP_LANES_FP64 = 55.78
+KERR_STEP_LANES_FP64 = 54.14
 CU_COUNT = 256
 SHADER_CLOCK_HZ = 2.4e9
 VALU_ISSUE_PEAK_PER_CU_CLK = 1.9
